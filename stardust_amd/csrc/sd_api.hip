@@ -1,0 +1,1404 @@
+// sd_api.cpp -- host side of the C ABI (include/stardust/sd_api.h).
+//
+// Plays the role of Sim::Impl in src/sim/Sim.cu: owns every device buffer of
+// one simulation, sequences the kernels of a step on the handle's stream and
+// keeps the per-step scalars (dt, kick, momentum, time) on the device so a
+// step has no host round trip. Error handling follows the reference's sticky
+// g_failed / g_failMsg (Sim.cu:52-75).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stardust/sd_api.h"
+#include "sd_internal.h"
+
+using namespace sd;
+
+// ---------------------------------------------------------------------------
+// sticky failure state (Sim.cu:52-75)
+// ---------------------------------------------------------------------------
+static bool g_failed = false;
+static char g_failMsg[256] = {0};
+
+static void mark_failure(const char* fmt, ...) {
+    if (g_failed) return;
+    g_failed = true;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_failMsg, sizeof(g_failMsg), fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "[stardust_amd] failure: %s\n", g_failMsg);
+}
+
+#define HK(x)                                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            mark_failure("%s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__);         \
+        }                                                                                  \
+    } while (0)
+
+static int arg_error(const char* what) {
+    // argument errors are reported but are not sticky
+    snprintf(g_failMsg, sizeof(g_failMsg), "%s", what);
+    fprintf(stderr, "[stardust_amd] invalid argument: %s\n", what);
+    return SD_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// the handle
+// ---------------------------------------------------------------------------
+struct KernelTimer {
+    std::string name;
+    std::vector<hipEvent_t> pool;  // start/stop pairs
+    int used = 0;
+    double totalMs = 0.0;
+    long long launches = 0;
+};
+
+struct sd_sim {
+    sd_config cfg{};
+    sd_modes modes{};
+    bool ready = false;
+    int G = 0, S = 0, P = 0, periodic = 0;
+    int allocN = 0;  // slots (mass per particle = gravity / allocN)
+    int active = 0;  // slots in use (prefix), for state files
+    float fixedDt = 0.f;
+    hipStream_t st = nullptr;
+
+    // particles
+    float4 *pos = nullptr, *vel = nullptr, *posTmp = nullptr, *velTmp = nullptr;
+    uint32_t *keysA = nullptr, *keysB = nullptr;
+    int32_t *idxA = nullptr, *idxB = nullptr;
+    uint32_t *hist = nullptr, *chunk = nullptr;
+    SortBuffers sb{};
+    bool keysSorted = false;
+
+    // mesh
+    long long* rhoFix = nullptr;
+    float* rhoF = nullptr;
+    float2 *A = nullptr, *B = nullptr;
+    float* green = nullptr;
+    float* pot = nullptr;
+    float4* accG = nullptr;
+    float2* tw = nullptr;
+    float4* accContact = nullptr;
+    int32_t *cellStart = nullptr, *cellEnd = nullptr;
+    long long cellTableCells = 0;
+    MeshPlan plan{};
+
+    // reductions
+    ReducePartial* partials = nullptr;
+    int nparts = 0;
+    StepScalars* sc = nullptr;
+    StepScalars* scHost = nullptr;  // pinned mirror for reads
+
+    // state
+    bool rhoValid = false;   // rhoFix holds the deposit of the current positions
+    bool meshValid = false;  // accG matches the current positions
+    bool primed = false;     // sc->dt / sc->kick are set for the next integrate
+    long long stepCount = 0;
+
+    // timings
+    hipEvent_t evA = nullptr, ev1 = nullptr, evB = nullptr;
+    sd_timings tm{};
+    KernelTimer timer;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+static float light_speed_sq(float lengthScale, float timeUnitScale) {
+    // lightSpeedFor, Sim.h:131-148: c = (1e7 T) / (1e5 L)
+    const double L = 1.0e5 * (double)(lengthScale > 1e-6f ? lengthScale : 1e-6f);
+    const double T = 1.0e7 * (double)(timeUnitScale > 1e-6f ? timeUnitScale : 1e-6f);
+    const float c = (float)(T / (L > 1e-30 ? L : 1e-30));
+    return c * c;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+    if (p) {
+        (void)hipFree((void*)p);
+        p = nullptr;
+    }
+}
+
+template <typename T>
+static bool dalloc(T*& p, size_t count) {
+    dfree(p);
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        mark_failure("hipMalloc(%zu bytes): %s", sizeof(T) * count, hipGetErrorString(e));
+        p = nullptr;
+        return false;
+    }
+    return true;
+}
+
+static void timer_begin(sd_sim* s, const char* name) {
+    KernelTimer& t = s->timer;
+    if (t.name.empty() || t.name != name) return;
+    if (t.used + 2 > (int)t.pool.size()) {
+        // harvest what is pending to recycle events
+        HK(hipStreamSynchronize(s->st));
+        for (int k = 0; k + 1 < t.used; k += 2) {
+            float ms = 0.f;
+            HK(hipEventElapsedTime(&ms, t.pool[k], t.pool[k + 1]));
+            t.totalMs += ms;
+            t.launches++;
+        }
+        t.used = 0;
+    }
+    HK(hipEventRecord(t.pool[t.used], s->st));
+}
+static void timer_end(sd_sim* s, const char* name) {
+    KernelTimer& t = s->timer;
+    if (t.name.empty() || t.name != name) return;
+    HK(hipEventRecord(t.pool[t.used + 1], s->st));
+    t.used += 2;
+}
+static void timer_harvest(sd_sim* s) {
+    KernelTimer& t = s->timer;
+    if (t.used == 0) return;
+    HK(hipStreamSynchronize(s->st));
+    for (int k = 0; k + 1 < t.used; k += 2) {
+        float ms = 0.f;
+        HK(hipEventElapsedTime(&ms, t.pool[k], t.pool[k + 1]));
+        t.totalMs += ms;
+        t.launches++;
+    }
+    t.used = 0;
+}
+
+#define TIMED(s, name, stmt)     \
+    do {                         \
+        timer_begin(s, name);    \
+        stmt;                    \
+        timer_end(s, name);      \
+    } while (0)
+
+static void free_buffers(sd_sim* s) {
+    dfree(s->pos); dfree(s->vel); dfree(s->posTmp); dfree(s->velTmp);
+    dfree(s->keysA); dfree(s->keysB); dfree(s->idxA); dfree(s->idxB);
+    dfree(s->hist); dfree(s->chunk);
+    dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
+    dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
+    dfree(s->cellStart); dfree(s->cellEnd); dfree(s->partials); dfree(s->sc);
+    s->cellTableCells = 0;
+    if (s->scHost) {
+        (void)hipHostFree(s->scHost);
+        s->scHost = nullptr;
+    }
+    s->ready = false;
+}
+
+static int valid_grid(int G, int boundary, int exact) {
+    if (!exact) return 1;
+    if (boundary == SD_BOUNDARY_PERIODIC) return is_pow2(G) && fft_length_supported(G);
+    return fft_length_supported(2 * G);
+}
+
+// Reference grid policy (Sim.cu:4081-4086, 4158), or exact sizes.
+static int choose_grid(const sd_config* c, const sd_modes* m) {
+    if (m->exactGrid) return c->gridSize;
+    const int cap = sd_max_grid_size(c->boundary);
+    int g = c->gridSize > 0 ? c->gridSize : 64;
+    if (g > cap) g = cap;
+    if (g < 16) g = 16;
+    int p = 16;
+    while (p * 2 <= g) p *= 2;
+    return p;
+}
+
+static int check_config(const sd_config* c) {
+    if (!c) return arg_error("null config");
+    if (c->boundary != SD_BOUNDARY_ISOLATED && c->boundary != SD_BOUNDARY_PERIODIC)
+        return arg_error("boundary must be isolated (0) or periodic (1)");
+    if (c->pressureEnabled || c->coolingEnabled || c->starFormationEnabled || c->blackHoleEnabled ||
+        c->strongForceEnabled || c->emForceEnabled || c->weakForceEnabled || c->collapseEnabled)
+        return arg_error("only the plain-PM path is provided: pressure, cooling, star formation, "
+                         "black holes, strong/EM/weak forces and collapse must be off");
+    if (c->preset != SD_PRESET_EMPTY)
+        return arg_error("preset Filament is not provided; use SD_PRESET_EMPTY and sd_set_particles");
+    if (c->particleCount < 0) return arg_error("particleCount < 0");
+    return SD_OK;
+}
+
+static void upload_twiddles(sd_sim* s) {
+    std::vector<float2> h(s->S);
+    for (int m = 0; m < s->S; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)s->S;
+        h[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    HK(hipMemcpy(s->tw, h.data(), sizeof(float2) * s->S, hipMemcpyHostToDevice));
+}
+
+static void reset_scalars(sd_sim* s) {
+    StepScalars z{};
+    z.kdkOpen = 1;
+    HK(hipMemcpyAsync(s->sc, &z, sizeof(z), hipMemcpyHostToDevice, s->st));
+    HK(hipStreamSynchronize(s->st));
+    s->primed = false;
+    s->stepCount = 0;
+}
+
+static void build_green(sd_sim* s) {
+    if (s->periodic) return;
+    const int S = s->S;
+    float2* ta = nullptr;
+    float2* tb = nullptr;
+    const size_t n = (size_t)S * S * s->P;
+    if (!dalloc(ta, n) || !dalloc(tb, n)) {
+        dfree(ta);
+        dfree(tb);
+        return;
+    }
+    // kGreen uses soft = max(softeningCells, 0.5), Sim.cu:3572
+    const float soft = fmaxf(s->cfg.softeningCells, 0.5f);
+    mesh_build_green(s->plan, soft, ta, tb, s->green, s->st);
+    HK(hipGetLastError());
+    HK(hipStreamSynchronize(s->st));
+    dfree(ta);
+    dfree(tb);
+}
+
+static int allocate(sd_sim* s) {
+    free_buffers(s);
+    const int G = s->G, S = s->S, P = s->P;
+    const size_t N = (size_t)(s->allocN > 0 ? s->allocN : 1);
+    const size_t cells = (size_t)G * G * G;
+    bool ok = dalloc(s->pos, N) && dalloc(s->vel, N) && dalloc(s->posTmp, N) && dalloc(s->velTmp, N) &&
+              dalloc(s->keysA, N) && dalloc(s->keysB, N) && dalloc(s->idxA, N) && dalloc(s->idxB, N);
+    const int ntiles = sort_tiles_for((int)N);
+    ok = ok && dalloc(s->hist, (size_t)256 * ntiles) && dalloc(s->chunk, (size_t)sort_chunks_for((int)N) + 1);
+    ok = ok && dalloc(s->rhoFix, cells) && dalloc(s->A, (size_t)G * G * P) && dalloc(s->B, (size_t)G * S * P) &&
+         dalloc(s->pot, cells) && dalloc(s->accG, cells) && dalloc(s->tw, (size_t)S);
+    if (!s->periodic) ok = ok && dalloc(s->green, (size_t)(S / 2 + 1) * (S / 2 + 1) * P);
+    s->nparts = integrate_blocks((int)N);
+    ok = ok && dalloc(s->partials, (size_t)s->nparts) && dalloc(s->sc, 1);
+    if (ok && hipHostMalloc((void**)&s->scHost, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess) {
+        mark_failure("hipHostMalloc failed");
+        ok = false;
+    }
+    if (!ok) {
+        free_buffers(s);
+        return SD_ERR_DEVICE;
+    }
+    s->sb.keys = s->keysA;
+    s->sb.idx = s->idxA;
+    s->sb.keysAlt = s->keysB;
+    s->sb.idxAlt = s->idxB;
+    s->sb.hist = s->hist;
+    s->sb.chunk = s->chunk;
+    s->sb.maxTiles = ntiles;
+    upload_twiddles(s);
+    s->plan.G = G;
+    s->plan.S = S;
+    s->plan.P = P;
+    s->plan.periodic = s->periodic;
+    s->plan.tw = s->tw;
+    // kPoissonPeriodic scale (Sim.cu:3594): norm / (4 pi^2) with norm = 1/S^3
+    const float norm = 1.0f / (float)((double)S * S * S);
+    s->plan.perScale = -(norm / (4.0f * 3.14159265f * 3.14159265f));
+    {
+        const float sg = 6.2831853f * s->cfg.softeningCells / (float)G;
+        s->plan.perSoft = s->cfg.softeningCells > 0.f ? sg * sg : 0.f;
+    }
+    launch_fill_i64(s->rhoFix, (long long)cells, s->st);
+    build_green(s);
+    if (g_failed) return SD_ERR_DEVICE;
+    // all slots inactive
+    std::vector<float4> hp(N, make_float4(-1.f, -1.f, -1.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
+    HK(hipMemcpy(s->pos, hp.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    HK(hipMemcpy(s->vel, hv.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    s->active = 0;
+    s->rhoValid = false;
+    s->meshValid = false;
+    s->keysSorted = false;
+    reset_scalars(s);
+    s->ready = !g_failed;
+    return g_failed ? SD_ERR_DEVICE : SD_OK;
+}
+
+// Cell-table size: G^3 for row-major keys; Morton keys of a 3*2^k mesh run
+// up to the next power of two cubed.
+static long long cell_table_size(const sd_sim* s) {
+    long long g = s->G;
+    if (s->modes.keyMode == SD_KEY_MORTON) {
+        g = 1;
+        while (g < s->G) g <<= 1;
+    }
+    return g * g * g;
+}
+
+static bool ensure_contact_buffers(sd_sim* s) {
+    const size_t cells = (size_t)cell_table_size(s);
+    if (s->cellTableCells < (long long)cells) {
+        dfree(s->cellStart);
+        dfree(s->cellEnd);
+    }
+    s->cellTableCells = (long long)cells;
+    if (!s->accContact && !dalloc(s->accContact, (size_t)s->allocN)) return false;
+    if (!s->cellStart && !dalloc(s->cellStart, cells)) return false;
+    if (!s->cellEnd && !dalloc(s->cellEnd, cells)) return false;
+    return true;
+}
+
+static bool ensure_float_rho(sd_sim* s) {
+    if (s->rhoF) return true;
+    return dalloc(s->rhoF, (size_t)s->G * s->G * s->G);
+}
+
+static int key_bits(const sd_sim* s) {
+    // keys are cell + 1 <= G^3 (row-major) or morton + 1 < 2^(3 ceil(log2 G)) + 1
+    unsigned long long maxKey;
+    if (s->modes.keyMode == SD_KEY_MORTON) {
+        int b = 0;
+        while ((1 << b) < s->G) ++b;
+        maxKey = (1ull << (3 * b));
+    } else {
+        maxKey = (unsigned long long)s->G * s->G * s->G;
+    }
+    int bits = 0;
+    while ((1ull << bits) <= maxKey) ++bits;
+    return bits;
+}
+
+// Impl::sortParticles, Sim.cu:3656-3671. forContact selects the contact
+// cell key (Morton of the cell) instead of the stencil-node key.
+static void sort_particles(sd_sim* s, bool forContact = false) {
+    const int n = s->allocN;
+    const int kmode = s->modes.keyMode == SD_KEY_MORTON ? (forContact ? 1 : 2) : 0;
+    TIMED(s, "sort", {
+        launch_cell_keys(s->pos, n, s->G, kmode, s->sb.keys, s->sb.idx, s->st);
+        radix_sort_pairs(s->sb, n, key_bits(s), s->st);
+        launch_reorder(s->pos, s->vel, s->sb.idx, s->posTmp, s->velTmp, n, s->st);
+    });
+    std::swap(s->pos, s->posTmp);
+    std::swap(s->vel, s->velTmp);
+    s->keysSorted = true;
+}
+
+static DtPolicy dt_policy(const sd_sim* s) {
+    DtPolicy p{};
+    p.fixedDt = s->fixedDt;
+    const float ts = s->cfg.timeScale;
+    // dtScaleFor, Sim.h:89-93 (kMaxStepReps = 3)
+    const float scale = ts <= 1.0f ? ts : (ts <= 3.0f ? 1.0f : ts / 3.0f);
+    p.baseDt = 0.0016f * scale;
+    p.cell = 1.0f / (float)s->G;
+    p.dtStiff = s->cfg.contactEnabled ? 0.5f / sqrtf(fmaxf(s->cfg.contactStiffness, 1.0f)) : 0.f;
+    p.leapfrog = s->modes.integrator == SD_INTEGRATOR_KDK ? 1 : 0;
+    return p;
+}
+
+// deposit of the current positions into rhoFix (or rhoF in float mode)
+static void deposit_now(sd_sim* s) {
+    const long long cells = (long long)s->G * s->G * s->G;
+    if (s->modes.depositMode == SD_DEPOSIT_FLOAT) {
+        if (!ensure_float_rho(s)) return;
+        launch_fill_f32(s->rhoF, cells, 0.f, s->st);
+        TIMED(s, "deposit", launch_deposit_float(s->pos, s->allocN, s->G, s->periodic, s->rhoF, s->st));
+    } else {
+        // rhoFix is zero here (allocated zero, or cleared by the mesh pass)
+        TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
+        s->rhoValid = true;
+    }
+}
+
+// X pass for the float deposit: convert rhoF into the int64 staging format is
+// not needed -- the float grid is read through the fixed-point path by scaling.
+// (Float mode deposits into rhoF; rows_fwd reads rhoFix, so float mode copies.)
+__global__ void k_float_to_fix(const float* __restrict__ f, long long* __restrict__ q, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = (long long)llrintf(f[i] * (float)SD_FIX_ONE);
+}
+
+// solvePoisson + kGridAccel (Sim.cu:3586-3611): consumes (and clears) rhoFix.
+static void solve_mesh(sd_sim* s) {
+    const long long cells = (long long)s->G * s->G * s->G;
+    if (s->modes.depositMode == SD_DEPOSIT_FLOAT) {
+        hipLaunchKernelGGL(k_float_to_fix, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->st, s->rhoF,
+                           s->rhoFix, cells);
+    }
+    TIMED(s, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, s->st));
+    TIMED(s, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, s->st));
+    const float potScale = s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1);
+    TIMED(s, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, s->st));
+    s->rhoValid = false;
+    s->meshValid = true;
+}
+
+// Invariant (fixed-point mode): rhoFix is all zero whenever !rhoValid.
+static void ensure_rho(sd_sim* s) {
+    if (s->modes.depositMode == SD_DEPOSIT_FLOAT || !s->rhoValid) deposit_now(s);
+}
+
+static void ensure_mesh(sd_sim* s) {
+    if (s->meshValid) return;
+    ensure_rho(s);
+    solve_mesh(s);
+}
+
+// doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685)
+static bool contact_pass(sd_sim* s) {
+    const int G = s->G;
+    if (!s->cfg.contactEnabled) return false;
+    if ((double)s->allocN > 0.764 * (double)G * G * G) return false;
+    if (!ensure_contact_buffers(s)) return false;
+    sort_particles(s, true);
+    const long long cells = cell_table_size(s);
+    TIMED(s, "contact", {
+        launch_cell_range(s->sb.keys, s->allocN, (int)cells, s->cellStart, s->cellEnd, s->st);
+        launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellStart, s->cellEnd,
+                       s->cfg.contactRadiusCells / (float)G, s->cfg.contactStiffness, s->cfg.contactDamping,
+                       s->accContact, s->st);
+    });
+    return true;
+}
+
+static void prime_scalars(sd_sim* s) {
+    launch_vmax(s->pos, s->vel, s->allocN, s->partials, s->st);
+    launch_finalize(s->partials, s->nparts, 2 | 4, dt_policy(s), s->sc, s->st);
+    s->primed = true;
+}
+
+// One step; no host synchronisation inside.
+static void enqueue_step(sd_sim* s) {
+    const bool fused = s->modes.pipeline == SD_PIPELINE_FUSED;
+    HK(hipEventRecord(s->evA, s->st));
+    const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
+                         !s->cfg.contactEnabled;
+    if (sortDue) sort_particles(s);
+    if (fused) {
+        ensure_mesh(s);
+    } else {
+        ensure_rho(s);
+        solve_mesh(s);
+    }
+    const bool contact = contact_pass(s);
+    HK(hipEventRecord(s->ev1, s->st));
+
+    IntegrateArgs a{};
+    a.accG = s->accG;
+    a.pos = s->pos;
+    a.vel = s->vel;
+    a.n = s->allocN;
+    a.G = s->G;
+    a.periodic = s->periodic;
+    a.c2 = light_speed_sq(s->cfg.lengthScale, s->cfg.timeUnitScale);
+    a.accContact = contact ? s->accContact : nullptr;
+    a.softBoundR = s->periodic ? 0.f : fmaxf(s->cfg.softBoundR, 0.f);
+    a.darkEnergy = s->cfg.darkEnergy;
+    a.sc = s->sc;
+    const DtPolicy pol = dt_policy(s);
+    if (fused) {
+        if (!s->primed) prime_scalars(s);
+        a.applyMeanSub = s->modes.momentumZero ? 1 : 0;
+        const bool fusedDeposit = s->modes.depositMode == SD_DEPOSIT_FIXED;
+        a.depositTo = fusedDeposit ? s->rhoFix : nullptr;
+        a.partials = s->partials;
+        TIMED(s, "integrate", launch_integrate(a, s->st));
+        const int mode = (s->modes.momentumZero ? 1 : 0) | 2 | 8 | 4;
+        TIMED(s, "finalize", launch_finalize(s->partials, s->nparts, mode, pol, s->sc, s->st));
+        s->primed = true;
+        s->rhoValid = fusedDeposit;
+    } else {
+        // kMaxSpeed + dt policy, then kIntegrate, then momentum removal
+        launch_vmax(s->pos, s->vel, s->allocN, s->partials, s->st);
+        launch_finalize(s->partials, s->nparts, 2 | 4, pol, s->sc, s->st);
+        a.applyMeanSub = 0;
+        a.depositTo = nullptr;
+        a.partials = nullptr;
+        TIMED(s, "integrate", launch_integrate(a, s->st));
+        if (s->modes.momentumZero) {
+            launch_momentum(s->pos, s->vel, s->allocN, s->partials, s->st);
+            launch_finalize(s->partials, s->nparts, 1 | 8, pol, s->sc, s->st);
+            launch_subtract_mean(s->pos, s->vel, s->allocN, s->sc, s->st);
+        } else {
+            launch_finalize(s->partials, 0, 8, pol, s->sc, s->st);
+        }
+        s->primed = false;
+        s->rhoValid = false;
+    }
+    s->meshValid = false;
+    s->keysSorted = false;
+    s->stepCount++;
+    HK(hipEventRecord(s->evB, s->st));
+}
+
+static void collect_timings(sd_sim* s) {
+    HK(hipEventSynchronize(s->evB));
+    float a = 0.f, c = 0.f, tot = 0.f;
+    HK(hipEventElapsedTime(&a, s->evA, s->ev1));
+    HK(hipEventElapsedTime(&c, s->ev1, s->evB));
+    HK(hipEventElapsedTime(&tot, s->evA, s->evB));
+    HK(hipMemcpy(s->scHost, s->sc, sizeof(StepScalars), hipMemcpyDeviceToHost));
+    s->tm.scatterMs = a;
+    s->tm.poissonMs = 0.f;
+    s->tm.gatherMs = c;
+    s->tm.totalMs = tot;
+    s->tm.dtUsed = s->scHost->dtPrev;
+    s->tm.maxSpeed = s->scHost->vmax;
+}
+
+__global__ void k_restart_integrator(StepScalars* sc) {
+    sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
+    sc->kdkOpen = 1;
+}
+
+static void synced_velocity(sd_sim* s);
+
+static int check_ready(const sd_sim* s) {
+    if (g_failed) return SD_ERR_FAILED;
+    if (!s || !s->ready) return SD_ERR_STATE;
+    return SD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// defaults / statics
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void sd_config_default(sd_config* c) {
+    if (!c) return;
+    memset(c, 0, sizeof(*c));
+    c->particleCount = 2000000;
+    c->gridSize = 128;
+    c->boundary = SD_BOUNDARY_ISOLATED;
+    c->preset = SD_PRESET_FILAMENT;
+    c->gravity = 0.6f;
+    c->softeningCells = 3.0f;
+    c->timeScale = 1.0f;
+    c->sortInterval = 40;
+    c->lengthScale = 1.0f;
+    c->timeUnitScale = 1.0f;
+    c->softBoundR = 0.f;
+    c->darkEnergy = 0.f;
+    c->contactEnabled = 0;
+    c->contactStiffness = 1.0e6f;
+    c->contactDamping = 0.35f;
+    c->pressureEnabled = 1;  // SimConfig defaults (Sim.h); plain PM turns them off
+    c->coolingEnabled = 1;
+    c->contactRadiusCells = 0.5f;
+}
+
+void sd_modes_default(sd_modes* m) {
+    if (!m) return;
+    m->keyMode = SD_KEY_MORTON;
+    m->depositMode = SD_DEPOSIT_FIXED;
+    m->integrator = SD_INTEGRATOR_KDK;
+    m->pipeline = SD_PIPELINE_FUSED;
+    m->momentumZero = 1;
+    m->exactGrid = 0;
+}
+
+int sd_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0;
+}
+
+int sd_device_name(char* buf, size_t len) {
+    if (!buf || len == 0) return SD_ERR_ARG;
+    buf[0] = 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return SD_ERR_DEVICE;
+    snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+    return SD_OK;
+}
+
+int sd_device_driver(char* buf, size_t len) {
+    if (!buf || len == 0) return SD_ERR_ARG;
+    int v = 0;
+    buf[0] = 0;
+    if (hipDriverGetVersion(&v) != hipSuccess) return SD_ERR_DEVICE;
+    snprintf(buf, len, "HIP %d.%d", v / 10000000, (v / 100000) % 100);
+    return SD_OK;
+}
+
+size_t sd_device_free_bytes(void) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+    return f;
+}
+
+int sd_device_multiprocessors(void) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) return 0;
+    return v;
+}
+
+int sd_max_grid_size(int boundary) { return boundary == SD_BOUNDARY_ISOLATED ? 256 : 512; }
+
+size_t sd_estimate_bytes(int particleCount, int gridSize, int boundary) {
+    const size_t N = (size_t)(particleCount > 0 ? particleCount : 1);
+    const size_t G = (size_t)(gridSize > 0 ? gridSize : 1);
+    const size_t S = boundary == SD_BOUNDARY_ISOLATED ? 2 * G : G;
+    const size_t P = (size_t)fft_pitch((int)S);
+    const size_t cells = G * G * G;
+    size_t b = 0;
+    b += 16 * N * 4;                    // pos, vel, posTmp, velTmp
+    b += 4 * N * 4;                     // keys x2, idx x2
+    b += 16 * N;                        // accContact (when contact is on)
+    b += 8 * cells;                     // fixed-point density
+    b += 8 * G * G * P + 8 * G * S * P; // half-spectrum work arrays
+    b += 4 * cells + 16 * cells;        // pot, accG
+    b += 8 * cells;                     // cell ranges (contact)
+    if (boundary == SD_BOUNDARY_ISOLATED) {
+        b += 4 * (S / 2 + 1) * (S / 2 + 1) * P;  // Green octant
+        b += 2 * 8 * S * S * P;                  // transient Green build arrays
+    }
+    b += 64ull * 1024 * 1024;           // partials, sort tables, slack
+    return b;
+}
+
+int sd_max_particles_for(int gridSize, int boundary, size_t freeBytes) {
+    const size_t gridPart = sd_estimate_bytes(1, gridSize, boundary);
+    if (freeBytes <= gridPart) return 0;
+    const size_t per = 16 * 5 + 4 * 4;
+    const size_t n = (size_t)((double)(freeBytes - gridPart) * 0.8) / per;
+    return n > 1000000000ull ? 1000000000 : (int)n;
+}
+
+double sd_device_bandwidth_gbs(void) {
+    int kHz = 0, bits = 0;
+    if (hipDeviceGetAttribute(&kHz, hipDeviceAttributeMemoryClockRate, 0) != hipSuccess) return 0.0;
+    if (hipDeviceGetAttribute(&bits, hipDeviceAttributeMemoryBusWidth, 0) != hipSuccess) return 0.0;
+    if (kHz <= 0 || bits <= 0) return 0.0;
+    return (double)kHz * 1000.0 * ((double)bits / 8.0) * 2.0 / 1.0e9;
+}
+
+double sd_estimate_step_ms(int particleCount, int gridSize, int boundary) {
+    const double bw = sd_device_bandwidth_gbs();
+    if (bw <= 0.0) return 0.0;
+    const double G = (double)(gridSize > 0 ? gridSize : 1);
+    // compulsory bytes per step: 64 B per particle (fused gather / kick / drift /
+    // deposit) + 140 (isolated) or 60 (periodic) B per mesh cell
+    const double mesh = (boundary == SD_BOUNDARY_ISOLATED ? 140.0 : 60.0) * G * G * G;
+    const double part = 64.0 * (double)particleCount;
+    return (mesh + part) / (bw * 0.5e9) * 1000.0;
+}
+
+int sd_failed(void) { return g_failed ? 1 : 0; }
+const char* sd_fail_message(void) { return g_failMsg; }
+void sd_clear_failure(void) {
+    g_failed = false;
+    g_failMsg[0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+sd_sim* sd_create(void) {
+    sd_sim* s = new sd_sim();
+    sd_config_default(&s->cfg);
+    sd_modes_default(&s->modes);
+    return s;
+}
+
+void sd_destroy(sd_sim* s) {
+    if (!s) return;
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    free_buffers(s);
+    for (hipEvent_t e : s->timer.pool) (void)hipEventDestroy(e);
+    if (s->evA) (void)hipEventDestroy(s->evA);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->evB) (void)hipEventDestroy(s->evB);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+}
+
+int sd_set_modes(sd_sim* s, const sd_modes* m) {
+    if (!s || !m) return SD_ERR_ARG;
+    if (m->keyMode < 0 || m->keyMode > 1 || m->depositMode < 0 || m->depositMode > 1 || m->integrator < 0 ||
+        m->integrator > 1 || m->pipeline < 0 || m->pipeline > 1)
+        return arg_error("invalid mode value");
+    if (s->ready) {
+        if (g_failed) return SD_ERR_FAILED;
+        if (m->exactGrid != s->modes.exactGrid) return arg_error("exactGrid can change only before sd_init");
+        // hand the state over synchronised: pending momentum correction and the
+        // leapfrog half kick applied, integrator restarted
+        synced_velocity(s);
+        std::swap(s->vel, s->velTmp);
+        hipLaunchKernelGGL(k_restart_integrator, dim3(1), dim3(1), 0, s->st, s->sc);
+        launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+        s->rhoValid = false;
+        s->meshValid = false;
+        s->primed = false;
+        HK(hipStreamSynchronize(s->st));
+    }
+    s->modes = *m;
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_get_modes(const sd_sim* s, sd_modes* m) {
+    if (!s || !m) return SD_ERR_ARG;
+    *m = s->modes;
+    return SD_OK;
+}
+
+int sd_reconfigure(sd_sim* s, const sd_config* c) {
+    if (!s) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    int rc = check_config(c);
+    if (rc) return rc;
+    const int G = choose_grid(c, &s->modes);
+    if (!valid_grid(G, c->boundary, s->modes.exactGrid))
+        return arg_error("grid size not supported (G = 2^k, or 3*2^k isolated; FFT side <= 512)");
+    if (c->boundary == SD_BOUNDARY_PERIODIC && !is_pow2(G)) return arg_error("periodic grid must be 2^k");
+    if (!s->st) {
+        HK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        HK(hipEventCreate(&s->evA));
+        HK(hipEventCreate(&s->ev1));
+        HK(hipEventCreate(&s->evB));
+        if (g_failed) return SD_ERR_DEVICE;
+    }
+    const bool structural = !s->ready || G != s->G || c->boundary != (s->periodic ? 1 : 0) ||
+                            c->particleCount != s->allocN ||
+                            (!s->periodic && c->softeningCells != s->cfg.softeningCells) ||
+                            (s->periodic && c->softeningCells != s->cfg.softeningCells);
+    s->cfg = *c;
+    s->cfg.gridSize = G;
+    if (!structural) return SD_OK;
+    const size_t need = sd_estimate_bytes(c->particleCount, G, c->boundary);
+    const size_t freeB = sd_device_free_bytes();
+    if (freeB > 0 && need > freeB) {
+        mark_failure("configuration needs %.1f GB, %.1f GB free", need / 1e9, freeB / 1e9);
+        return SD_ERR_FAILED;
+    }
+    s->G = G;
+    s->periodic = c->boundary == SD_BOUNDARY_PERIODIC ? 1 : 0;
+    s->S = s->periodic ? G : 2 * G;
+    s->P = fft_pitch(s->S);
+    s->allocN = c->particleCount;
+    return allocate(s);
+}
+
+int sd_init(sd_sim* s, const sd_config* c) {
+    if (!s) return SD_ERR_ARG;
+    s->ready = false;
+    return sd_reconfigure(s, c);
+}
+
+int sd_reset(sd_sim* s) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    // Empty preset: every slot inactive, time zero (Sim::reset, Sim.cu:4160-4238)
+    std::vector<float4> hp((size_t)s->allocN, make_float4(-1.f, -1.f, -1.f, 0.f));
+    std::vector<float4> hv((size_t)s->allocN, make_float4(0.f, 0.f, 0.f, 0.f));
+    HK(hipMemcpy(s->pos, hp.data(), sizeof(float4) * hp.size(), hipMemcpyHostToDevice));
+    HK(hipMemcpy(s->vel, hv.data(), sizeof(float4) * hv.size(), hipMemcpyHostToDevice));
+    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    s->active = 0;
+    s->rhoValid = s->meshValid = s->keysSorted = false;
+    reset_scalars(s);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_step(sd_sim* s) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (s->allocN <= 0) return SD_OK;
+    enqueue_step(s);
+    HK(hipGetLastError());
+    collect_timings(s);
+    timer_harvest(s);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_run(sd_sim* s, int steps) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (steps < 0) return SD_ERR_ARG;
+    if (s->allocN <= 0) return SD_OK;
+    for (int k = 0; k < steps; ++k) enqueue_step(s);
+    HK(hipGetLastError());
+    if (steps > 0) {
+        HK(hipStreamSynchronize(s->st));
+        collect_timings(s);
+    }
+    timer_harvest(s);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_synchronize(sd_sim* s) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_get_config(const sd_sim* s, sd_config* c) {
+    if (!s || !c) return SD_ERR_ARG;
+    *c = s->cfg;
+    return SD_OK;
+}
+
+int sd_timings_get(sd_sim* s, sd_timings* t) {
+    if (!s || !t) return SD_ERR_ARG;
+    *t = s->tm;
+    return SD_OK;
+}
+
+static void read_scalars(sd_sim* s) {
+    HK(hipStreamSynchronize(s->st));
+    HK(hipMemcpy(s->scHost, s->sc, sizeof(StepScalars), hipMemcpyDeviceToHost));
+}
+
+double sd_sim_time(sd_sim* s) {
+    if (check_ready(s)) return 0.0;
+    read_scalars(s);
+    return s->scHost->simTime;
+}
+
+long long sd_step_count(sd_sim* s) { return s ? s->stepCount : 0; }
+int sd_grid_size(const sd_sim* s) { return s ? s->G : 0; }
+int sd_particle_count(const sd_sim* s) { return s ? s->allocN : 0; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// reductions used by the measures (device side)
+// ---------------------------------------------------------------------------
+struct GridStats {
+    double sum;
+    long long occupied;
+    float maxv;
+    int maxCount;
+};
+
+__global__ void k_grid_stats(const long long* __restrict__ q, long long n, double* sums, long long* occ,
+                             unsigned long long* maxBits) {
+    __shared__ double s_sum[4];
+    __shared__ long long s_occ[4];
+    __shared__ long long s_max[4];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long v = i < n ? q[i] : 0;
+    double x = (double)v;
+    long long o = ((double)v / (double)SD_FIX_ONE > 1e-6) ? 1 : 0;
+    long long m = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        o += __shfl_xor(o, off, 64);
+        const long long mo = __shfl_xor(m, off, 64);
+        m = mo > m ? mo : m;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_sum[w] = x;
+        s_occ[w] = o;
+        s_max[w] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0;
+        long long to = 0, tm = 0;
+        for (int k = 0; k < 4; ++k) {
+            ts += s_sum[k];
+            to += s_occ[k];
+            tm = s_max[k] > tm ? s_max[k] : tm;
+        }
+        sums[blockIdx.x] = ts;
+        occ[blockIdx.x] = to;
+        atomicMax(maxBits, (unsigned long long)tm);
+    }
+}
+
+__global__ void k_particle_stats(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
+                                 double* __restrict__ out5 /* per block: sx, sy, cnt, ke, pad */) {
+    __shared__ double s[4][4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sx = 0, sy = 0, c = 0, ke = 0;
+    if (i < n) {
+        const float4 p = pos[i];
+        if (p.x >= 0.f) {
+            const float4 v = vel[i];
+            sx = p.x;
+            sy = p.y;
+            c = 1;
+            ke = (double)(v.x * v.x + v.y * v.y + v.z * v.z) * 0.5;
+        }
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    c = wave_sum(c);
+    ke = wave_sum(ke);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s[w][0] = sx;
+        s[w][1] = sy;
+        s[w][2] = c;
+        s[w][3] = ke;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) {
+            double t = 0;
+            for (int k = 0; k < 4; ++k) t += s[k][q];
+            out5[(size_t)blockIdx.x * 4 + q] = t;
+        }
+    }
+}
+
+__global__ void k_cell_count(const float4* __restrict__ pos, int n, int G, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    if (p.x < 0.f) return;
+    const float fG = (float)G;
+    const int cx = min(max((int)(p.x * fG), 0), G - 1);
+    const int cy = min(max((int)(p.y * fG), 0), G - 1);
+    const int cz = min(max((int)(p.z * fG), 0), G - 1);
+    atomicAdd(cnt + ((size_t)cz * G + cy) * G + cx, 1);
+}
+
+__global__ void k_max_i32(const int* __restrict__ a, long long n, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int v = i < n ? a[i] : 0;
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
+__global__ void k_gather_acc(const float4* __restrict__ accG, const float4* __restrict__ pos, int n, int G,
+                             int periodic, float4* __restrict__ out);
+
+// Density grid of the current positions into rhoFix for a query; returns true
+// if the caller must clear rhoFix afterwards (the mesh already consumed it and
+// the fused pipeline expects it empty).
+static bool query_density(sd_sim* s) {
+    if (s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED) return false;
+    const bool clearAfter = s->meshValid && s->modes.pipeline == SD_PIPELINE_FUSED;
+    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st);
+    if (!clearAfter && s->modes.depositMode == SD_DEPOSIT_FIXED) s->rhoValid = true;
+    return clearAfter || s->modes.depositMode != SD_DEPOSIT_FIXED;
+}
+
+static GridStats grid_stats(sd_sim* s) {
+    GridStats r{};
+    const bool clearAfter = query_density(s);
+    const long long cells = (long long)s->G * s->G * s->G;
+    const int nb = (int)((cells + 255) / 256);
+    double* dsum = nullptr;
+    long long* docc = nullptr;
+    unsigned long long* dmax = nullptr;
+    if (!dalloc(dsum, nb) || !dalloc(docc, nb) || !dalloc(dmax, 1)) {
+        dfree(dsum); dfree(docc); dfree(dmax);
+        return r;
+    }
+    launch_fill_i64((long long*)dmax, 1, s->st);
+    hipLaunchKernelGGL(k_grid_stats, dim3(nb), dim3(256), 0, s->st, s->rhoFix, cells, dsum, docc, dmax);
+    std::vector<double> hs(nb);
+    std::vector<long long> ho(nb);
+    unsigned long long hm = 0;
+    HK(hipMemcpyAsync(hs.data(), dsum, sizeof(double) * nb, hipMemcpyDeviceToHost, s->st));
+    HK(hipMemcpyAsync(ho.data(), docc, sizeof(long long) * nb, hipMemcpyDeviceToHost, s->st));
+    HK(hipMemcpyAsync(&hm, dmax, sizeof(hm), hipMemcpyDeviceToHost, s->st));
+    if (clearAfter) launch_fill_i64(s->rhoFix, cells, s->st);
+    HK(hipStreamSynchronize(s->st));
+    for (int b = 0; b < nb; ++b) {
+        r.sum += hs[b];
+        r.occupied += ho[b];
+    }
+    r.sum /= (double)SD_FIX_ONE;
+    r.maxv = (float)((double)hm / (double)SD_FIX_ONE);
+    dfree(dsum); dfree(docc); dfree(dmax);
+    return r;
+}
+
+// velocities as handed out (pending momentum correction, leapfrog sync) into velTmp
+static void synced_velocity(sd_sim* s) {
+    const bool kdk = s->modes.integrator == SD_INTEGRATOR_KDK;
+    if (kdk) ensure_mesh(s);
+    launch_sync_velocity(kdk ? s->accG : nullptr, s->pos, s->vel, s->allocN, s->G, s->periodic, s->sc, s->velTmp,
+                         s->st);
+}
+
+extern "C" {
+
+int sd_active_count(sd_sim* s) {
+    if (check_ready(s)) return 0;
+    const int nb = (s->allocN + 255) / 256;
+    double* d = nullptr;
+    if (!dalloc(d, (size_t)nb * 4)) return 0;
+    hipLaunchKernelGGL(k_particle_stats, dim3(nb), dim3(256), 0, s->st, s->pos, s->vel, s->allocN, d);
+    std::vector<double> h((size_t)nb * 4);
+    HK(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    dfree(d);
+    double c = 0;
+    for (int b = 0; b < nb; ++b) c += h[(size_t)b * 4 + 2];
+    return (int)c;
+}
+
+double sd_measure_total_grid_mass(sd_sim* s) {
+    if (check_ready(s)) return 0.0;
+    return grid_stats(s).sum;
+}
+
+double sd_measure_max_density(sd_sim* s) {
+    if (check_ready(s)) return 0.0;
+    return grid_stats(s).maxv;
+}
+
+int sd_measure_occupied_cells(sd_sim* s) {
+    if (check_ready(s)) return 0;
+    return (int)grid_stats(s).occupied;
+}
+
+static int particle_stats(sd_sim* s, bool synced, double out[4]) {
+    const int nb = (s->allocN + 255) / 256;
+    double* d = nullptr;
+    if (!dalloc(d, (size_t)nb * 4)) return SD_ERR_DEVICE;
+    const float4* v = s->vel;
+    if (synced) {
+        synced_velocity(s);
+        v = s->velTmp;
+    }
+    hipLaunchKernelGGL(k_particle_stats, dim3(nb), dim3(256), 0, s->st, s->pos, v, s->allocN, d);
+    std::vector<double> h((size_t)nb * 4);
+    HK(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    dfree(d);
+    for (int q = 0; q < 4; ++q) out[q] = 0;
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 4; ++q) out[q] += h[(size_t)b * 4 + q];
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_measure_centroid(sd_sim* s, double* cx, double* cy) {
+    if (cx) *cx = 0.5;
+    if (cy) *cy = 0.5;
+    int rc = check_ready(s);
+    if (rc) return rc;
+    double o[4];
+    rc = particle_stats(s, false, o);
+    if (rc) return rc;
+    if (o[2] > 0) {
+        if (cx) *cx = o[0] / o[2];
+        if (cy) *cy = o[1] / o[2];
+    }
+    return SD_OK;
+}
+
+double sd_measure_kinetic_energy(sd_sim* s) {
+    if (check_ready(s)) return 0.0;
+    double o[4];
+    if (particle_stats(s, true, o)) return 0.0;
+    return o[3];
+}
+
+int sd_peak_cell_count(sd_sim* s) {
+    if (check_ready(s)) return -1;
+    const long long cells = (long long)s->G * s->G * s->G;
+    int* cnt = nullptr;
+    int* mx = nullptr;
+    if (!dalloc(cnt, (size_t)cells) || !dalloc(mx, 1)) {
+        dfree(cnt);
+        dfree(mx);
+        return -1;
+    }
+    launch_fill_i32(cnt, cells, 0, s->st);
+    launch_fill_i32(mx, 1, 0, s->st);
+    if (s->allocN > 0)
+        hipLaunchKernelGGL(k_cell_count, dim3((s->allocN + 255) / 256), dim3(256), 0, s->st, s->pos, s->allocN, s->G,
+                           cnt);
+    hipLaunchKernelGGL(k_max_i32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->st, cnt, cells, mx);
+    int h = 0;
+    HK(hipMemcpyAsync(&h, mx, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    dfree(cnt);
+    dfree(mx);
+    return g_failed ? -1 : h;
+}
+
+// ---------------------------------------------------------------------------
+// particles in / out
+// ---------------------------------------------------------------------------
+int sd_set_particles(sd_sim* s, const float* pos4, const float* vel4, int n) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (n < 0 || n > s->allocN || (n > 0 && (!pos4 || !vel4))) return arg_error("sd_set_particles: bad n or null");
+    HK(hipStreamSynchronize(s->st));
+    if (n > 0) {
+        HK(hipMemcpy(s->pos, pos4, sizeof(float4) * (size_t)n, hipMemcpyHostToDevice));
+        HK(hipMemcpy(s->vel, vel4, sizeof(float4) * (size_t)n, hipMemcpyHostToDevice));
+    }
+    if (n < s->allocN) {
+        std::vector<float4> hp((size_t)(s->allocN - n), make_float4(-1.f, -1.f, -1.f, 0.f));
+        std::vector<float4> hv((size_t)(s->allocN - n), make_float4(0.f, 0.f, 0.f, 0.f));
+        HK(hipMemcpy(s->pos + n, hp.data(), sizeof(float4) * hp.size(), hipMemcpyHostToDevice));
+        HK(hipMemcpy(s->vel + n, hv.data(), sizeof(float4) * hv.size(), hipMemcpyHostToDevice));
+    }
+    s->active = n;
+    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    s->rhoValid = s->meshValid = s->keysSorted = false;
+    // keep simTime, restart the integrator state
+    read_scalars(s);
+    StepScalars z{};
+    z.kdkOpen = 1;
+    z.simTime = s->scHost->simTime;
+    HK(hipMemcpy(s->sc, &z, sizeof(z), hipMemcpyHostToDevice));
+    s->primed = false;
+    s->stepCount = 0;
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_get_particles(sd_sim* s, float* pos4, float* vel4, int n) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (n < 0 || n > s->allocN) return arg_error("sd_get_particles: bad n");
+    if (n == 0) return SD_OK;
+    if (vel4) synced_velocity(s);
+    if (pos4) HK(hipMemcpyAsync(pos4, s->pos, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, s->st));
+    if (vel4) HK(hipMemcpyAsync(vel4, s->velTmp, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_set_fixed_dt(sd_sim* s, float dt) {
+    if (!s || dt < 0.f) return SD_ERR_ARG;
+    s->fixedDt = dt;
+    s->primed = false;
+    return SD_OK;
+}
+
+const float* sd_particle_pos_device_ptr(sd_sim* s) { return s && s->ready ? (const float*)s->pos : nullptr; }
+const float* sd_particle_vel_device_ptr(sd_sim* s) { return s && s->ready ? (const float*)s->vel : nullptr; }
+
+int sd_compute_forces(sd_sim* s) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    ensure_mesh(s);
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_get_grid(sd_sim* s, int which, void* out, size_t bytes) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    const size_t cells = (size_t)s->G * s->G * s->G;
+    if (!out) return SD_ERR_ARG;
+    if (which == SD_GRID_DENSITY_FIXED || which == SD_GRID_DENSITY) {
+        const size_t need = which == SD_GRID_DENSITY_FIXED ? cells * 8 : cells * 4;
+        if (bytes < need) return arg_error("sd_get_grid: buffer too small");
+        const bool clearAfter = query_density(s);
+        std::vector<long long> h(cells);
+        HK(hipMemcpyAsync(h.data(), s->rhoFix, cells * 8, hipMemcpyDeviceToHost, s->st));
+        if (clearAfter) launch_fill_i64(s->rhoFix, (long long)cells, s->st);
+        HK(hipStreamSynchronize(s->st));
+        if (which == SD_GRID_DENSITY_FIXED) {
+            memcpy(out, h.data(), cells * 8);
+        } else {
+            float* f = (float*)out;
+            for (size_t i = 0; i < cells; ++i) f[i] = (float)((double)h[i] / (double)SD_FIX_ONE);
+        }
+        return g_failed ? SD_ERR_FAILED : SD_OK;
+    }
+    if (which == SD_GRID_POTENTIAL || which == SD_GRID_ACCEL) {
+        const size_t need = which == SD_GRID_POTENTIAL ? cells * 4 : cells * 16;
+        if (bytes < need) return arg_error("sd_get_grid: buffer too small");
+        ensure_mesh(s);
+        HK(hipMemcpyAsync(out, which == SD_GRID_POTENTIAL ? (const void*)s->pot : (const void*)s->accG, need,
+                          hipMemcpyDeviceToHost, s->st));
+        HK(hipStreamSynchronize(s->st));
+        return g_failed ? SD_ERR_FAILED : SD_OK;
+    }
+    return arg_error("sd_get_grid: unknown grid");
+}
+
+int sd_sort_now(sd_sim* s, uint32_t* keysOut, int32_t* orderOut) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    sort_particles(s);
+    if (keysOut)
+        HK(hipMemcpyAsync(keysOut, s->sb.keys, sizeof(uint32_t) * s->allocN, hipMemcpyDeviceToHost, s->st));
+    if (orderOut)
+        HK(hipMemcpyAsync(orderOut, s->sb.idx, sizeof(int32_t) * s->allocN, hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_particle_accel(sd_sim* s, float* acc4, int n) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!acc4 || n < 0 || n > s->allocN) return SD_ERR_ARG;
+    ensure_mesh(s);
+    hipLaunchKernelGGL(k_gather_acc, dim3((s->allocN + 255) / 256), dim3(256), 0, s->st, s->accG, s->pos, s->allocN,
+                       s->G, s->periodic, s->velTmp);
+    HK(hipMemcpyAsync(acc4, s->velTmp, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_direct_sum(sd_sim* s, const int32_t* targets, int nt, double gm, double eps, double* acc3) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (nt <= 0 || !targets || !acc3) return SD_ERR_ARG;
+    for (int t = 0; t < nt; ++t)
+        if (targets[t] < 0 || targets[t] >= s->allocN) return arg_error("sd_direct_sum: target out of range");
+    const int nch = direct_chunks_for(s->allocN);
+    int32_t* dt = nullptr;
+    double* part = nullptr;
+    double* out = nullptr;
+    if (!dalloc(dt, (size_t)nt) || !dalloc(part, (size_t)nch * nt * 3) || !dalloc(out, (size_t)nt * 3)) {
+        dfree(dt); dfree(part); dfree(out);
+        return SD_ERR_DEVICE;
+    }
+    HK(hipMemcpyAsync(dt, targets, sizeof(int32_t) * nt, hipMemcpyHostToDevice, s->st));
+    TIMED(s, "direct", launch_direct_sum(s->pos, s->allocN, dt, nt, (float)(eps * eps), part, nch, out, gm, s->st));
+    HK(hipMemcpyAsync(acc3, out, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    timer_harvest(s);
+    dfree(dt); dfree(part); dfree(out);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_kernel_timer_enable(sd_sim* s, const char* name) {
+    if (!s) return SD_ERR_ARG;
+    KernelTimer& t = s->timer;
+    t.name = name ? name : "";
+    t.used = 0;
+    t.totalMs = 0.0;
+    t.launches = 0;
+    if (!t.name.empty() && t.pool.empty()) {
+        t.pool.resize(1024);
+        for (auto& e : t.pool) HK(hipEventCreate(&e));
+    }
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_kernel_time_ms(sd_sim* s, double* totalMs, long long* launches) {
+    if (!s) return SD_ERR_ARG;
+    if (s->st) timer_harvest(s);
+    if (totalMs) *totalMs = s->timer.totalMs;
+    if (launches) *launches = s->timer.launches;
+    return SD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// STARDUST v2 state files (Sim.cu:4985-5120), portable stdio
+// ---------------------------------------------------------------------------
+struct StateHeader {
+    char magic[8];
+    int version;
+    int count;
+    int gridSize;
+    int preset;
+    int boundary;
+    float gravity;
+    float simTime;
+    float bhX, bhY, bhRs, bhMass;
+    int bhActive;
+    int reserved[5];
+};
+static_assert(sizeof(StateHeader) == 76, "STARDUST header is 76 bytes");
+
+int sd_save_state(sd_sim* s, const char* path) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!path) return SD_ERR_ARG;
+    const int n = s->active > 0 ? s->active : s->allocN;
+    std::vector<float4> hp((size_t)n), hv((size_t)n);
+    rc = sd_get_particles(s, (float*)hp.data(), (float*)hv.data(), n);
+    if (rc) return rc;
+    StateHeader h{};
+    memcpy(h.magic, "STARDUST", 8);
+    h.version = 2;
+    h.count = n;
+    h.gridSize = s->G;
+    h.preset = s->cfg.preset;
+    h.boundary = s->cfg.boundary;
+    h.gravity = s->cfg.gravity;
+    h.simTime = (float)sd_sim_time(s);
+    h.bhX = h.bhY = 0.5f;
+    h.reserved[0] = 0;  // no black holes on this path
+    FILE* f = fopen(path, "wb");
+    if (!f) return SD_ERR_IO;
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(hp.data(), sizeof(float4), (size_t)n, f) == (size_t)n &&
+              fwrite(hv.data(), sizeof(float4), (size_t)n, f) == (size_t)n;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SD_OK : SD_ERR_IO;
+}
+
+int sd_load_state(sd_sim* s, const char* path) {
+    if (!s || !path) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    FILE* f = fopen(path, "rb");
+    if (!f) return SD_ERR_IO;
+    StateHeader h{};
+    if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, "STARDUST", 8) != 0 || (h.version != 1 && h.version != 2) ||
+        h.count <= 0 || h.count > 100000000) {
+        fclose(f);
+        return SD_ERR_IO;
+    }
+    const int bh = h.version >= 2 ? h.reserved[0] : (h.bhActive ? 1 : 0);
+    if (bh != 0) {
+        fclose(f);
+        return arg_error("state file holds black holes, which this path does not simulate");
+    }
+    sd_config c = s->cfg;
+    c.particleCount = h.count;
+    c.gridSize = h.gridSize;
+    c.preset = SD_PRESET_EMPTY;
+    c.boundary = h.boundary;
+    c.gravity = h.gravity;
+    int rc = sd_reconfigure(s, &c);
+    if (rc) {
+        fclose(f);
+        return rc;
+    }
+    const int n = h.count < s->allocN ? h.count : s->allocN;
+    std::vector<float4> hp((size_t)n), hv((size_t)n);
+    const bool ok = fread(hp.data(), sizeof(float4), (size_t)n, f) == (size_t)n &&
+                    fread(hv.data(), sizeof(float4), (size_t)n, f) == (size_t)n;
+    fclose(f);
+    if (!ok) return SD_ERR_IO;
+    rc = sd_set_particles(s, (const float*)hp.data(), (const float*)hv.data(), n);
+    if (rc) return rc;
+    StepScalars z{};
+    z.kdkOpen = 1;
+    z.simTime = (double)h.simTime;
+    HK(hipMemcpy(s->sc, &z, sizeof(z), hipMemcpyHostToDevice));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+}  // extern "C"
+
+__global__ void k_gather_acc(const float4* __restrict__ accG, const float4* __restrict__ pos, int n, int G,
+                             int periodic, float4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    if (p.x < 0.f) {
+        out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    // sampleAcc, Sim.cu:614-629
+    const float fG = (float)G;
+    const float gx = fmaf(p.x, fG, -0.5f), gy = fmaf(p.y, fG, -0.5f), gz = fmaf(p.z, fG, -0.5f);
+    const int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? fx : 1.f - fx) * (oy ? fy : 1.f - fy) * (oz ? fz : 1.f - fz);
+        const int x = wrap_or_clamp(ix + ox, G, periodic), y = wrap_or_clamp(iy + oy, G, periodic),
+                  z = wrap_or_clamp(iz + oz, G, periodic);
+        const float4 g = accG[((size_t)z * G + y) * G + x];
+        ax = fmaf(g.x, w, ax);
+        ay = fmaf(g.y, w, ay);
+        az = fmaf(g.z, w, az);
+    }
+    out[i] = make_float4(ax, ay, az, 0.f);
+}

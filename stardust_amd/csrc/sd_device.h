@@ -1,0 +1,68 @@
+// sd_device.h -- device-side helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SD_FIX_SHIFT 24
+#define SD_FIX_ONE (1LL << SD_FIX_SHIFT)
+
+namespace sd {
+
+// Cell-index clamp/wrap of the reference's gidx3 (Sim.cu:96-100) on a G-sided grid.
+__device__ __forceinline__ int wrap_or_clamp(int c, int G, int periodic) {
+    if (periodic) {
+        // G is a power of two whenever the boundary is periodic; the modulo form
+        // also serves 3*2^k meshes.
+        c %= G;
+        return c < 0 ? c + G : c;
+    }
+    return min(max(c, 0), G - 1);
+}
+
+__device__ __forceinline__ uint32_t morton_spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return morton_spread10(x) | (morton_spread10(y) << 1) | (morton_spread10(z) << 2);
+}
+
+// Wavefront-wide reductions (64 lanes on CDNA).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Per-step scalars that live on the device so a step never waits on the host.
+struct StepScalars {
+    float dt;          // dt of the step being executed
+    float kick;        // kick weight applied by the integrate kernel
+    float dtPrev;      // previous step's dt (leapfrog kick = (dtPrev + dt)/2)
+    float vmax;        // max |v| measured for the CFL rule
+    float meanSub[4];  // pending momentum correction (x, y, z, unused)
+    double simTime;
+    double mom[4];     // last momentum sums (x, y, z, count)
+    long long stepCount;
+    int kdkOpen;       // 1 until the first leapfrog step has been taken
+    int pad;
+};
+
+struct ReducePartial {  // one per integrate block
+    double sx, sy, sz, cnt;
+    float vmax;
+    float pad[3];
+};
+
+}  // namespace sd

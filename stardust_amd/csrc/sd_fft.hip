@@ -1,0 +1,621 @@
+// sd_fft.hip -- pruned 3D FFT Poisson solve for gfx950.
+//
+// Replaces the reference's solvePoisson (src/sim/Sim.cu:3586-3603: cuFFT R2C,
+// kMulSpec / kPoissonPeriodic, cuFFT C2R) and buildGreen (Sim.cu:3569-3576 +
+// kGreen 563-573). Instead of three full S^3 cuFFT transforms plus a separate
+// multiply, the solve is five passes over HBM, each an LDS-resident batch of
+// one-dimensional Stockham transforms (Govindaraju et al., "High performance
+// discrete Fourier transforms on graphics processors", SC08: iteration
+// v = x[j + r N/R] * W^{r (j mod Ns)}, R-point DFT, store at
+// (j/Ns) Ns R + (j mod Ns) + r Ns):
+//
+//   X  rows     : fixed-point density (G^3, int64) -> A[z][y][kx]   (R2C, input pruned to G)
+//   Y  columns  : A -> B[z][ky][kx]                                  (input pruned to G)
+//   Z  columns  : B -> B, forward * Green * inverse fused in LDS     (only z < G kept)
+//   Y' columns  : B -> A                                             (only y < G kept)
+//   X' rows     : A -> pot[z][y][x]                                  (C2R, only x < G kept)
+//
+// Complex arrays keep kx (S/2+1 values) contiguous with a row pitch P that is
+// a multiple of 16 complex (128 B), so every column tile of 16 is one cache
+// line. The isolated Green spectrum is real and even in each axis; only its
+// octant kx, ky, kz <= S/2 is stored, pre-multiplied by 1/S^3.
+#include "sd_device.h"
+
+namespace sd {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---- small DFTs in registers ---------------------------------------------
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct Dft<3, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float s3 = 0.86602540378443864676f;
+        const float2 t = cadd(v[1], v[2]);
+        const float2 d = csub(v[1], v[2]);
+        const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        // forward: X1 = m - i s3 d, X2 = m + i s3 d
+        const float2 id = make_float2(-s3 * d.y, s3 * d.x);  // i * s3 * d
+        v[0] = cadd(v[0], t);
+        if (INV) {
+            v[1] = cadd(m, id);
+            v[2] = csub(m, id);
+        } else {
+            v[1] = csub(m, id);
+            v[2] = cadd(m, id);
+        }
+    }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const float2 t2 = cadd(v[1], v[3]), t3 = rot90<INV>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float c = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e);
+        Dft<4, INV>::run(o);
+        // o[k] *= W8^k, W8 = exp(-+ i pi/4)
+        const float sg = INV ? 1.f : -1.f;
+        o[1] = cmul(o[1], make_float2(c, sg * c));
+        o[2] = rot90<INV>(o[2]);
+        o[3] = cmul(o[3], make_float2(-c, sg * c));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+template <bool INV>
+struct Dft<16, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 e[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        Dft<8, INV>::run(e);
+        Dft<8, INV>::run(o);
+        const float sg = INV ? 1.f : -1.f;
+        // W16^k = (cos(pi k/8), -+ sin(pi k/8))
+        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+        const float c2 = 0.70710678118654752440f;
+        o[1] = cmul(o[1], make_float2(c1, sg * s1));
+        o[2] = cmul(o[2], make_float2(c2, sg * c2));
+        o[3] = cmul(o[3], make_float2(s1, sg * c1));
+        o[4] = rot90<INV>(o[4]);
+        o[5] = cmul(o[5], make_float2(-s1, sg * c1));
+        o[6] = cmul(o[6], make_float2(-c2, sg * c2));
+        o[7] = cmul(o[7], make_float2(-c1, sg * s1));
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 8] = csub(e[k], o[k]);
+        }
+    }
+};
+
+// ---- one Stockham stage over an LDS line ---------------------------------
+// tw[m] = exp(-2 pi i m / N) (FP64-computed table); INV uses the conjugate.
+template <int N, int R, int NS, int T, bool INV>
+__device__ __forceinline__ void stage(float2* __restrict__ line, const float2* __restrict__ tw,
+                                      int t) {
+    constexpr int NBF = N / R;
+    constexpr int PER = (NBF + T - 1) / T;
+    float2 v[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * T;
+        if (PER * T == NBF || j < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = line[j + r * NBF];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * T;
+        if (PER * T == NBF || j < NBF) {
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * k * (N / (NS * R))];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            Dft<R, INV>::run(v[q]);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[base + r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Radix plans: T threads per line; every stage is a compile-time (R, Ns).
+template <int N>
+struct Plan;
+template <>
+struct Plan<16> {
+    static constexpr int T = 4;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<16, 4, 1, T, INV>(l, tw, t);
+        stage<16, 4, 4, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<32> {
+    static constexpr int T = 8;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<32, 4, 1, T, INV>(l, tw, t);
+        stage<32, 8, 4, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<64> {
+    static constexpr int T = 8;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<64, 8, 1, T, INV>(l, tw, t);
+        stage<64, 8, 8, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<128> {
+    static constexpr int T = 16;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<128, 8, 1, T, INV>(l, tw, t);
+        stage<128, 16, 8, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<192> {
+    static constexpr int T = 24;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<192, 3, 1, T, INV>(l, tw, t);
+        stage<192, 8, 3, T, INV>(l, tw, t);
+        stage<192, 8, 24, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<256> {
+    static constexpr int T = 16;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<256, 16, 1, T, INV>(l, tw, t);
+        stage<256, 16, 16, T, INV>(l, tw, t);
+    }
+};
+template <>
+struct Plan<512> {
+    static constexpr int T = 64;
+    template <bool INV>
+    __device__ static void run(float2* l, const float2* tw, int t) {
+        stage<512, 8, 1, T, INV>(l, tw, t);
+        stage<512, 8, 8, T, INV>(l, tw, t);
+        stage<512, 8, 64, T, INV>(l, tw, t);
+    }
+};
+
+constexpr int TB = 16;  // lines per block (columns of a tile / rows of a row batch)
+
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __restrict__ tw) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
+}
+
+// ---- X pass: rows, real input -> half spectrum ----------------------------
+// SRC 0: fixed-point density rows (int64, nin values per row, zeroed after the
+//        read when clearSrc != 0 so the next deposit starts from an empty grid).
+// SRC 1: the isolated Green function g = -1/(|d| + soft) of kGreen
+//        (Sim.cu:563-573) generated on the fly for an S^3 grid (rows = S^2).
+struct RowsFwdArgs {
+    long long* src;
+    float2* out;
+    int rows;      // number of rows
+    int nin;       // input values per row
+    int P;         // output row pitch (complex)
+    int clearSrc;
+    int S;         // Green: grid side
+    float soft;    // Green: softening (cells)
+};
+
+template <int N, int SRC>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
+    constexpr int LP = N + 1;
+    __shared__ float2 s_line[TB * LP];
+    __shared__ float2 s_tw[N];
+    load_twiddles<N>(s_tw, tw);
+    const int row0 = blockIdx.x * TB;
+    const int nthreads = TB * Plan<N>::T;
+    // load (zero padded)
+    for (int i = threadIdx.x; i < TB * N; i += nthreads) {
+        const int c = i / N, x = i - c * N;
+        const int row = row0 + c;
+        float val = 0.f;
+        if (row < a.rows && x < a.nin) {
+            if (SRC == 0) {
+                long long* p = a.src + (size_t)row * a.nin + x;
+                const long long q = *p;
+                val = (float)q * (1.0f / (float)SD_FIX_ONE);
+                if (a.clearSrc) *p = 0;
+            } else {
+                const int y = row % a.S, z = row / a.S;
+                const int S = a.S, h = S / 2;
+                const int dx = x <= h ? x : S - x;
+                const int dy = y <= h ? y : S - y;
+                const int dz = z <= h ? z : S - z;
+                const float r = sqrtf((float)(dx * dx + dy * dy + dz * dz)) + a.soft;
+                val = -1.0f / r;
+            }
+        }
+        s_line[c * LP + x] = make_float2(val, 0.f);
+    }
+    __syncthreads();
+    const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
+    Plan<N>::template run<false>(s_line + c * LP, s_tw, t);
+    constexpr int NK = N / 2 + 1;
+    for (int i = threadIdx.x; i < TB * NK; i += nthreads) {
+        const int cc = i / NK, k = i - cc * NK;
+        const int row = row0 + cc;
+        if (row < a.rows) a.out[(size_t)row * a.P + k] = s_line[cc * LP + k];
+    }
+}
+
+// ---- X' pass: rows, half spectrum -> real output --------------------------
+struct RowsInvArgs {
+    const float2* in;
+    float* out;
+    int rows;
+    int P;
+    int nout;  // real values kept per row (row stride of out)
+};
+
+template <int N>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
+    constexpr int LP = N + 1;
+    __shared__ float2 s_line[TB * LP];
+    __shared__ float2 s_tw[N];
+    load_twiddles<N>(s_tw, tw);
+    const int row0 = blockIdx.x * TB;
+    const int nthreads = TB * Plan<N>::T;
+    constexpr int NK = N / 2 + 1;
+    for (int i = threadIdx.x; i < TB * NK; i += nthreads) {
+        const int c = i / NK, k = i - c * NK;
+        const int row = row0 + c;
+        float2 v = make_float2(0.f, 0.f);
+        if (row < a.rows) v = a.in[(size_t)row * a.P + k];
+        s_line[c * LP + k] = v;
+        if (k > 0 && k < N / 2) s_line[c * LP + N - k] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
+    Plan<N>::template run<true>(s_line + c * LP, s_tw, t);
+    for (int i = threadIdx.x; i < TB * a.nout; i += nthreads) {
+        const int cc = i / a.nout, x = i - cc * a.nout;
+        const int row = row0 + cc;
+        if (row < a.rows) a.out[(size_t)row * a.nout + x] = s_line[cc * LP + x].x;
+    }
+}
+
+// ---- column passes ---------------------------------------------------------
+// Column b of slab a: element n at in[a*inA + n*inN + b]; output k at
+// out[a*outA + k*outN + b]. A block owns TB adjacent columns of one slab.
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_Z_ISO = 2, COL_Z_PER = 3, COL_GREEN = 4 };
+
+struct ColArgs {
+    const float2* in;
+    float2* out;
+    long long inA, inN, outA, outN;
+    int nb;     // columns per slab
+    int nin;    // rows loaded (zero beyond)
+    int nout;   // rows stored
+    // Z passes
+    const float* green;  // octant table [kz'][ky'][kx], pitch P, (S/2+1)^2 rows
+    float* greenOut;     // COL_GREEN target
+    int P;
+    int S;               // transform length
+    float perScale;      // periodic: -(1/G^3)/(4 pi^2)
+    float perSoft;       // periodic: (2 pi soft / G)^2 (0 disables)
+    float greenNorm;     // COL_GREEN: 1/S^3
+};
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float2* __restrict__ tw) {
+    constexpr int LP = N + 1;
+    __shared__ float2 s_line[TB * LP];
+    __shared__ float2 s_tw[N];
+    const int slab = blockIdx.y;
+    if (MODE == COL_GREEN && slab > N / 2) return;  // only ky <= S/2 is stored
+    load_twiddles<N>(s_tw, tw);
+    const int b0 = blockIdx.x * TB;
+    const int nthreads = TB * Plan<N>::T;
+    const float2* in = a.in + (size_t)slab * a.inA + b0;
+    for (int i = threadIdx.x; i < TB * N; i += nthreads) {
+        const int c = i % TB, n = i / TB;
+        float2 v = make_float2(0.f, 0.f);
+        if (n < a.nin && b0 + c < a.nb) v = in[(size_t)n * a.inN + c];
+        s_line[c * LP + n] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
+    float2* line = s_line + c * LP;
+    if (MODE == COL_INV) {
+        Plan<N>::template run<true>(line, s_tw, t);
+    } else {
+        Plan<N>::template run<false>(line, s_tw, t);
+    }
+    if (MODE == COL_GREEN) {
+        const int h = N / 2;
+        for (int i = threadIdx.x; i < TB * (h + 1); i += nthreads) {
+            const int cc = i % TB, kz = i / TB;
+            if (b0 + cc < a.nb)
+                a.greenOut[((size_t)kz * (h + 1) + slab) * a.P + b0 + cc] =
+                    s_line[cc * LP + kz].x * a.greenNorm;
+        }
+        return;
+    }
+    if (MODE == COL_Z_ISO || MODE == COL_Z_PER) {
+        const int h = N / 2;
+        const int ky = slab;
+        const int kyf = ky <= h ? ky : N - ky;
+        for (int i = threadIdx.x; i < TB * N; i += nthreads) {
+            const int cc = i % TB, kz = i / TB;
+            const int kx = b0 + cc;
+            if (kx >= a.nb) continue;
+            float m;
+            if (MODE == COL_Z_ISO) {
+                const int kzf = kz <= h ? kz : N - kz;
+                m = a.green[((size_t)kzf * (h + 1) + kyf) * a.P + kx];
+            } else {
+                // kPoissonPeriodic, Sim.cu:528-557
+                const int sky = ky <= h ? ky : ky - N;
+                const int skz = kz <= h ? kz : kz - N;
+                const float k2 = (float)(kx * kx + sky * sky + skz * skz);
+                if (k2 < 1e-6f) {
+                    m = 0.f;
+                } else {
+                    m = a.perScale / k2;
+                    if (a.perSoft > 0.f) m *= expf(-0.5f * k2 * a.perSoft);
+                }
+            }
+            float2 v = s_line[cc * LP + kz];
+            s_line[cc * LP + kz] = make_float2(v.x * m, v.y * m);
+        }
+        __syncthreads();
+        Plan<N>::template run<true>(line, s_tw, t);
+    }
+    float2* out = a.out + (size_t)slab * a.outA + b0;
+    for (int i = threadIdx.x; i < TB * a.nout; i += nthreads) {
+        const int cc = i % TB, k = i / TB;
+        if (b0 + cc < a.nb) out[(size_t)k * a.outN + cc] = s_line[cc * LP + k];
+    }
+}
+
+// ---- gradient: kGridAccel, Sim.cu:589-611 ----------------------------------
+// a = -(pot[+1] - pot[-1]) * (0.5 * G * potScale); isolated clamps the stencil,
+// periodic wraps. pot and acc are G^3 [z][y][x].
+__global__ void __launch_bounds__(256) k_grid_accel(const float* __restrict__ pot, float4* __restrict__ acc,
+                                                    int G, float h, int periodic) {
+    const long long cells = (long long)G * G * G;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const int x = (int)(i % G), y = (int)((i / G) % G), z = (int)(i / ((long long)G * G));
+    int xm, xp, ym, yp, zm, zp;
+    if (periodic) {
+        xm = x == 0 ? G - 1 : x - 1; xp = x == G - 1 ? 0 : x + 1;
+        ym = y == 0 ? G - 1 : y - 1; yp = y == G - 1 ? 0 : y + 1;
+        zm = z == 0 ? G - 1 : z - 1; zp = z == G - 1 ? 0 : z + 1;
+    } else {
+        xm = max(x - 1, 0); xp = min(x + 1, G - 1);
+        ym = max(y - 1, 0); yp = min(y + 1, G - 1);
+        zm = max(z - 1, 0); zp = min(z + 1, G - 1);
+    }
+    const size_t row = ((size_t)z * G + y) * G;
+    const float ax = -(pot[row + xp] - pot[row + xm]) * h;
+    const float ay = -(pot[((size_t)z * G + yp) * G + x] - pot[((size_t)z * G + ym) * G + x]) * h;
+    const float az = -(pot[((size_t)zp * G + y) * G + x] - pot[((size_t)zm * G + y) * G + x]) * h;
+    acc[i] = make_float4(ax, ay, az, 0.f);
+}
+
+}  // namespace sd
+
+// ---------------------------------------------------------------------------
+// Host launchers (C linkage inside the library; declared in sd_internal.h)
+// ---------------------------------------------------------------------------
+#include "sd_internal.h"
+
+namespace sd {
+
+template <int N>
+static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hipStream_t st) {
+    dim3 grid((a.rows + TB - 1) / TB), block(TB * Plan<N>::T);
+    if (src == 0)
+        hipLaunchKernelGGL((k_rows_fwd<N, 0>), grid, block, 0, st, a, tw);
+    else
+        hipLaunchKernelGGL((k_rows_fwd<N, 1>), grid, block, 0, st, a, tw);
+}
+template <int N>
+static void launch_rows_inv(const RowsInvArgs& a, const float2* tw, hipStream_t st) {
+    dim3 grid((a.rows + TB - 1) / TB), block(TB * Plan<N>::T);
+    hipLaunchKernelGGL((k_rows_inv<N>), grid, block, 0, st, a, tw);
+}
+template <int N>
+static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw, hipStream_t st) {
+    dim3 grid((a.nb + TB - 1) / TB, slabs), block(TB * Plan<N>::T);
+    switch (mode) {
+        case COL_FWD: hipLaunchKernelGGL((k_cols<N, COL_FWD>), grid, block, 0, st, a, tw); break;
+        case COL_INV: hipLaunchKernelGGL((k_cols<N, COL_INV>), grid, block, 0, st, a, tw); break;
+        case COL_Z_ISO: hipLaunchKernelGGL((k_cols<N, COL_Z_ISO>), grid, block, 0, st, a, tw); break;
+        case COL_Z_PER: hipLaunchKernelGGL((k_cols<N, COL_Z_PER>), grid, block, 0, st, a, tw); break;
+        default: hipLaunchKernelGGL((k_cols<N, COL_GREEN>), grid, block, 0, st, a, tw); break;
+    }
+}
+
+#define SD_DISPATCH_N(N_, CALL)                 \
+    switch (N_) {                               \
+        case 16: CALL(16); break;               \
+        case 32: CALL(32); break;               \
+        case 64: CALL(64); break;               \
+        case 128: CALL(128); break;             \
+        case 192: CALL(192); break;             \
+        case 256: CALL(256); break;             \
+        case 512: CALL(512); break;             \
+        default: return -1;                     \
+    }
+
+bool fft_length_supported(int n) {
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512;
+}
+
+int fft_pitch(int S) { return ((S / 2 + 1) + 15) / 16 * 16; }
+
+int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, hipStream_t st) {
+    RowsFwdArgs a{};
+    a.src = rhoFix;
+    a.out = A;
+    a.rows = m.G * m.G;
+    a.nin = m.G;
+    a.P = m.P;
+    a.clearSrc = clearSrc;
+#define CALL(NN) launch_rows_fwd<NN>(a, 0, m.tw, st)
+    SD_DISPATCH_N(m.S, CALL)
+#undef CALL
+    return 0;
+}
+
+int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
+                    hipStream_t st) {
+    const int G = m.G, S = m.S, P = m.P, NK = S / 2 + 1;
+    // Y forward: A[z][y][kx] (y < G) -> B[z][ky][kx]
+    {
+        ColArgs a{};
+        a.in = A; a.out = B;
+        a.inA = (long long)G * P; a.inN = P;
+        a.outA = (long long)S * P; a.outN = P;
+        a.nb = NK; a.nin = G; a.nout = S; a.P = P; a.S = S;
+#define CALL(NN) launch_cols<NN>(a, COL_FWD, G, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    // Z fused: columns over z of B for each ky
+    {
+        ColArgs a{};
+        a.in = B; a.out = B;
+        a.inA = P; a.inN = (long long)S * P;
+        a.outA = P; a.outN = (long long)S * P;
+        a.nb = NK; a.nin = G; a.nout = G; a.P = P; a.S = S;
+        a.green = green;
+        a.perScale = m.perScale;
+        a.perSoft = m.perSoft;
+        const int mode = m.periodic ? COL_Z_PER : COL_Z_ISO;
+#define CALL(NN) launch_cols<NN>(a, mode, S, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    // Y inverse: B -> A (y < G)
+    {
+        ColArgs a{};
+        a.in = B; a.out = A;
+        a.inA = (long long)S * P; a.inN = P;
+        a.outA = (long long)G * P; a.outN = P;
+        a.nb = NK; a.nin = S; a.nout = G; a.P = P; a.S = S;
+#define CALL(NN) launch_cols<NN>(a, COL_INV, G, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    // X inverse: A rows -> pot (x < G)
+    {
+        RowsInvArgs a{};
+        a.in = A; a.out = pot; a.rows = G * G; a.P = P; a.nout = G;
+#define CALL(NN) launch_rows_inv<NN>(a, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    return 0;
+}
+
+int mesh_build_green(const MeshPlan& m, float softCells, float2* tmpA, float2* tmpB, float* greenOut,
+                     hipStream_t st) {
+    const int S = m.S, P = m.P, NK = S / 2 + 1;
+    {
+        RowsFwdArgs a{};
+        a.out = tmpA; a.rows = S * S; a.nin = S; a.P = P; a.S = S; a.soft = softCells;
+#define CALL(NN) launch_rows_fwd<NN>(a, 1, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    {
+        ColArgs a{};
+        a.in = tmpA; a.out = tmpB;
+        a.inA = (long long)S * P; a.inN = P;
+        a.outA = (long long)S * P; a.outN = P;
+        a.nb = NK; a.nin = S; a.nout = S; a.P = P; a.S = S;
+#define CALL(NN) launch_cols<NN>(a, COL_FWD, S, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    {
+        ColArgs a{};
+        a.in = tmpB;
+        a.inA = P; a.inN = (long long)S * P;
+        a.nb = NK; a.nin = S; a.nout = 0; a.P = P; a.S = S;
+        a.greenOut = greenOut;
+        a.greenNorm = (float)(1.0 / ((double)S * S * S));
+#define CALL(NN) launch_cols<NN>(a, COL_GREEN, S, m.tw, st)
+        SD_DISPATCH_N(S, CALL)
+#undef CALL
+    }
+    return 0;
+}
+
+void mesh_grid_accel(const float* pot, float4* acc, int G, float potScale, int periodic,
+                     hipStream_t st) {
+    const long long cells = (long long)G * G * G;
+    const float h = 0.5f * (float)G * potScale;
+    hipLaunchKernelGGL(k_grid_accel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, pot,
+                       acc, G, h, periodic);
+}
+
+}  // namespace sd

@@ -1,0 +1,113 @@
+// sd_internal.h -- host-side launchers shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sd_device.h"
+
+namespace sd {
+
+// ---- mesh -------------------------------------------------------------------
+struct MeshPlan {
+    int G;          // mesh side holding mass
+    int S;          // FFT side (2G isolated, G periodic)
+    int P;          // complex row pitch of the half spectrum
+    int periodic;
+    const float2* tw;   // exp(-2 pi i m / S), m < S
+    float perScale;     // periodic multiplier scale (negative)
+    float perSoft;      // periodic Gaussian (2 pi soft / G)^2, 0 = off
+};
+
+bool fft_length_supported(int n);
+int fft_pitch(int S);
+int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, hipStream_t st);
+int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
+                    hipStream_t st);
+int mesh_build_green(const MeshPlan& m, float softCells, float2* tmpA, float2* tmpB,
+                     float* greenOut, hipStream_t st);
+void mesh_grid_accel(const float* pot, float4* acc, int G, float potScale, int periodic,
+                     hipStream_t st);
+
+// ---- keys / sort ------------------------------------------------------------
+struct SortBuffers {
+    uint32_t* keys;     // n
+    int32_t* idx;       // n
+    uint32_t* keysAlt;  // n
+    int32_t* idxAlt;    // n
+    uint32_t* hist;     // 256 * maxTiles
+    uint32_t* chunk;    // scan partials
+    int maxTiles;
+};
+void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
+                      hipStream_t st);
+int sort_tiles_for(int n);
+int sort_chunks_for(int n);
+// Stable LSD sort of (keys, idx) by the low `bits` bits. The result is in
+// buf.keys / buf.idx on return.
+void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st);
+void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
+                    float4* dstV, int n, hipStream_t st);
+void launch_cell_range(const uint32_t* keys, int n, int ncells, int32_t* start, int32_t* end,
+                       hipStream_t st);
+
+// ---- deposit / integrate ----------------------------------------------------
+void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long long* rho,
+                          hipStream_t st);
+void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* rho,
+                          hipStream_t st);
+void launch_fill_i64(long long* p, long long n, hipStream_t st);
+void launch_fill_f32(float* p, long long n, float v, hipStream_t st);
+void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st);
+
+struct IntegrateArgs {
+    const float4* accG;
+    float4* pos;
+    float4* vel;
+    int n;
+    int G;
+    int periodic;
+    float c2;
+    const float4* accContact;
+    float softBoundR;
+    float darkEnergy;
+    StepScalars* sc;
+    int applyMeanSub;     // subtract sc->meanSub on load
+    long long* depositTo; // fused deposit of the new positions (nullptr = off)
+    ReducePartial* partials;
+};
+int integrate_blocks(int n);
+void launch_integrate(const IntegrateArgs& a, hipStream_t st);
+
+// reductions / scalars
+void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* partials,
+                 hipStream_t st);
+void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial* partials,
+                     hipStream_t st);
+struct DtPolicy {
+    float fixedDt;      // > 0: fixed step
+    float baseDt;       // 0.0016 * dtScaleFor(timeScale)
+    float cell;         // 1/G
+    float dtStiff;      // contact cap (0 = none)
+    int leapfrog;       // kick = (dtPrev + dt)/2
+};
+// Reduce `nparts` partials; mode bit0 = momentum (store mean into sc->meanSub and
+// sc->mom), bit1 = vmax -> sc->vmax, bit2 = compute next dt/kick from vmax,
+// bit3 = advance simTime/stepCount by the current dt.
+void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol,
+                     StepScalars* sc, hipStream_t st);
+void launch_subtract_mean(const float4* pos, float4* vel, int n, StepScalars* sc, hipStream_t st);
+void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G,
+                          int periodic, const StepScalars* sc, float4* velOut, hipStream_t st);
+
+// ---- contact ------------------------------------------------------------------
+void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
+                    const int32_t* cellStart, const int32_t* cellEnd, float radius,
+                    float stiffness, float damping, float4* accOut, hipStream_t st);
+
+// ---- direct sum -------------------------------------------------------------
+void launch_direct_sum(const float4* pos, int n, const int32_t* targets, int nt, float eps2,
+                       double* partial, int nchunks, double* out3, double gm, hipStream_t st);
+int direct_chunks_for(int n);
+
+}  // namespace sd

@@ -1,0 +1,653 @@
+// sd_particles.hip -- per-particle kernels of the PM step on gfx950.
+//
+//   deposit   : kScatter (src/sim/Sim.cu:503-521) as an int64 fixed-point CIC
+//               (exact, order-independent) or as the reference's float adds;
+//   integrate : kIntegrate + sampleAcc (Sim.cu:614-629, 685-906), plain-PM
+//               subset, with an optional fused deposit of the new positions and
+//               fused block partials for the momentum sum (kMomentumAccum,
+//               2252-2268) and the CFL speed (kMaxSpeed, 2923-2930);
+//   finalize  : deterministic reduction of those partials and the dt policy
+//               (Sim.cu:4254-4296) evaluated on the device, so a step needs no
+//               host round trip;
+//   contact   : kContact (Sim.cu:1030-1203), contact term.
+//
+// Arithmetic mirrors the reference as built by nvcc (fused multiply-adds
+// where nvcc contracts "a*b + c"), written as explicit fmaf with contraction
+// otherwise off, so results are reproducible bit for bit against the CPU
+// restatement given the same mesh accelerations.
+#include "sd_device.h"
+#include "sd_internal.h"
+
+#include <climits>
+
+#pragma clang fp contract(off)
+
+namespace sd {
+
+constexpr int PT = 256;  // threads per particle block
+
+// ---- fills -------------------------------------------------------------------
+__global__ void k_fill_i64(long long* p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void k_fill_f32(float* p, long long n, float v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, long long n, int v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i64(long long* p, long long n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+void launch_fill_f32(float* p, long long n, float v, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+// ---- CIC helpers -------------------------------------------------------------
+struct Cic {
+    int ix, iy, iz;
+    float fx, fy, fz;
+};
+__device__ __forceinline__ Cic cic_setup(float4 p, float fG) {
+    Cic c;
+    const float gx = fmaf(p.x, fG, -0.5f), gy = fmaf(p.y, fG, -0.5f), gz = fmaf(p.z, fG, -0.5f);
+    c.ix = (int)floorf(gx);
+    c.iy = (int)floorf(gy);
+    c.iz = (int)floorf(gz);
+    c.fx = gx - (float)c.ix;
+    c.fy = gy - (float)c.iy;
+    c.fz = gz - (float)c.iz;
+    return c;
+}
+
+// The 2^24 mass units of a particle split exactly along x, y, z (w1 = rint(f w),
+// w0 = w - w1), so the eight weights always sum to 2^24.
+__device__ __forceinline__ void fixed_weights(const Cic& c, long long w[8]) {
+    long long wx[2], wxy[4];
+    wx[1] = (long long)rintf(c.fx * (float)SD_FIX_ONE);
+    wx[0] = SD_FIX_ONE - wx[1];
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox) {
+        const long long y1 = (long long)rintf(c.fy * (float)wx[ox]);
+        wxy[ox + 2] = y1;
+        wxy[ox] = wx[ox] - y1;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long z1 = (long long)rintf(c.fz * (float)wxy[j]);
+        w[j + 4] = z1;
+        w[j] = wxy[j] - z1;
+    }
+}
+
+// Deterministic deposit of one wavefront's particles. Lanes whose CIC base
+// node matches the previous lane's form a run (particles sorted by the Morton
+// index of that node make runs long); an inclusive segmented scan over the
+// wavefront sums each run's eight integer weights, and only the last lane of a
+// run issues the eight global atomics. Integer sums are exact, so the grid is
+// bit-identical whatever the order. Must be called by all 64 lanes.
+__device__ __forceinline__ void deposit_wave(bool alive, const Cic& c, int G, int periodic, long long* rho) {
+    const int lane = threadIdx.x & 63;
+    long long v[8];
+    if (alive) {
+        fixed_weights(c, v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = 0;
+    }
+    const int ix = alive ? c.ix : INT_MIN, iy = alive ? c.iy : INT_MIN, iz = alive ? c.iz : INT_MIN;
+    const int pix = __shfl_up(ix, 1, 64), piy = __shfl_up(iy, 1, 64), piz = __shfl_up(iz, 1, 64);
+    const bool head = lane == 0 || !alive || pix != ix || piy != iy || piz != iz;
+    const uint64_t headMask = __ballot(head);
+    const uint64_t le = lane == 63 ? headMask : (headMask & ((2ull << lane) - 1ull));
+    const int start = 63 - __clzll(le);
+    const bool tail = lane == 63 || ((headMask >> (lane + 1)) & 1ull);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool take = lane - off >= start;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long long y = __shfl_up(v[k], off, 64);
+            if (take) v[k] += y;
+        }
+    }
+    if (!(alive && tail)) return;
+    int xs[2], ys[2], zs[2];
+    xs[0] = wrap_or_clamp(c.ix, G, periodic);
+    xs[1] = wrap_or_clamp(c.ix + 1, G, periodic);
+    ys[0] = wrap_or_clamp(c.iy, G, periodic);
+    ys[1] = wrap_or_clamp(c.iy + 1, G, periodic);
+    zs[0] = wrap_or_clamp(c.iz, G, periodic);
+    zs[1] = wrap_or_clamp(c.iz + 1, G, periodic);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        if (v[k] != 0) {
+            const size_t idx = ((size_t)zs[oz] * G + ys[oy]) * G + xs[ox];
+            atomicAdd((unsigned long long*)(rho + idx), (unsigned long long)v[k]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PT) k_deposit_fixed(const float4* __restrict__ pos, int n, int G, int periodic,
+                                                     long long* __restrict__ rho) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
+    if (i < n) p = pos[i];
+    const bool alive = p.x >= 0.f;
+    Cic c = cic_setup(alive ? p : make_float4(0.f, 0.f, 0.f, 0.f), (float)G);
+    deposit_wave(alive, c, G, periodic, rho);
+}
+
+// Reference float-atomic deposit (order-dependent) for parity runs.
+__global__ void __launch_bounds__(PT) k_deposit_float(const float4* __restrict__ pos, int n, int G, int periodic,
+                                                     float* __restrict__ rho) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    if (p.x < 0.f) return;
+    const Cic c = cic_setup(p, (float)G);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
+        const int x = wrap_or_clamp(c.ix + ox, G, periodic), y = wrap_or_clamp(c.iy + oy, G, periodic),
+                  z = wrap_or_clamp(c.iz + oz, G, periodic);
+        atomicAdd(rho + ((size_t)z * G + y) * G + x, w);
+    }
+}
+
+void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long long* rho, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_deposit_fixed, dim3((n + PT - 1) / PT), dim3(PT), 0, st, pos, n, G, periodic, rho);
+}
+void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* rho, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_deposit_float, dim3((n + PT - 1) / PT), dim3(PT), 0, st, pos, n, G, periodic, rho);
+}
+
+// sampleAcc, Sim.cu:614-629.
+__device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, const Cic& c, int G, int periodic) {
+    const int x0 = wrap_or_clamp(c.ix, G, periodic), x1 = wrap_or_clamp(c.ix + 1, G, periodic);
+    const int y0 = wrap_or_clamp(c.iy, G, periodic), y1 = wrap_or_clamp(c.iy + 1, G, periodic);
+    const int z0 = wrap_or_clamp(c.iz, G, periodic), z1 = wrap_or_clamp(c.iz + 1, G, periodic);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
+        const int x = ox ? x1 : x0, y = oy ? y1 : y0, z = oz ? z1 : z0;
+        const float4 g = accG[((size_t)z * G + y) * G + x];
+        ax = fmaf(g.x, w, ax);
+        ay = fmaf(g.y, w, ay);
+        az = fmaf(g.z, w, az);
+    }
+    return make_float3(ax, ay, az);
+}
+
+// Block reduction of (sum v, count, max |v|) into one partial per block.
+__device__ __forceinline__ void block_partial(double sx, double sy, double sz, double cnt, float vmax,
+                                              ReducePartial* out) {
+    __shared__ double s_d[PT / 64][4];
+    __shared__ float s_f[PT / 64];
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    cnt = wave_sum(cnt);
+    vmax = wave_max(vmax);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        s_d[w][0] = sx;
+        s_d[w][1] = sy;
+        s_d[w][2] = sz;
+        s_d[w][3] = cnt;
+        s_f[w] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ReducePartial r;
+        r.sx = r.sy = r.sz = r.cnt = 0.0;
+        r.vmax = 0.f;
+        for (int k = 0; k < PT / 64; ++k) {
+            r.sx += s_d[k][0];
+            r.sy += s_d[k][1];
+            r.sz += s_d[k][2];
+            r.cnt += s_d[k][3];
+            r.vmax = fmaxf(r.vmax, s_f[k]);
+        }
+        r.pad[0] = r.pad[1] = r.pad[2] = 0.f;
+        out[blockIdx.x] = r;
+    }
+}
+
+__device__ __forceinline__ float speed3(float x, float y, float z) {
+    return sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
+}
+
+// kIntegrate, Sim.cu:685-906 (plain PM: no black holes, no pressure).
+__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+    float vm = 0.f;
+    bool depAlive = false;
+    Cic depC;
+    depC.ix = depC.iy = depC.iz = 0;
+    depC.fx = depC.fy = depC.fz = 0.f;
+    if (i < a.n) {
+        float4 p = a.pos[i];
+        if (p.x >= 0.f) {
+            const float kick = a.sc->kick, drift = a.sc->dt;
+            float4 v = a.vel[i];
+            if (a.applyMeanSub) {
+                v.x -= a.sc->meanSub[0];
+                v.y -= a.sc->meanSub[1];
+                v.z -= a.sc->meanSub[2];
+            }
+            const float fG = (float)a.G;
+            float3 acc = sample_acc(a.accG, cic_setup(p, fG), a.G, a.periodic);
+            if (a.accContact) {
+                const float4 c = a.accContact[i];
+                acc.x += c.x;
+                acc.y += c.y;
+                acc.z += c.z;
+            }
+            if (a.darkEnergy != 0.f) {
+                float k = 1.0f;
+                if (a.softBoundR > 0.f) {
+                    const float dx = p.x - 0.5f, dy = p.y - 0.5f, dz = p.z - 0.5f;
+                    const float r = speed3(dx, dy, dz);
+                    if (r > a.softBoundR) {
+                        const float R1 = 0.497f;
+                        k = 1.0f - fminf((r - a.softBoundR) / fmaxf(R1 - a.softBoundR, 1e-6f), 1.0f);
+                    }
+                }
+                if (k > 0.f) {
+                    acc.x = fmaf(a.darkEnergy * k, p.x - 0.5f, acc.x);
+                    acc.y = fmaf(a.darkEnergy * k, p.y - 0.5f, acc.y);
+                    acc.z = fmaf(a.darkEnergy * k, p.z - 0.5f, acc.z);
+                }
+            }
+            v.x = fmaf(acc.x, kick, v.x);
+            v.y = fmaf(acc.y, kick, v.y);
+            v.z = fmaf(acc.z, kick, v.z);
+            {
+                const float sp2 = fmaf(v.z, v.z, fmaf(v.y, v.y, v.x * v.x));
+                if (sp2 > a.c2) {
+                    const float k = sqrtf(a.c2 / sp2);
+                    v.x *= k;
+                    v.y *= k;
+                    v.z *= k;
+                }
+            }
+            p.x = fmaf(v.x, drift, p.x);
+            p.y = fmaf(v.y, drift, p.y);
+            p.z = fmaf(v.z, drift, p.z);
+            bool alive = true;
+            if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(v.x) || !isfinite(v.y) ||
+                !isfinite(v.z)) {
+                p = make_float4(-1.f, -1.f, -1.f, 0.f);
+                v = make_float4(0.f, 0.f, 0.f, 0.f);
+                alive = false;
+            } else if (a.periodic) {
+                p.x -= floorf(p.x);
+                p.y -= floorf(p.y);
+                p.z -= floorf(p.z);
+            } else if (a.softBoundR > 0.f) {
+                const float dx = p.x - 0.5f, dy = p.y - 0.5f, dz = p.z - 0.5f;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float R1 = 0.497f;
+                if (r2 > a.softBoundR * a.softBoundR) {
+                    const float r = sqrtf(r2);
+                    const float inv = 1.0f / fmaxf(r, 1e-6f);
+                    const float nx = dx * inv, ny = dy * inv, nz = dz * inv;
+                    const float t = fminf((r - a.softBoundR) / fmaxf(R1 - a.softBoundR, 1e-6f), 1.0f);
+                    const float vr = fmaf(v.z, nz, fmaf(v.y, ny, v.x * nx));
+                    if (vr > 0.f) {
+                        const float cut = vr * t * t;
+                        v.x = fmaf(-nx, cut, v.x);
+                        v.y = fmaf(-ny, cut, v.y);
+                        v.z = fmaf(-nz, cut, v.z);
+                    }
+                    const float pull = 2.0f * t * t;
+                    v.x = fmaf(-(nx * pull), drift, v.x);
+                    v.y = fmaf(-(ny * pull), drift, v.y);
+                    v.z = fmaf(-(nz * pull), drift, v.z);
+                    if (r > R1) {
+                        const float k = R1 * inv;
+                        p.x = fmaf(dx, k, 0.5f);
+                        p.y = fmaf(dy, k, 0.5f);
+                        p.z = fmaf(dz, k, 0.5f);
+                    }
+                }
+            } else {
+                if (p.x < 0.002f) { p.x = 0.002f; v.x = fabsf(v.x) * 0.25f; }
+                if (p.x > 0.998f) { p.x = 0.998f; v.x = -fabsf(v.x) * 0.25f; }
+                if (p.y < 0.002f) { p.y = 0.002f; v.y = fabsf(v.y) * 0.25f; }
+                if (p.y > 0.998f) { p.y = 0.998f; v.y = -fabsf(v.y) * 0.25f; }
+                if (p.z < 0.002f) { p.z = 0.002f; v.z = fabsf(v.z) * 0.25f; }
+                if (p.z > 0.998f) { p.z = 0.998f; v.z = -fabsf(v.z) * 0.25f; }
+            }
+            a.pos[i] = p;
+            a.vel[i] = v;
+            if (alive) {
+                depAlive = true;
+                depC = cic_setup(p, fG);
+                sx = (double)v.x;
+                sy = (double)v.y;
+                sz = (double)v.z;
+                cnt = 1.0;
+                vm = speed3(v.x, v.y, v.z);
+            }
+        }
+    }
+    if (a.depositTo) deposit_wave(depAlive, depC, a.G, a.periodic, a.depositTo);
+    if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials);
+}
+
+int integrate_blocks(int n) { return (n + PT - 1) / PT; }
+
+void launch_integrate(const IntegrateArgs& a, hipStream_t st) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_integrate, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
+}
+
+// kMaxSpeed (Sim.cu:2923-2930) as block partials.
+__global__ void __launch_bounds__(PT) k_vmax(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
+                                            ReducePartial* partials) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    float vm = 0.f;
+    if (i < n && pos[i].x >= 0.f) {
+        const float4 v = vel[i];
+        vm = speed3(v.x, v.y, v.z);
+    }
+    block_partial(0.0, 0.0, 0.0, 0.0, vm, partials);
+}
+
+// kMomentumAccum (Sim.cu:2252-2268) as block partials.
+__global__ void __launch_bounds__(PT) k_momentum(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                int n, ReducePartial* partials) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+    if (i < n && pos[i].x >= 0.f) {
+        const float4 v = vel[i];
+        if (isfinite(v.x) && isfinite(v.y) && isfinite(v.z)) {
+            sx = (double)v.x;
+            sy = (double)v.y;
+            sz = (double)v.z;
+            cnt = 1.0;
+        }
+    }
+    block_partial(sx, sy, sz, cnt, 0.f, partials);
+}
+
+void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_vmax, dim3(integrate_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
+}
+void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_momentum, dim3(integrate_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
+}
+
+// Deterministic reduction (fixed summation order) + dt policy.
+__global__ void __launch_bounds__(256) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
+                                                  DtPolicy pol, StepScalars* sc) {
+    __shared__ double s_d[256][4];
+    __shared__ float s_f[256];
+    double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+    float vm = 0.f;
+    for (int k = threadIdx.x; k < np; k += 256) {
+        const ReducePartial r = parts[k];
+        sx += r.sx;
+        sy += r.sy;
+        sz += r.sz;
+        cnt += r.cnt;
+        vm = fmaxf(vm, r.vmax);
+    }
+    s_d[threadIdx.x][0] = sx;
+    s_d[threadIdx.x][1] = sy;
+    s_d[threadIdx.x][2] = sz;
+    s_d[threadIdx.x][3] = cnt;
+    s_f[threadIdx.x] = vm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s_d[threadIdx.x][q] += s_d[threadIdx.x + s][q];
+            s_f[threadIdx.x] = fmaxf(s_f[threadIdx.x], s_f[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (mode & 1) {
+        for (int q = 0; q < 4; ++q) sc->mom[q] = s_d[0][q];
+        const double c = s_d[0][3];
+        if (c >= 1.0) {
+            sc->meanSub[0] = (float)(s_d[0][0] / c);
+            sc->meanSub[1] = (float)(s_d[0][1] / c);
+            sc->meanSub[2] = (float)(s_d[0][2] / c);
+        } else {
+            sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
+        }
+    }
+    if (mode & 2) sc->vmax = s_f[0];
+    if (mode & 8) {  // advance time by the step just taken
+        sc->simTime += (double)sc->dt;
+        sc->dtPrev = sc->dt;
+        sc->kdkOpen = 0;
+        sc->stepCount += 1;
+    }
+    if (mode & 4) {  // dt (and kick) of the next integrate
+        float dt;
+        if (pol.fixedDt > 0.f) {
+            dt = pol.fixedDt;
+        } else {
+            const float vmax = sc->vmax;
+            dt = pol.baseDt;
+            const float dtMax = (vmax > 1e-6f) ? (0.8f * pol.cell / vmax) : dt;
+            if (dt > dtMax) dt = dtMax;
+            if (pol.dtStiff > 0.f && dt > pol.dtStiff) dt = pol.dtStiff;
+        }
+        sc->dt = dt;
+        sc->kick = pol.leapfrog ? 0.5f * ((sc->kdkOpen ? 0.f : sc->dtPrev) + dt) : dt;
+    }
+}
+
+void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol, StepScalars* sc,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nparts, mode, pol, sc);
+}
+
+// kSubtractMeanVelDev (Sim.cu:2282-2291); the pending mean is cleared after use
+// by the last block to finish (a device-wide counter).
+__global__ void __launch_bounds__(PT) k_subtract_mean(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                     int n, StepScalars* sc) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    const float mx = sc->meanSub[0], my = sc->meanSub[1], mz = sc->meanSub[2];
+    if (i < n && pos[i].x >= 0.f) {
+        float4 v = vel[i];
+        v.x -= mx;
+        v.y -= my;
+        v.z -= mz;
+        vel[i] = v;
+    }
+}
+__global__ void k_clear_mean(StepScalars* sc) {
+    sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
+}
+void launch_subtract_mean(const float4* pos, float4* vel, int n, StepScalars* sc, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_subtract_mean, dim3(integrate_blocks(n)), dim3(PT), 0, st, pos, vel, n, sc);
+    hipLaunchKernelGGL(k_clear_mean, dim3(1), dim3(1), 0, st, sc);
+}
+
+// Velocity read-out: leapfrog state holds v(n-1/2); synchronise with half a
+// kick of the current mesh force. KD state is returned as stored. A pending
+// momentum correction is applied in both cases.
+__global__ void __launch_bounds__(PT) k_sync_velocity(const float4* __restrict__ accG, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ vel, int n, int G, int periodic,
+                                                     const StepScalars* sc, int halfKick, float4* __restrict__ out) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    float4 v = vel[i];
+    if (p.x >= 0.f) {
+        v.x -= sc->meanSub[0];
+        v.y -= sc->meanSub[1];
+        v.z -= sc->meanSub[2];
+        if (halfKick && !sc->kdkOpen) {
+            const float h = 0.5f * sc->dtPrev;
+            const float3 a = sample_acc(accG, cic_setup(p, (float)G), G, periodic);
+            v.x = fmaf(a.x, h, v.x);
+            v.y = fmaf(a.y, h, v.y);
+            v.z = fmaf(a.z, h, v.z);
+        }
+    }
+    out[i] = v;
+}
+void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G, int periodic,
+                          const StepScalars* sc, float4* velOut, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sync_velocity, dim3(integrate_blocks(n)), dim3(PT), 0, st, accG, pos, vel, n, G, periodic,
+                       sc, accG ? 1 : 0, velOut);
+}
+
+// ---- contact: kContact, Sim.cu:1030-1203 (contact term) ---------------------------
+__device__ __forceinline__ int cell_slot(int x, int y, int z, int G, int periodic, int keyMode) {
+    x = wrap_or_clamp(x, G, periodic);
+    y = wrap_or_clamp(y, G, periodic);
+    z = wrap_or_clamp(z, G, periodic);
+    return keyMode ? (int)morton3((uint32_t)x, (uint32_t)y, (uint32_t)z) : (z * G + y) * G + x;
+}
+
+__global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
+                                               int G, int periodic, int keyMode, const int32_t* __restrict__ cellStart,
+                                               const int32_t* __restrict__ cellEnd, float radius, float stiffness,
+                                               float damping, float4* __restrict__ accOut) {
+    const int i = blockIdx.x * PT + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (p.x < 0.f) {
+        accOut[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float4 v = vel[i];
+    const float d0 = radius * 2.0f;
+    const float d02 = d0 * d0;
+    const float fG = (float)G;
+    const int cx = min(max((int)(p.x * fG), 0), G - 1);
+    const int cy = min(max((int)(p.y * fG), 0), G - 1);
+    const int cz = min(max((int)(p.z * fG), 0), G - 1);
+    const int kMaxPeers = 96;
+    int seen = 0;
+    for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz)
+        for (int dy = -1; dy <= 1 && seen < kMaxPeers; ++dy)
+            for (int dx = -1; dx <= 1 && seen < kMaxPeers; ++dx) {
+                const int c = cell_slot(cx + dx, cy + dy, cz + dz, G, periodic, keyMode);
+                const int s0 = cellStart[c], e0 = cellEnd[c];
+                if (s0 < 0 || e0 <= s0) continue;
+                for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
+                    if (j == i) continue;
+                    ++seen;
+                    const float4 q = pos[j];
+                    if (q.x < 0.f) continue;
+                    float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
+                    if (periodic) {
+                        if (ex > 0.5f) ex -= 1.f; else if (ex < -0.5f) ex += 1.f;
+                        if (ey > 0.5f) ey -= 1.f; else if (ey < -0.5f) ey += 1.f;
+                        if (ez > 0.5f) ez -= 1.f; else if (ez < -0.5f) ez += 1.f;
+                    }
+                    const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                    if (dd >= d02 || dd < 1e-16f) continue;
+                    const float d = sqrtf(dd);
+                    const float nx = ex / d, ny = ey / d, nz = ez / d;
+                    const float4 vj = vel[j];
+                    const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
+                    float f = 0.f;
+                    if (d < d0) {
+                        float fc = fmaf(stiffness, d0 - d, -(damping * vn));
+                        if (fc < 0.f) fc = 0.f;
+                        f += fc;
+                    }
+                    ax = fmaf(f, nx, ax);
+                    ay = fmaf(f, ny, ay);
+                    az = fmaf(f, nz, az);
+                }
+            }
+    accOut[i] = make_float4(ax, ay, az, 0.f);
+}
+
+void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
+                    const int32_t* cellStart, const int32_t* cellEnd, float radius, float stiffness, float damping,
+                    float4* accOut, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_contact, dim3(integrate_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
+                       cellStart, cellEnd, radius, stiffness, damping, accOut);
+}
+
+// ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------
+constexpr int DS_CHUNK = 16384;  // sources per block
+
+__global__ void __launch_bounds__(256) k_direct_partial(const float4* __restrict__ pos, int n,
+                                                       const int32_t* __restrict__ targets, int nt, float eps2,
+                                                       double* __restrict__ partial) {
+    __shared__ float4 s_src[256];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < nt) p = pos[targets[t]];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const int j0 = chunk * DS_CHUNK, j1 = min(n, j0 + DS_CHUNK);
+    for (int base = j0; base < j1; base += 256) {
+        const int j = base + threadIdx.x;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // w = 0 -> no contribution
+        if (j < j1) {
+            q = pos[j];
+            q.w = (q.x < 0.f) ? 0.f : 1.f;
+        }
+        s_src[threadIdx.x] = q;
+        __syncthreads();
+        const int lim = min(256, j1 - base);
+        for (int k = 0; k < lim; ++k) {
+            const float4 s = s_src[k];
+            const float dx = s.x - p.x, dy = s.y - p.y, dz = s.z - p.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float inv = rsqrtf(r2);
+            const float f = s.w * inv * inv * inv;
+            ax = fmaf(dx, f, ax);
+            ay = fmaf(dy, f, ay);
+            az = fmaf(dz, f, az);
+        }
+        __syncthreads();
+    }
+    if (t < nt) {
+        double* o = partial + ((size_t)chunk * nt + t) * 3;
+        o[0] = ax;
+        o[1] = ay;
+        o[2] = az;
+    }
+}
+
+__global__ void k_direct_reduce(const double* __restrict__ partial, int nt, int nchunks, double gm,
+                                double* __restrict__ out3) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nchunks; ++c)
+        for (int q = 0; q < 3; ++q) s[q] += partial[((size_t)c * nt + t) * 3 + q];
+    for (int q = 0; q < 3; ++q) out3[(size_t)t * 3 + q] = gm * s[q];
+}
+
+int direct_chunks_for(int n) { return (n + DS_CHUNK - 1) / DS_CHUNK; }
+
+void launch_direct_sum(const float4* pos, int n, const int32_t* targets, int nt, float eps2, double* partial,
+                       int nchunks, double* out3, double gm, hipStream_t st) {
+    if (nt <= 0) return;
+    hipLaunchKernelGGL(k_direct_partial, dim3((nt + 255) / 256, nchunks), dim3(256), 0, st, pos, n, targets, nt,
+                       eps2, partial);
+    hipLaunchKernelGGL(k_direct_reduce, dim3((nt + 255) / 256), dim3(256), 0, st, partial, nt, nchunks, gm, out3);
+}
+
+}  // namespace sd

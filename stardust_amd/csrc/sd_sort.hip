@@ -1,0 +1,297 @@
+// sd_sort.hip -- cell keys, stable LSD radix sort, reorder and cell ranges.
+//
+// Replaces Impl::sortParticles (src/sim/Sim.cu:3656-3671): kCellKey
+// (2476-2486), cub::DeviceRadixSort::SortPairs (3660-3667) and kReorder
+// (2488-2495); and the contact cell table kFillInt + kBuildCellRange
+// (1004-1011, 3689-3691).
+//
+// Key modes: 0 row-major cell (kCellKey), 1 Morton index of the cell
+// (contact hash grid), 2 Morton index of the CIC base node floor(p G - 0.5)
+// clamped to [0, G-1] (step order: particles sharing a deposit / gather
+// stencil are contiguous).
+// Keys are unsigned "reference key + 1" (inactive -> 0), so an unsigned sort
+// reproduces CUB's signed order (the -1 keys first) and only
+// ceil(log2(maxKey + 1)) bits are sorted. Each 8-bit pass is three launches:
+//   hist    : per-tile digit counts (wavefront ballot match -> one LDS add per
+//             run of equal digits) written digit-major,
+//   scan    : exclusive scan of the digit-major table (reduce / top / down),
+//   scatter : stable in-tile ranks (same ballot match, running per-digit
+//             counters across rounds and waves) -> global positions.
+#include "sd_device.h"
+#include "sd_internal.h"
+
+namespace sd {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ROUNDS = 16;  // keys per thread per tile
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;
+constexpr int SCAN_CHUNK = 1024;  // hist entries per scan block
+constexpr int WAVES = SORT_THREADS / 64;
+
+__global__ void __launch_bounds__(256) k_cell_keys(const float4* __restrict__ pos, int n, int G, int mode,
+                                                   uint32_t* __restrict__ keys, int32_t* __restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    idx[i] = i;
+    if (p.x < 0.f) {
+        keys[i] = 0u;
+        return;
+    }
+    const float fG = (float)G;
+    int cx, cy, cz;
+    if (mode == 2) {
+        // CIC base node floor(p G - 0.5) (the deposit / gather stencil origin)
+        cx = min(max((int)floorf(fmaf(p.x, fG, -0.5f)), 0), G - 1);
+        cy = min(max((int)floorf(fmaf(p.y, fG, -0.5f)), 0), G - 1);
+        cz = min(max((int)floorf(fmaf(p.z, fG, -0.5f)), 0), G - 1);
+    } else {
+        cx = min(max((int)(p.x * fG), 0), G - 1);
+        cy = min(max((int)(p.y * fG), 0), G - 1);
+        cz = min(max((int)(p.z * fG), 0), G - 1);
+    }
+    uint32_t k = mode ? morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)
+                      : (uint32_t)((cz * G + cy) * G + cx);
+    keys[i] = k + 1u;
+}
+
+// Lanes of this wavefront whose digit equals mine (8-bit digits).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint64_t bb = __ballot((d >> b) & 1u);
+        m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                            uint32_t* __restrict__ hist, int ntiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int i = base + r * SORT_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
+        const uint64_t active = __ballot(valid);
+        if (valid) {
+            const uint64_t m = match_digit(d, active);
+            const int lane = threadIdx.x & 63;
+            // highest lane of the group adds the group size
+            if ((m >> lane) == 1ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Block-wide exclusive scan of SCAN_CHUNK entries (4 per thread), returns total.
+__device__ uint32_t block_scan_excl(uint32_t* v4, uint32_t* s_wave) {
+    uint32_t loc = v4[0] + v4[1] + v4[2] + v4[3];
+    // inclusive wave scan
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int k = 0; k < WAVES; ++k) {
+        if (k < w) wbase += s_wave[k];
+        total += s_wave[k];
+    }
+    __syncthreads();
+    uint32_t run = wbase + x - loc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t t = v4[k];
+        v4[k] = run;
+        run += t;
+    }
+    return total;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ h, int total,
+                                                     uint32_t* __restrict__ chunk) {
+    __shared__ uint32_t s_wave[WAVES];
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4;
+    uint32_t v4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v4[k] = (base + k < total) ? h[base + k] : 0u;
+    const uint32_t t = block_scan_excl(v4, s_wave);
+    if (threadIdx.x == 0) chunk[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_scan_top(uint32_t* __restrict__ chunk, int nchunks) {
+    __shared__ uint32_t s_wave[WAVES];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int off = 0; off < nchunks; off += SCAN_CHUNK) {
+        uint32_t v4[4];
+        const int base = off + threadIdx.x * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v4[k] = (base + k < nchunks) ? chunk[base + k] : 0u;
+        const uint32_t t = block_scan_excl(v4, s_wave);
+        const uint32_t carry = s_carry;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (base + k < nchunks) chunk[base + k] = v4[k] + carry;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + t;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(uint32_t* __restrict__ h, int total,
+                                                   const uint32_t* __restrict__ chunk) {
+    __shared__ uint32_t s_wave[WAVES];
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4;
+    uint32_t v4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v4[k] = (base + k < total) ? h[base + k] : 0u;
+    block_scan_excl(v4, s_wave);
+    const uint32_t c = chunk[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < total) h[base + k] = v4[k] + c;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* __restrict__ keys,
+                                                               const int32_t* __restrict__ vals, int n, int shift,
+                                                               const uint32_t* __restrict__ offs, int ntiles,
+                                                               uint32_t* __restrict__ keysOut,
+                                                               int32_t* __restrict__ valsOut) {
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wcnt[WAVES][256];
+    __shared__ uint32_t s_wbase[WAVES][256];
+    const int t = threadIdx.x;
+    s_off[t] = offs[(size_t)t * ntiles + blockIdx.x];
+    s_run[t] = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s_wcnt[w][t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    const int lane = t & 63, wv = t >> 6;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        const bool valid = i < n;
+        uint32_t key = 0;
+        int32_t val = 0;
+        if (valid) {
+            key = keys[i];
+            val = vals[i];
+        }
+        const uint32_t d = (key >> shift) & 255u;
+        const uint64_t active = __ballot(valid);
+        uint32_t rank = 0;
+        if (valid) {
+            const uint64_t m = match_digit(d, active);
+            rank = (uint32_t)__popcll(m & lanemask_lt());
+            if ((m >> lane) == 1ull) s_wcnt[wv][d] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        {
+            uint32_t run = s_run[t];
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                s_wbase[w][t] = run;
+                run += s_wcnt[w][t];
+                s_wcnt[w][t] = 0;
+            }
+            s_run[t] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = s_off[d] + s_wbase[wv][d] + rank;
+            keysOut[dst] = key;
+            valsOut[dst] = val;
+        }
+        // s_wbase is rewritten next round only after the barrier that follows
+        // the next round's counting, so no extra barrier is needed here.
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP, const float4* __restrict__ srcV,
+                                                 const int32_t* __restrict__ order, float4* __restrict__ dstP,
+                                                 float4* __restrict__ dstV, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = order[i];
+    dstP[i] = srcP[j];
+    dstV[i] = srcV[j];
+}
+
+__global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
+                                                    int32_t* __restrict__ start, int32_t* __restrict__ end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k == 0u) return;
+    const int c = (int)(k - 1u);
+    if (i == 0 || keys[i - 1] != k) start[c] = i;
+    if (i == n - 1 || keys[i + 1] != k) end[c] = i + 1;
+}
+
+// ---------------------------------------------------------------------------
+void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
+                      hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pos, n, G, mode, keys, idx);
+}
+
+int sort_tiles_for(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+int sort_chunks_for(int n) {
+    const int total = 256 * sort_tiles_for(n);
+    return (total + SCAN_CHUNK - 1) / SCAN_CHUNK;
+}
+
+void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
+    if (n <= 1) return;
+    const int ntiles = sort_tiles_for(n);
+    const int total = 256 * ntiles;
+    const int nchunks = (total + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
+                           buf.hist, ntiles);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, buf.chunk, nchunks);
+        hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
+                           shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
+        uint32_t* tk = buf.keys;
+        buf.keys = buf.keysAlt;
+        buf.keysAlt = tk;
+        int32_t* ti = buf.idx;
+        buf.idx = buf.idxAlt;
+        buf.idxAlt = ti;
+    }
+}
+
+void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
+                    float4* dstV, int n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, st, srcP, srcV, order, dstP, dstV, n);
+}
+
+void launch_cell_range(const uint32_t* keys, int n, int ncells, int32_t* start, int32_t* end,
+                       hipStream_t st) {
+    launch_fill_i32(start, ncells, -1, st);
+    launch_fill_i32(end, ncells, -1, st);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, start, end);
+}
+
+}  // namespace sd

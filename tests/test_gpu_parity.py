@@ -1,0 +1,324 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: bit-exact for keys, sort permutation and the fixed-point density; FP32
+tolerances (stated per test) for the FFT potential, accelerations and
+trajectories, where the oracle solves the mesh in FP64.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import stardust_amd as sd
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def random_particles(n, seed, inactive=0.05, lo=0.05, hi=0.95, clustered=True):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    if clustered:
+        # half uniform, half in a few tight clumps (exercises contention)
+        m = n // 2
+        pos[:m, :3] = rng.uniform(lo, hi, (m, 3))
+        centres = rng.uniform(0.3, 0.7, (4, 3))
+        pick = rng.integers(0, 4, n - m)
+        pos[m:, :3] = centres[pick] + rng.normal(0, 0.02, (n - m, 3))
+    else:
+        pos[:, :3] = rng.uniform(lo, hi, (n, 3))
+    pos[:, :3] = np.clip(pos[:, :3], 0.003, 0.997)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0, 0.05, (n, 3))
+    dead = rng.random(n) < inactive
+    pos[dead] = (-1, -1, -1, 0)
+    vel[dead] = 0
+    return pos, vel
+
+
+def make_sim(n, G, periodic=False, soft=1.5, gravity=1.0, sort_interval=0, **modes):
+    cfg = sd.default_config(particleCount=n, gridSize=G,
+                            boundary=sd.BOUNDARY_PERIODIC if periodic else sd.BOUNDARY_ISOLATED,
+                            softeningCells=soft, gravity=gravity, sortInterval=sort_interval)
+    m = sd.default_modes(**modes)
+    return sd.Sim(cfg, m)
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("key_mode", [sd.KEY_ROWMAJOR, sd.KEY_MORTON])
+@pytest.mark.parametrize("n,G", [(1, 16), (777, 16), (20000, 32), (300000, 128)])
+def test_sort_bit_exact(gpu, oracle, key_mode, n, G):
+    _say("sort", key_mode, n, G)
+    pos, vel = random_particles(n, seed=n + G)
+    s = make_sim(n, G, keyMode=key_mode)
+    s.set_particles(pos, vel)
+    keys, order = s.sort_now()
+    # the step sort of Morton mode orders by the CIC base node
+    ok, oi = oracle.cell_keys(pos, G, oracle.KEY_MORTON_NODE if key_mode else oracle.KEY_ROWMAJOR)
+    sk, so = oracle.stable_sort_pairs(ok, oi)
+    assert np.array_equal(keys, sk)
+    assert np.array_equal(order, so)
+    p2, v2 = s.get_particles()
+    assert np.array_equal(p2, pos[so])
+    s.close()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("n,G", [(5000, 16), (100000, 32), (400000, 64)])
+def test_deposit_fixed_bit_exact(gpu, oracle, periodic, n, G):
+    _say("deposit", periodic, n, G)
+    pos, vel = random_particles(n, seed=3 * n + G, lo=-0.02 if periodic else 0.05)
+    if periodic:
+        pos[:, :3] = np.where(pos[:, 0:1] >= 0, np.mod(pos[:, :3], 1.0), pos[:, :3])
+    s = make_sim(n, G, periodic=periodic)
+    s.set_particles(pos, vel)
+    rho = s.get_grid(sd.GRID_DENSITY_FIXED)
+    ref = oracle.deposit_fixed(pos, G, periodic)
+    assert np.array_equal(rho, ref)
+    live = int((pos[:, 0] >= 0).sum())
+    assert int(rho.sum()) == live * sd.FIX_ONE  # exact mass
+    # against the reference's float deposit (order-dependent): its sequential
+    # FP32 sums carry ~count * 2^-24 relative error, the fixed point 2^-24 per particle
+    rf = oracle.deposit_float(pos, G, G, periodic)
+    assert np.abs(rho / sd.FIX_ONE - rf).max() < 1e-5 * max(1.0, rf.max())
+    s.close()
+
+
+@pytest.mark.parametrize("periodic,G", [(False, 8), (False, 16), (False, 32), (False, 64),
+                                        (True, 16), (True, 32), (True, 64)])
+def test_potential_and_accel(gpu, oracle, periodic, G):
+    _say("mesh", periodic, G)
+    n = 20000
+    pos, vel = random_particles(n, seed=G + 11)
+    if periodic:
+        pos[:, :3] = np.where(pos[:, 0:1] >= 0, np.mod(pos[:, :3], 1.0), pos[:, :3])
+    soft = 1.5
+    gravity = 0.8
+    s = make_sim(n, G, periodic=periodic, soft=soft, gravity=gravity, exactGrid=1)
+    s.set_particles(pos, vel)
+    s.compute_forces()
+    pot = s.get_grid(sd.GRID_POTENTIAL).astype(np.float64)
+    acc = s.get_grid(sd.GRID_ACCEL)
+    rho = oracle.deposit_fixed(pos, G, periodic).astype(np.float64) / oracle.FIX_ONE
+    if periodic:
+        pref = oracle.poisson_periodic(rho, soft)
+    else:
+        pref = oracle.poisson_iso(rho, soft)
+    scale = np.abs(pref).max()
+    err = np.abs(pot - pref).max() / scale
+    _say("  pot rel max err", err)
+    assert err < 2e-5  # FP32 transforms vs FP64
+    potScale = np.float32(gravity) / np.float32(n)
+    aref = oracle.grid_accel(pref, float(potScale), periodic)
+    d = np.linalg.norm(acc[..., :3] - aref[..., :3], axis=-1)
+    rms = np.sqrt((d ** 2).mean()) / np.sqrt((np.linalg.norm(aref[..., :3], axis=-1) ** 2).mean())
+    _say("  acc rel rms err", rms)
+    assert rms < 1e-4
+    s.close()
+
+
+def test_potential_point_mass(gpu):
+    """Isolated Green's function: a point mass at a cell centre gives
+    pot = -1/(r_cells + soft) exactly at every node (no periodic images)."""
+    G = 32
+    n = 1
+    pos = np.array([[(10 + 0.5) / G, (12 + 0.5) / G, (9 + 0.5) / G, 0]], np.float32)
+    vel = np.zeros((1, 4), np.float32)
+    s = make_sim(n, G, soft=2.0)
+    s.set_particles(pos, vel)
+    pot = s.get_grid(sd.GRID_POTENTIAL).astype(np.float64)
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    r = np.sqrt((x - 10.0) ** 2 + (y - 12.0) ** 2 + (z - 9.0) ** 2)
+    ref = -1.0 / (r + 2.0)
+    assert np.abs(pot - ref).max() < 2e-6
+    s.close()
+
+
+def _run_both(pos, vel, G, steps, periodic=False, soft=1.5, gravity=0.5, fixed_dt=0.0,
+              sort_interval=3, pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
+              key_mode=sd.KEY_ROWMAJOR, momentum=1, oracle=None):
+    n = pos.shape[0]
+    s = make_sim(n, G, periodic=periodic, soft=soft, gravity=gravity, sort_interval=sort_interval,
+                 pipeline=pipeline, integrator=integrator, keyMode=key_mode, momentumZero=momentum)
+    s.set_fixed_dt(fixed_dt)
+    s.set_particles(pos, vel)
+    s.run(steps)
+    gp, gv = s.get_particles()
+    t = s.sim_time()
+    s.close()
+    prm = oracle.make_params(G, periodic=periodic, gravity=gravity, softeningCells=soft,
+                             sortInterval=sort_interval, keyMode=key_mode, fixedDt=fixed_dt,
+                             momentumZero=momentum, allocN=n)
+    st = oracle.Stepper(prm, pos, vel)
+    st.step(steps)
+    return gp, gv, t, st
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("fixed_dt", [0.0, 2e-3])
+def test_reference_step_vs_oracle(gpu, oracle, periodic, fixed_dt):
+    _say("step", periodic, fixed_dt)
+    n, G, steps = 6000, 32, 6
+    pos, vel = random_particles(n, seed=5)
+    if periodic:
+        pos[:, :3] = np.where(pos[:, 0:1] >= 0, np.mod(pos[:, :3], 1.0), pos[:, :3])
+    gp, gv, t, st = _run_both(pos, vel, G, steps, periodic=periodic, fixed_dt=fixed_dt,
+                              oracle=oracle)
+    # same permutation (row-major keys, same sort schedule)
+    assert np.array_equal(gp[:, 0] < 0, st.pos[:, 0] < 0)
+    dp = np.abs(gp[:, :3] - st.pos[:, :3]).max()
+    vs = np.abs(st.vel[:, :3]).max()
+    dv = np.abs(gv[:, :3] - st.vel[:, :3]).max() / vs
+    _say("  max |dx|", dp, "max rel dv", dv)
+    assert dp < 2e-6
+    assert dv < 1e-4
+    assert abs(t - st.state.simTime) < 1e-6 * max(1.0, st.state.simTime)
+
+
+@pytest.mark.parametrize("integrator", [sd.INTEGRATOR_KD, sd.INTEGRATOR_KDK])
+def test_fused_pipeline_matches_reference_order(gpu, oracle, integrator):
+    """The fused pipeline (deposit and reductions inside integrate, Morton
+    keys) reproduces the reference-order pipeline under a fixed dt."""
+    _say("fused", integrator)
+    n, G, steps = 20000, 32, 8
+    pos, vel = random_particles(n, seed=7, inactive=0.0)
+    pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)  # identity carried in w
+    out = []
+    for pipe, key in ((sd.PIPELINE_REFERENCE, sd.KEY_ROWMAJOR), (sd.PIPELINE_FUSED, sd.KEY_MORTON)):
+        s = make_sim(n, G, soft=1.5, gravity=0.5, sort_interval=4, pipeline=pipe,
+                     integrator=integrator, keyMode=key)
+        s.set_fixed_dt(2e-3)
+        s.set_particles(pos, vel)
+        s.run(steps)
+        p, v = s.get_particles()
+        o = np.argsort(p[:, 3])
+        out.append((p[o], v[o], s.sim_time()))
+        s.close()
+    (p0, v0, t0), (p1, v1, t1) = out
+    assert np.abs(p0[:, :3] - p1[:, :3]).max() < 1e-6
+    assert np.abs(v0[:, :3] - v1[:, :3]).max() < 1e-5 * np.abs(v0[:, :3]).max()
+    assert abs(t0 - t1) < 1e-9
+
+
+def test_kdk_equals_shifted_kd(gpu):
+    """Leapfrog from v0 equals the kick-drift scheme started from
+    v0 - a(x0) dt / 2 (positions), a closed-form relation of the two."""
+    n, G, steps, dt = 8000, 32, 5, 2e-3
+    pos, vel = random_particles(n, seed=9, inactive=0.0)
+    s = make_sim(n, G, soft=1.5, gravity=0.5, sort_interval=0, integrator=sd.INTEGRATOR_KDK)
+    s.set_fixed_dt(dt)
+    s.set_particles(pos, vel)
+    a0 = s.particle_accel()
+    s.run(steps)
+    pk, vk = s.get_particles()
+    s.close()
+    v0s = vel.copy()
+    v0s[:, :3] -= 0.5 * dt * a0[:, :3]
+    s = make_sim(n, G, soft=1.5, gravity=0.5, sort_interval=0, integrator=sd.INTEGRATOR_KD,
+                 momentumZero=1)
+    s.set_fixed_dt(dt)
+    s.set_particles(pos, v0s)
+    s.run(steps)
+    pd, vd = s.get_particles()
+    s.close()
+    assert np.abs(pk[:, :3] - pd[:, :3]).max() < 1e-6
+
+
+def test_contact_vs_oracle(gpu, oracle):
+    _say("contact")
+    n, G = 4000, 16
+    rng = np.random.default_rng(3)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.3, 0.7, (n, 3))
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0, 0.01, (n, 3))
+    cfg = sd.default_config(particleCount=n, gridSize=G, softeningCells=1.0, gravity=0.1,
+                            sortInterval=1, contactEnabled=1, contactRadiusCells=0.5,
+                            contactStiffness=1e3)
+    for key in (sd.KEY_ROWMAJOR, sd.KEY_MORTON):
+        s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
+                                         keyMode=key))
+        s.set_particles(pos, vel)
+        s.run(3)
+        gp, gv = s.get_particles()
+        s.close()
+        prm = oracle.make_params(G, gravity=0.1, softeningCells=1.0, sortInterval=1, keyMode=key,
+                                 contact=1, contactRadiusCells=0.5, contactStiffness=1e3, allocN=n)
+        st = oracle.Stepper(prm, pos, vel)
+        st.step(3)
+        assert np.abs(gp[:, :3] - st.pos[:, :3]).max() < 2e-6
+        assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
+
+
+def test_direct_sum_vs_fp64(gpu, oracle):
+    n = 50000
+    pos, vel = random_particles(n, seed=21)
+    s = make_sim(n, 32)
+    s.set_particles(pos, vel)
+    rng = np.random.default_rng(1)
+    live = np.nonzero(pos[:, 0] >= 0)[0]
+    targets = rng.choice(live, 512, replace=False).astype(np.int32)
+    gm, eps = 1.0 / n, 0.01
+    a = s.direct_sum(targets, gm, eps)
+    ref = oracle.direct_sum(pos, targets, gm, eps)
+    err = np.linalg.norm(a - ref, axis=1).max() / np.linalg.norm(ref, axis=1).max()
+    assert err < 2e-5
+    s.close()
+
+
+@pytest.mark.parametrize("G", [32, 64, 128])
+def test_mass_conservation_50_steps(gpu, G):
+    """sim_tests.cpp:31-52 (total grid mass stays N); exact here."""
+    n = 50000
+    pos, vel = random_particles(n, seed=G, inactive=0.0)
+    s = make_sim(n, G, soft=3.0 * G / 128, gravity=0.6, sort_interval=40)
+    s.set_particles(pos, vel)
+    s.run(50)
+    assert s.measure_total_grid_mass() == pytest.approx(n, abs=0)
+    assert s.active_count() == n
+    s.close()
+
+
+def test_state_file_roundtrip(gpu, tmp_path):
+    n, G = 3000, 16
+    pos, vel = random_particles(n, seed=4, inactive=0.0)
+    s = make_sim(n, G, integrator=sd.INTEGRATOR_KD)
+    s.set_particles(pos, vel)
+    s.run(3)
+    p, v = s.get_particles()
+    path = str(tmp_path / "state.bin")
+    s.save_state(path)
+    t = s.sim_time()
+    s2 = sd.Sim(modes=sd.default_modes(integrator=sd.INTEGRATOR_KD))
+    s2.init(sd.default_config(particleCount=10, gridSize=32))
+    s2.load_state(path)
+    p2, v2 = s2.get_particles()
+    assert s2.grid_size() == G and s2.particle_count() == n
+    assert np.array_equal(p, p2) and np.array_equal(v, v2)
+    assert s2.sim_time() == pytest.approx(t, rel=1e-6)
+    s.close()
+    s2.close()
+
+
+def test_determinism_and_measures(gpu):
+    n, G = 200000, 64
+    pos, vel = random_particles(n, seed=8)
+    rhos = []
+    for _ in range(2):
+        s = make_sim(n, G, sort_interval=5)
+        s.set_fixed_dt(1e-3)
+        s.set_particles(pos, vel)
+        s.run(12)
+        rhos.append(s.get_grid(sd.GRID_DENSITY_FIXED))
+        p, _ = s.get_particles()
+        live = p[:, 0] >= 0
+        cx, cy = s.measure_centroid()
+        assert cx == pytest.approx(float(p[live, 0].astype(np.float64).mean()), abs=1e-9)
+        assert s.measure_occupied_cells() == int((rhos[-1] > sd.FIX_ONE * 1e-6).sum())
+        assert s.peak_cell_count() > 0
+        s.close()
+    assert np.array_equal(rhos[0], rhos[1])
