@@ -94,7 +94,8 @@ struct sd_sim {
 
     // reductions
     ReducePartial* partials = nullptr;
-    int nparts = 0;
+    int nparts = 0;     // one-particle-per-thread reductions
+    int npartsInt = 0;  // k_integrate partials
     StepScalars* sc = nullptr;
     StepScalars* scHost = nullptr;  // pinned mirror for reads
 
@@ -283,7 +284,8 @@ static int allocate(sd_sim* s) {
     ok = ok && dalloc(s->rhoFix, cells) && dalloc(s->A, (size_t)G * G * P) && dalloc(s->B, (size_t)G * S * P) &&
          dalloc(s->pot, cells) && dalloc(s->accG, cells) && dalloc(s->tw, (size_t)S);
     if (!s->periodic) ok = ok && dalloc(s->green, (size_t)(S / 2 + 1) * (S / 2 + 1) * P);
-    s->nparts = integrate_blocks((int)N);
+    s->nparts = particle_blocks((int)N);
+    s->npartsInt = integrate_blocks((int)N);
     ok = ok && dalloc(s->partials, (size_t)s->nparts) && dalloc(s->sc, 1);
     if (ok && hipHostMalloc((void**)&s->scHost, sizeof(StepScalars), hipHostMallocDefault) != hipSuccess) {
         mark_failure("hipHostMalloc failed");
@@ -509,7 +511,7 @@ static void enqueue_step(sd_sim* s) {
         a.partials = s->partials;
         TIMED(s, "integrate", launch_integrate(a, s->st));
         const int mode = (s->modes.momentumZero ? 1 : 0) | 2 | 8 | 4;
-        TIMED(s, "finalize", launch_finalize(s->partials, s->nparts, mode, pol, s->sc, s->st));
+        TIMED(s, "finalize", launch_finalize(s->partials, s->npartsInt, mode, pol, s->sc, s->st));
         s->primed = true;
         s->rhoValid = fusedDeposit;
     } else {

@@ -76,7 +76,8 @@ struct IntegrateArgs {
     long long* depositTo; // fused deposit of the new positions (nullptr = off)
     ReducePartial* partials;
 };
-int integrate_blocks(int n);
+int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
+int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)
 void launch_integrate(const IntegrateArgs& a, hipStream_t st);
 
 // reductions / scalars

@@ -86,15 +86,28 @@ __device__ __forceinline__ void fixed_weights(const Cic& c, long long w[8]) {
     }
 }
 
-// Deterministic deposit of one wavefront's particles. Lanes whose CIC base
-// node matches the previous lane's form a run (particles sorted by the Morton
-// index of that node make runs long); an inclusive segmented scan over the
-// wavefront sums each run's eight integer weights, and only the last lane of a
-// run issues the eight global atomics. Integer sums are exact, so the grid is
-// bit-identical whatever the order. Must be called by all 64 lanes.
-__device__ __forceinline__ void deposit_wave(bool alive, const Cic& c, int G, int periodic, long long* rho) {
+// ---- deterministic deposit ------------------------------------------------------
+// Three levels, all in exact integer arithmetic (so the grid is bit-identical
+// whatever the order or the level that handled a particle):
+//  1. wavefront: lanes whose CIC base node equals the previous lane's form a
+//     run (the step sort orders particles by the Morton index of that node);
+//     an inclusive segmented scan sums each run's eight weights into its last
+//     lane;
+//  2. block: run tails add into an LDS tile spanning the block's bounding box
+//     of base nodes (+1 for the upper stencil corner);
+//  3. grid: each non-zero tile cell is added to the global int64 grid once.
+// Blocks whose box exceeds DEP_CAP cells skip level 2 and add run sums to the
+// global grid directly. Same-address global atomics from consecutive waves
+// were the bottleneck of a per-particle deposit; a block of KP particles
+// issues one per touched cell instead.
+constexpr int DEP_CAP = 2048;  // LDS tile cells (16 KiB)
+constexpr int KP = 4;          // particles per thread in integrate / deposit
+constexpr int BP = PT * KP;    // particles per block
+
+// Segmented scan over the wavefront; returns true on the lane that holds a
+// run's total in v[8].
+__device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, long long v[8]) {
     const int lane = threadIdx.x & 63;
-    long long v[8];
     if (alive) {
         fixed_weights(c, v);
     } else {
@@ -117,7 +130,11 @@ __device__ __forceinline__ void deposit_wave(bool alive, const Cic& c, int G, in
             if (take) v[k] += y;
         }
     }
-    if (!(alive && tail)) return;
+    return alive && tail;
+}
+
+__device__ __forceinline__ void add_global8(const Cic& c, const long long v[8], int G, int periodic,
+                                            long long* rho) {
     int xs[2], ys[2], zs[2];
     xs[0] = wrap_or_clamp(c.ix, G, periodic);
     xs[1] = wrap_or_clamp(c.ix + 1, G, periodic);
@@ -135,17 +152,99 @@ __device__ __forceinline__ void deposit_wave(bool alive, const Cic& c, int G, in
     }
 }
 
-__global__ void __launch_bounds__(PT) k_deposit_fixed(const float4* __restrict__ pos, int n, int G, int periodic,
-                                                     long long* __restrict__ rho) {
-    const int i = blockIdx.x * PT + threadIdx.x;
-    float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
-    if (i < n) p = pos[i];
-    const bool alive = p.x >= 0.f;
-    Cic c = cic_setup(alive ? p : make_float4(0.f, 0.f, 0.f, 0.f), (float)G);
-    deposit_wave(alive, c, G, periodic, rho);
+// Block-wide deposit of KP particles per thread. Must be called by every
+// thread of the block (it synchronises).
+__device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[KP], int G, int periodic,
+                                              long long* rho) {
+    __shared__ unsigned long long s_tile[DEP_CAP];
+    __shared__ int s_box[PT / 64][6];
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        if (alive[k]) {
+            lo[0] = min(lo[0], c[k].ix); hi[0] = max(hi[0], c[k].ix);
+            lo[1] = min(lo[1], c[k].iy); hi[1] = max(hi[1], c[k].iy);
+            lo[2] = min(lo[2], c[k].iz); hi[2] = max(hi[2], c[k].iz);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int d = 0; d < 3; ++d) {
+            s_box[w][d] = lo[d];
+            s_box[w][3 + d] = hi[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PT / 64; ++q) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], s_box[q][d]);
+            hi[d] = max(hi[d], s_box[q][3 + d]);
+        }
+    }
+    if (lo[0] == INT_MAX) return;  // no live particle in the block (uniform)
+    const long long dx = (long long)hi[0] - lo[0] + 2, dy = (long long)hi[1] - lo[1] + 2,
+                    dz = (long long)hi[2] - lo[2] + 2;
+    const long long vol = dx * dy * dz;
+    if (vol > DEP_CAP) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            long long v[8];
+            if (wave_runs(alive[k], c[k], v)) add_global8(c[k], v, G, periodic, rho);
+        }
+        return;
+    }
+    const int nv = (int)vol, ndx = (int)dx, ndy = (int)dy;
+    for (int t = threadIdx.x; t < nv; t += PT) s_tile[t] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        long long v[8];
+        if (wave_runs(alive[k], c[k], v)) {
+            const int bx = c[k].ix - lo[0], by = c[k].iy - lo[1], bz = c[k].iz - lo[2];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int ox = q & 1, oy = (q >> 1) & 1, oz = (q >> 2) & 1;
+                if (v[q] != 0)
+                    atomicAdd(&s_tile[((bz + oz) * ndy + (by + oy)) * ndx + (bx + ox)], (unsigned long long)v[q]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv; t += PT) {
+        const unsigned long long v = s_tile[t];
+        if (v == 0ull) continue;
+        const int x = t % ndx, y = (t / ndx) % ndy, z = t / (ndx * ndy);
+        const int gx = wrap_or_clamp(lo[0] + x, G, periodic), gy = wrap_or_clamp(lo[1] + y, G, periodic),
+                  gz = wrap_or_clamp(lo[2] + z, G, periodic);
+        atomicAdd((unsigned long long*)(rho + ((size_t)gz * G + gy) * G + gx), v);
+    }
 }
 
-// Reference float-atomic deposit (order-dependent) for parity runs.
+__global__ void __launch_bounds__(PT) k_deposit_fixed(const float4* __restrict__ pos, int n, int G, int periodic,
+                                                     long long* __restrict__ rho) {
+    bool alive[KP];
+    Cic c[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int i = blockIdx.x * BP + k * PT + threadIdx.x;
+        float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
+        if (i < n) p = pos[i];
+        alive[k] = p.x >= 0.f;
+        c[k] = cic_setup(alive[k] ? p : make_float4(0.f, 0.f, 0.f, 0.f), (float)G);
+    }
+    block_deposit(alive, c, G, periodic, rho);
+}
+
+// Reference float-atomic deposit (kScatter, order-dependent) for parity runs.
 __global__ void __launch_bounds__(PT) k_deposit_float(const float4* __restrict__ pos, int n, int G, int periodic,
                                                      float* __restrict__ rho) {
     const int i = blockIdx.x * PT + threadIdx.x;
@@ -164,7 +263,7 @@ __global__ void __launch_bounds__(PT) k_deposit_float(const float4* __restrict__
 }
 
 void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long long* rho, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_deposit_fixed, dim3((n + PT - 1) / PT), dim3(PT), 0, st, pos, n, G, periodic, rho);
+    if (n > 0) hipLaunchKernelGGL(k_deposit_fixed, dim3((n + BP - 1) / BP), dim3(PT), 0, st, pos, n, G, periodic, rho);
 }
 void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* rho, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_deposit_float, dim3((n + PT - 1) / PT), dim3(PT), 0, st, pos, n, G, periodic, rho);
@@ -228,16 +327,21 @@ __device__ __forceinline__ float speed3(float x, float y, float z) {
     return sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
 }
 
-// kIntegrate, Sim.cu:685-906 (plain PM: no black holes, no pressure).
+// kIntegrate, Sim.cu:685-906 (plain PM: no black holes, no pressure). Each
+// thread advances KP particles (i = block * BP + k * PT + thread, coalesced for
+// every k); the new positions feed the block deposit when depositTo is set.
 __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
-    const int i = blockIdx.x * PT + threadIdx.x;
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
-    bool depAlive = false;
-    Cic depC;
-    depC.ix = depC.iy = depC.iz = 0;
-    depC.fx = depC.fy = depC.fz = 0.f;
-    if (i < a.n) {
+    bool depAlive[KP];
+    Cic depC[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        depAlive[k] = false;
+        depC[k].ix = depC[k].iy = depC[k].iz = 0;
+        depC[k].fx = depC[k].fy = depC[k].fz = 0.f;
+        const int i = blockIdx.x * BP + k * PT + threadIdx.x;
+        if (i >= a.n) continue;
         float4 p = a.pos[i];
         if (p.x >= 0.f) {
             const float kick = a.sc->kick, drift = a.sc->dt;
@@ -334,21 +438,22 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
             a.pos[i] = p;
             a.vel[i] = v;
             if (alive) {
-                depAlive = true;
-                depC = cic_setup(p, fG);
-                sx = (double)v.x;
-                sy = (double)v.y;
-                sz = (double)v.z;
-                cnt = 1.0;
-                vm = speed3(v.x, v.y, v.z);
+                depAlive[k] = true;
+                depC[k] = cic_setup(p, fG);
+                sx += (double)v.x;
+                sy += (double)v.y;
+                sz += (double)v.z;
+                cnt += 1.0;
+                vm = fmaxf(vm, speed3(v.x, v.y, v.z));
             }
         }
     }
-    if (a.depositTo) deposit_wave(depAlive, depC, a.G, a.periodic, a.depositTo);
+    if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
     if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials);
 }
 
-int integrate_blocks(int n) { return (n + PT - 1) / PT; }
+int integrate_blocks(int n) { return (n + BP - 1) / BP; }
+int particle_blocks(int n) { return (n + PT - 1) / PT; }
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t st) {
     if (a.n <= 0) return;
@@ -385,20 +490,21 @@ __global__ void __launch_bounds__(PT) k_momentum(const float4* __restrict__ pos,
 }
 
 void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* partials, hipStream_t st) {
-    hipLaunchKernelGGL(k_vmax, dim3(integrate_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
+    hipLaunchKernelGGL(k_vmax, dim3(particle_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
 }
 void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial* partials, hipStream_t st) {
-    hipLaunchKernelGGL(k_momentum, dim3(integrate_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
+    hipLaunchKernelGGL(k_momentum, dim3(particle_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
 }
 
 // Deterministic reduction (fixed summation order) + dt policy.
-__global__ void __launch_bounds__(256) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
+constexpr int FIN = 1024;
+__global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
                                                   DtPolicy pol, StepScalars* sc) {
-    __shared__ double s_d[256][4];
-    __shared__ float s_f[256];
+    __shared__ double s_d[FIN][4];
+    __shared__ float s_f[FIN];
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
-    for (int k = threadIdx.x; k < np; k += 256) {
+    for (int k = threadIdx.x; k < np; k += FIN) {
         const ReducePartial r = parts[k];
         sx += r.sx;
         sy += r.sy;
@@ -412,7 +518,7 @@ __global__ void __launch_bounds__(256) k_finalize(const ReducePartial* __restric
     s_d[threadIdx.x][3] = cnt;
     s_f[threadIdx.x] = vm;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = FIN / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) s_d[threadIdx.x][q] += s_d[threadIdx.x + s][q];
@@ -457,7 +563,7 @@ __global__ void __launch_bounds__(256) k_finalize(const ReducePartial* __restric
 
 void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol, StepScalars* sc,
                      hipStream_t st) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nparts, mode, pol, sc);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN), 0, st, partials, nparts, mode, pol, sc);
 }
 
 // kSubtractMeanVelDev (Sim.cu:2282-2291); the pending mean is cleared after use
@@ -478,7 +584,7 @@ __global__ void k_clear_mean(StepScalars* sc) {
     sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
 }
 void launch_subtract_mean(const float4* pos, float4* vel, int n, StepScalars* sc, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_subtract_mean, dim3(integrate_blocks(n)), dim3(PT), 0, st, pos, vel, n, sc);
+    if (n > 0) hipLaunchKernelGGL(k_subtract_mean, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, sc);
     hipLaunchKernelGGL(k_clear_mean, dim3(1), dim3(1), 0, st, sc);
 }
 
@@ -509,7 +615,7 @@ __global__ void __launch_bounds__(PT) k_sync_velocity(const float4* __restrict__
 void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G, int periodic,
                           const StepScalars* sc, float4* velOut, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_sync_velocity, dim3(integrate_blocks(n)), dim3(PT), 0, st, accG, pos, vel, n, G, periodic,
+    hipLaunchKernelGGL(k_sync_velocity, dim3(particle_blocks(n)), dim3(PT), 0, st, accG, pos, vel, n, G, periodic,
                        sc, accG ? 1 : 0, velOut);
 }
 
@@ -583,7 +689,7 @@ void launch_contact(const float4* pos, const float4* vel, int n, int G, int peri
                     const int32_t* cellStart, const int32_t* cellEnd, float radius, float stiffness, float damping,
                     float4* accOut, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_contact, dim3(integrate_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
+    hipLaunchKernelGGL(k_contact, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
                        cellStart, cellEnd, radius, stiffness, damping, accOut);
 }
 

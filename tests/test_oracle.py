@@ -1,0 +1,392 @@
+"""Pins of the CPU oracle (no GPU): closed forms, invariants, numpy and brute
+force on small inputs. Each test names the oracle function it pins; a wrong
+sign, index or dropped term in the oracle fails at least one of them."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def rand_pos(n, seed, lo=0.05, hi=0.95):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((n, 4), np.float32)
+    p[:, :3] = rng.uniform(lo, hi, (n, 3))
+    return p
+
+
+# ---- FFT ----------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 12, 16, 24, 48, 64, 96, 128, 192, 256, 512, 5, 7])
+def test_fft1d_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(O.fft1d(x), np.fft.fft(x), atol=1e-11 * max(1, n))
+    assert np.allclose(O.fft1d(x, inverse=True), np.fft.ifft(x) * n, atol=1e-11 * max(1, n))
+
+
+def test_fft3d_matches_numpy():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((6, 12, 16)) + 1j * rng.standard_normal((6, 12, 16))
+    assert np.allclose(O.fft3d(x), np.fft.fftn(x), atol=1e-10)
+    assert np.allclose(O.fft3d(x, inverse=True), np.fft.ifftn(x) * x.size, atol=1e-10)
+
+
+# ---- Green's function and isolated Poisson ----------------------------------------
+def test_green_closed_form():
+    S, soft = 8, 1.5
+    g = O.green_real(S, soft)
+    for (z, y, x) in [(0, 0, 0), (1, 2, 3), (7, 1, 4), (4, 4, 4), (5, 6, 7)]:
+        d = [min(c, S - c) for c in (x, y, z)]
+        assert g[z, y, x] == pytest.approx(-1.0 / (math.sqrt(sum(t * t for t in d)) + soft), rel=1e-15)
+    assert np.allclose(g[1:], g[1:][::-1])  # g(S - z) = g(z)
+
+
+def test_poisson_iso_is_linear_convolution():
+    """On the G corner of the 2G padded grid the circular convolution equals
+    the direct sum pot(x) = sum_x' rho(x') * -1/(|x - x'| + soft)."""
+    G, soft = 6, 1.25
+    rng = np.random.default_rng(1)
+    rho = rng.random((G, G, G))
+    pot = O.poisson_iso(rho, soft)
+    idx = np.stack(np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij"), -1).reshape(-1, 3)
+    d = np.sqrt(((idx[:, None, :] - idx[None, :, :]) ** 2).sum(-1))
+    ref = (-1.0 / (d + soft)) @ rho.reshape(-1)
+    assert np.allclose(pot.reshape(-1), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_poisson_iso_softening_floor():
+    """buildGreen uses max(softeningCells, 0.5) (Sim.cu:3572)."""
+    G = 4
+    rho = np.zeros((G, G, G))
+    rho[1, 2, 3] = 1.0
+    pot = O.poisson_iso(rho, 0.1)
+    assert pot[1, 2, 3] == pytest.approx(-1.0 / 0.5)
+
+
+# ---- periodic Poisson -------------------------------------------------------------
+@pytest.mark.parametrize("soft", [0.0, 1.5])
+def test_poisson_periodic_single_mode(soft):
+    G = 16
+    k = np.array([2, -3, 1])  # (kx, ky, kz)
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    rho = np.cos(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) / G) + 3.0  # mean drops out
+    pot = O.poisson_periodic(rho, soft)
+    k2 = float((k ** 2).sum())
+    s = 2 * np.pi * soft / G
+    m = -1.0 / (4 * np.pi ** 2 * k2) * math.exp(-0.5 * k2 * s * s)
+    # norm 1/S^3 cancels the unnormalised inverse transform: grad^2 pot = rho - mean
+    expect = m * (rho - 3.0)
+    assert np.allclose(pot, expect, atol=1e-12)
+
+
+# ---- deposit ------------------------------------------------------------------------
+def test_fixed_weights_closed_form_and_exact_sum():
+    G = 32
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        p = rng.uniform(0.01, 0.99, 3).astype(np.float32)
+        b, w = O.cic_weights_fixed(p, G)
+        assert int(w.sum()) == O.FIX_ONE
+        g = p.astype(np.float32) * np.float32(G) - np.float32(0.5)
+        base = np.floor(g).astype(int)
+        f = (g - base).astype(np.float64)
+        assert np.array_equal(b, base)
+        for k in range(8):
+            ox, oy, oz = k & 1, (k >> 1) & 1, (k >> 2) & 1
+            wf = (f[0] if ox else 1 - f[0]) * (f[1] if oy else 1 - f[1]) * (f[2] if oz else 1 - f[2])
+            assert abs(w[k] / O.FIX_ONE - wf) < 4.0 / O.FIX_ONE
+
+
+def test_deposit_mass_and_edges():
+    G = 16
+    p = np.array([[0.001, 0.5, 0.5, 0], [0.999, 0.999, 0.001, 0], [0.5, 0.5, 0.5, 0],
+                  [-1, -1, -1, 0]], np.float32)
+    rho = O.deposit_fixed(p, G, periodic=False)
+    assert int(rho.sum()) == 3 * O.FIX_ONE  # edge mass clamped, not dropped; inactive skipped
+    rf = O.deposit_float(p, G, G, periodic=False)
+    assert rf.sum() == pytest.approx(3.0, abs=1e-5)
+    # a particle at a cell centre lands on one node only
+    c = np.array([[(5 + 0.5) / G, (6 + 0.5) / G, (7 + 0.5) / G, 0]], np.float32)
+    r1 = O.deposit_fixed(c, G)
+    assert r1[7, 6, 5] == O.FIX_ONE and int(r1.sum()) == O.FIX_ONE
+    # periodic wrap: mass beyond the last node reappears at node 0
+    w = np.array([[0.999, 0.5 + 0.5 / G, 0.5 + 0.5 / G, 0]], np.float32)
+    rp = O.deposit_fixed(w, G, periodic=True)
+    assert rp[G // 2, G // 2, 0] > 0 and rp[G // 2, G // 2, G - 1] > 0
+
+
+def test_deposit_translation_by_whole_cells():
+    G = 16
+    p = rand_pos(300, 3, 0.3, 0.5)
+    p[:, :3] = np.round(p[:, :3] * 65536) / 65536  # on a 2^-16 lattice: +3/16 is exact
+    r0 = O.deposit_fixed(p, G)
+    q = p.copy()
+    q[:, 0] += np.float32(3.0 / G)
+    r1 = O.deposit_fixed(q, G)
+    # translation by an exactly representable 3 cells moves the grid
+    assert np.array_equal(r1[:, :, 3:], r0[:, :, :-3])
+
+
+def test_fixed_vs_float_deposit():
+    G = 32
+    p = rand_pos(20000, 4)
+    a = O.deposit_fixed(p, G) / O.FIX_ONE
+    b = O.deposit_float(p, G, G)
+    assert np.abs(a - b).max() < 1e-4
+
+
+# ---- keys and sort ----------------------------------------------------------------
+def morton_np(x, y, z):
+    r = np.zeros_like(x, dtype=np.uint64)
+    for b in range(10):
+        r |= ((x.astype(np.uint64) >> b) & 1) << (3 * b)
+        r |= ((y.astype(np.uint64) >> b) & 1) << (3 * b + 1)
+        r |= ((z.astype(np.uint64) >> b) & 1) << (3 * b + 2)
+    return r.astype(np.uint32)
+
+
+@pytest.mark.parametrize("G", [16, 64, 128])
+def test_keys_closed_form(G):
+    p = rand_pos(5000, G, -0.05, 1.05)
+    p[::17] = (-1, -1, -1, 0)
+    fG = np.float32(G)
+    live = p[:, 0] >= 0
+    c = np.clip((p[:, :3] * fG).astype(np.int64), 0, G - 1)
+    k, idx = O.cell_keys(p, G, O.KEY_ROWMAJOR)
+    ref = np.where(live, (c[:, 2] * G + c[:, 1]) * G + c[:, 0] + 1, 0)
+    assert np.array_equal(k, ref.astype(np.uint32)) and np.array_equal(idx, np.arange(5000))
+    k, _ = O.cell_keys(p, G, O.KEY_MORTON)
+    assert np.array_equal(k, np.where(live, morton_np(c[:, 0], c[:, 1], c[:, 2]) + 1, 0))
+    cn = np.clip(np.floor(p[:, :3] * fG - np.float32(0.5)).astype(np.int64), 0, G - 1)
+    k, _ = O.cell_keys(p, G, O.KEY_MORTON_NODE)
+    assert np.array_equal(k, np.where(live, morton_np(cn[:, 0], cn[:, 1], cn[:, 2]) + 1, 0))
+
+
+def test_morton_interleave():
+    assert O.morton3(1, 0, 0) == 1 and O.morton3(0, 1, 0) == 2 and O.morton3(0, 0, 1) == 4
+    assert O.morton3(3, 3, 3) == 63 and O.morton3(1023, 0, 0) == 0o1111111111
+
+
+def test_stable_sort_matches_numpy():
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 5000, 100000).astype(np.uint32)
+    v = np.arange(100000, dtype=np.int32)
+    sk, sv = O.stable_sort_pairs(k, v)
+    o = np.argsort(k, kind="stable")
+    assert np.array_equal(sk, k[o]) and np.array_equal(sv, o)
+
+
+def test_cell_range():
+    k = np.array([0, 0, 1, 1, 1, 4, 7, 7], np.uint32)
+    s, e = O.cell_range(k, 8)
+    assert list(s) == [2, -1, -1, 5, -1, -1, 6, -1] and list(e) == [5, -1, -1, 6, -1, -1, 8, -1]
+
+
+# ---- gradient, gather, integrate ------------------------------------------------------
+@pytest.mark.parametrize("periodic", [False, True])
+def test_grid_accel_linear_potential(periodic):
+    G, ps = 8, 0.25
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    pot = 2.0 * x - 3.0 * y + 0.5 * z
+    a = O.grid_accel(pot, ps, periodic)
+    h = 0.5 * G * ps
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(a[inner][..., 0], -2.0 * 2 * h)
+    assert np.allclose(a[inner][..., 1], 3.0 * 2 * h)
+    assert np.allclose(a[inner][..., 2], -0.5 * 2 * h)
+    if not periodic:  # stencil clamped at the edge: one-sided, half weight
+        assert np.allclose(a[:, :, 0, 0], -2.0 * h) and np.allclose(a[:, :, -1, 0], -2.0 * h)
+    else:  # wrapped stencil sees the jump of the linear ramp
+        assert np.allclose(a[:, :, 0, 0], -(2.0 - 2.0 * (G - 1)) * h)
+
+
+def test_sample_acc_constant_and_linear():
+    G = 16
+    acc = np.zeros((G, G, G, 4), np.float32)
+    acc[..., 0] = 1.5
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    acc[..., 1] = x.astype(np.float32)
+    p = np.array([0.41, 0.52, 0.63], np.float32)
+    a = O.sample_acc(acc, p, G)
+    assert a[0] == pytest.approx(1.5, rel=1e-6)
+    assert a[1] == pytest.approx(float(p[0]) * G - 0.5, abs=1e-5)  # CIC is exact on linear data
+
+
+def test_integrate_constant_field_and_walls():
+    G = 8
+    acc = np.zeros((G, G, G, 4), np.float32)
+    acc[..., 0] = 2.0
+    p = np.array([[0.5, 0.5, 0.5, 0], [0.9975, 0.5, 0.5, 0], [-1, -1, -1, 0]], np.float32)
+    v = np.array([[0.1, -0.2, 0.0, 0], [1.0, 0, 0, 0], [0, 0, 0, 0]], np.float32)
+    dt = np.float32(0.01)
+    p1, v1 = O.integrate(acc, p, v, G, dt, dt)
+    vx = np.float32(0.1) + np.float32(2.0) * dt
+    assert v1[0, 0] == pytest.approx(float(vx), rel=1e-7)
+    assert p1[0, 0] == pytest.approx(0.5 + float(vx) * 0.01, rel=1e-7)
+    assert p1[0, 1] == pytest.approx(0.5 - 0.2 * 0.01, rel=1e-6)
+    # wall at 0.998: clamped, velocity reversed at a quarter
+    assert p1[1, 0] == np.float32(0.998) and v1[1, 0] == pytest.approx(-0.25 * (1.0 + 0.02), rel=1e-6)
+    assert np.array_equal(p1[2], p[2]) and np.array_equal(v1[2], v[2])  # inactive untouched
+
+
+def test_integrate_periodic_wrap_nan_kill_and_speed_clamp():
+    G = 8
+    acc = np.zeros((G, G, G, 4), np.float32)
+    p = np.array([[0.999, 0.001, 0.5, 0], [0.5, 0.5, 0.5, 0], [0.5, 0.5, 0.5, 0]], np.float32)
+    v = np.array([[0.5, -0.5, 0, 0], [np.nan, 0, 0, 0], [300.0, 400.0, 0, 0]], np.float32)
+    p1, v1 = O.integrate(acc, p, v, G, 0.01, 0.01, periodic=True, c2=100.0 ** 2)
+    assert 0.0 <= p1[0, 0] < 0.01 and 0.99 < p1[0, 1] < 1.0
+    assert p1[1, 0] == -1 and np.all(v1[1] == 0)  # non-finite slot killed
+    assert np.hypot(v1[2, 0], v1[2, 1]) == pytest.approx(100.0, rel=1e-6)  # |v| <= c
+
+
+def test_momentum_and_subtract():
+    rng = np.random.default_rng(6)
+    p = rand_pos(1000, 6)
+    p[::10] = (-1, -1, -1, 0)
+    v = rng.standard_normal((1000, 4)).astype(np.float32)
+    m = O.momentum(p, v)
+    live = p[:, 0] >= 0
+    assert np.allclose(m[:3], v[live, :3].astype(np.float64).sum(0)) and m[3] == live.sum()
+    v2 = O.subtract_mean(p, v, m)
+    assert np.abs(v2[live, :3].astype(np.float64).mean(0)).max() < 1e-6
+    assert np.array_equal(v2[~live], v[~live])
+
+
+def test_dt_policy():
+    prm = O.make_params(64, timeScale=1.0)
+    assert O.dt_policy(prm, 0.0) == pytest.approx(0.0016)
+    assert O.dt_policy(prm, 10.0) == pytest.approx(0.8 / 64 / 10.0)
+    assert O.dt_policy(O.make_params(64, timeScale=0.5), 0.0) == pytest.approx(0.0008)
+    assert O.dt_policy(O.make_params(64, timeScale=6.0), 0.0) == pytest.approx(0.0032)
+    assert O.dt_policy(O.make_params(64, contact=1, contactStiffness=1e6), 0.0) == pytest.approx(0.0005)
+    assert O.dt_policy(O.make_params(64, fixedDt=1e-3), 1e9) == pytest.approx(1e-3)
+
+
+# ---- contact --------------------------------------------------------------------------
+def test_contact_matches_brute_force():
+    G, n = 16, 400
+    rng = np.random.default_rng(7)
+    p = np.zeros((n, 4), np.float32)
+    p[:, :3] = rng.uniform(0.2, 0.8, (n, 3))
+    v = np.zeros((n, 4), np.float32)
+    v[:, :3] = rng.normal(0, 0.1, (n, 3))
+    radius, k, c = np.float32(0.5 / G), 1e3, 0.35
+    keys, idx = O.cell_keys(p, G)
+    sk, so = O.stable_sort_pairs(keys, idx)
+    ps, vs = p[so], v[so]
+    cs, ce = O.cell_range(sk, G ** 3)
+    a = O.contact(ps, vs, G, cs, ce, radius, k, c)
+    d0 = 2 * float(radius)
+    ref = np.zeros((n, 3))
+    for i in range(n):
+        e = ps[i, :3].astype(np.float64) - ps[:, :3]
+        dd = (e ** 2).sum(1)
+        for j in np.nonzero((dd < d0 * d0) & (dd > 1e-16))[0]:
+            d = math.sqrt(dd[j])
+            nrm = e[j] / d
+            vn = float(((vs[i, :3] - vs[j, :3]) * nrm).sum())
+            f = max(0.0, k * (d0 - d) - c * vn)
+            ref[i] += f * nrm
+    assert np.allclose(a[:, :3], ref, atol=1e-3 * max(1.0, np.abs(ref).max()))
+    assert np.abs(ref).max() > 0  # some pairs overlap
+
+
+# ---- direct sum -------------------------------------------------------------------------
+def test_direct_sum_two_bodies_and_newton3():
+    p = np.array([[0.2, 0.3, 0.4, 0], [0.5, 0.7, 0.4, 0]], np.float32)
+    a = O.direct_sum(p, np.array([0, 1]), 2.0, 0.1)
+    d = p[1, :3].astype(np.float64) - p[0, :3]
+    r2 = (d ** 2).sum() + 0.01
+    assert np.allclose(a[0], 2.0 * d / r2 ** 1.5) and np.allclose(a[1], -a[0])
+    q = rand_pos(300, 8)
+    aa = O.direct_sum(q, np.arange(300), 1.0, 0.05)
+    assert np.abs(aa.sum(0)).max() < 1e-9 * np.abs(aa).max()
+
+
+def test_rel_rms():
+    t = np.array([[1.0, 0, 0], [0, 2.0, 0]])
+    r, s = O.rel_rms(t * 1.1, t)
+    assert r == pytest.approx(0.1) and s == pytest.approx(1 / 1.1)
+
+
+# ---- MLP ---------------------------------------------------------------------------------
+def mlp_np(W, b, H, x):
+    nw = [H * 30, H * H, H * H, 3 * H]
+    offs = np.cumsum([0] + nw)
+    shapes = [(H, 30), (H, H), (H, H), (3, H)]
+    bo = np.cumsum([0, H, H, H, 3])
+    h = x
+    for l in range(4):
+        Wl = W[offs[l]:offs[l + 1]].reshape(shapes[l])
+        u = h @ Wl.T + b[bo[l]:bo[l + 1]]
+        h = u if l == 3 else 0.5 * u * (1 + np.tanh(math.sqrt(2 / math.pi) * (u + 0.044715 * u ** 3)))
+    return h
+
+
+def test_mlp_forward_and_finite_difference_gradient():
+    H = 8
+    nw, nb = O.mlp_counts(H)
+    assert nw == 30 * H + 2 * H * H + 3 * H and nb == 3 * H + 3
+    rng = np.random.default_rng(9)
+    W = rng.standard_normal(nw) * 0.3
+    b = rng.standard_normal(nb) * 0.1
+    x = rng.standard_normal((5, 30))
+    t = rng.standard_normal((5, 3))
+    assert np.allclose(O.mlp_forward(W, b, H, x), mlp_np(W, b, H, x), atol=1e-12)
+    loss, gW, gb = O.mlp_loss_grad(W, b, H, x, t)
+    assert loss == pytest.approx(0.5 / 5 * ((mlp_np(W, b, H, x) - t) ** 2).sum())
+    eps = 1e-6
+    for idx in rng.choice(nw, 25, replace=False):
+        Wp, Wm = W.copy(), W.copy()
+        Wp[idx] += eps
+        Wm[idx] -= eps
+        fd = (O.mlp_loss_grad(Wp, b, H, x, t)[0] - O.mlp_loss_grad(Wm, b, H, x, t)[0]) / (2 * eps)
+        assert gW[idx] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+    for idx in range(nb):
+        bp, bm = b.copy(), b.copy()
+        bp[idx] += eps
+        bm[idx] -= eps
+        fd = (O.mlp_loss_grad(W, bp, H, x, t)[0] - O.mlp_loss_grad(W, bm, H, x, t)[0]) / (2 * eps)
+        assert gb[idx] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+# ---- whole step (reference tests restated on the oracle) ------------------------------
+def cold_sphere(n, seed, r=0.15):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    rad = r * rng.random(n) ** (1 / 3)
+    p = np.zeros((n, 4), np.float32)
+    p[:, :3] = 0.5 + u * rad[:, None]
+    return p, np.zeros((n, 4), np.float32)
+
+
+def test_step_gravity_responds_and_conserves():
+    """sim_tests.cpp:58-82 (peak density rises under gravity) and 31-52 (mass)."""
+    n, G = 4000, 16
+    p, v = cold_sphere(n, 1)
+    prm = O.make_params(G, gravity=2.0, softeningCells=1.0, sortInterval=5, fixedDt=4e-3, allocN=n)
+    st = O.Stepper(prm, p, v)
+    rho0 = O.deposit_fixed(st.pos, G).max()
+    st.step(40)
+    rho1 = O.deposit_fixed(st.pos, G).max()
+    assert rho1 > 1.5 * rho0
+    assert int(O.deposit_fixed(st.pos, G).sum()) == n * O.FIX_ONE
+    m = O.momentum(st.pos, st.vel)
+    assert np.abs(m[:3]).max() / n < 1e-6  # momentum removed every step
+    assert st.state.simTime == pytest.approx(40 * 4e-3, rel=1e-6)
+
+
+def test_step_sort_is_a_permutation():
+    n, G = 3000, 16
+    p, v = cold_sphere(n, 2)
+    p[:, 3] = np.arange(n, dtype=np.float32)
+    prm = O.make_params(G, gravity=0.1, softeningCells=1.0, sortInterval=1, fixedDt=1e-3, allocN=n)
+    st = O.Stepper(prm, p, v)
+    st.step(1)
+    assert sorted(st.pos[:, 3].astype(int).tolist()) == list(range(n))
+    k, _ = O.cell_keys(st.pos, G, O.KEY_ROWMAJOR)
+    # the sort ran before the drift: keys of the pre-drift positions were ascending
+    assert np.mean(np.diff(k.astype(np.int64)) >= 0) > 0.95
