@@ -981,7 +981,7 @@ static bool query_density(sd_sim* s) {
     if (s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED) return false;
     const bool clearAfter = s->meshValid && s->modes.pipeline == SD_PIPELINE_FUSED;
     launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
-    launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st);
+    TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
     if (!clearAfter && s->modes.depositMode == SD_DEPOSIT_FIXED) s->rhoValid = true;
     return clearAfter || s->modes.depositMode != SD_DEPOSIT_FIXED;
 }

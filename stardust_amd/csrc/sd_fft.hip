@@ -130,6 +130,15 @@ struct Dft<16, INV> {
     }
 };
 
+// LDS line layout: element i lives at PADI(i) = i + i/16. The first Stockham
+// stage writes with a stride of R complex values; without the pad a radix-16
+// stage would put all 16 lanes of a line in one bank.
+__device__ __forceinline__ int PADI(int i) { return i + (i >> 4); }
+template <int N>
+struct Line {
+    static constexpr int LP = N + N / 16 + 1;  // padded line pitch (float2)
+};
+
 // ---- one Stockham stage over an LDS line ---------------------------------
 // tw[m] = exp(-2 pi i m / N) (FP64-computed table); INV uses the conjugate.
 template <int N, int R, int NS, int T, bool INV>
@@ -143,7 +152,7 @@ __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* _
         const int j = t + q * T;
         if (PER * T == NBF || j < NBF) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = line[j + r * NBF];
+            for (int r = 0; r < R; ++r) v[q][r] = line[PADI(j + r * NBF)];
         }
     }
     __syncthreads();
@@ -163,7 +172,7 @@ __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* _
             Dft<R, INV>::run(v[q]);
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[base + r * NS] = v[q][r];
+            for (int r = 0; r < R; ++r) line[PADI(base + r * NS)] = v[q][r];
         }
     }
     __syncthreads();
@@ -245,6 +254,11 @@ __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __rest
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
 }
 
+// Global <-> LDS staging: every thread first issues all of its tile loads
+// (16-byte vectors: two complex values, or two int64 density values) into
+// registers and only then writes LDS, so a block has its whole tile in flight
+// at once instead of one load per round trip.
+
 // ---- X pass: rows, real input -> half spectrum ----------------------------
 // SRC 0: fixed-point density rows (int64, nin values per row, zeroed after the
 //        read when clearSrc != 0 so the next deposit starts from an empty grid).
@@ -254,8 +268,8 @@ struct RowsFwdArgs {
     long long* src;
     float2* out;
     int rows;      // number of rows
-    int nin;       // input values per row
-    int P;         // output row pitch (complex)
+    int nin;       // input values per row (even)
+    int P;         // output row pitch (complex, > N/2 + 1)
     int clearSrc;
     int S;         // Green: grid side
     float soft;    // Green: softening (cells)
@@ -263,24 +277,49 @@ struct RowsFwdArgs {
 
 template <int N, int SRC>
 __global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
-    constexpr int LP = N + 1;
+    constexpr int LP = Line<N>::LP;
+    constexpr int THREADS = TB * Plan<N>::T;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
     const int row0 = blockIdx.x * TB;
-    const int nthreads = TB * Plan<N>::T;
-    // load (zero padded)
-    for (int i = threadIdx.x; i < TB * N; i += nthreads) {
+    // zero the padded half of every line
+    for (int i = threadIdx.x; i < TB * N; i += THREADS) {
         const int c = i / N, x = i - c * N;
-        const int row = row0 + c;
-        float val = 0.f;
-        if (row < a.rows && x < a.nin) {
-            if (SRC == 0) {
-                long long* p = a.src + (size_t)row * a.nin + x;
-                const long long q = *p;
-                val = (float)q * (1.0f / (float)SD_FIX_ONE);
-                if (a.clearSrc) *p = 0;
-            } else {
+        if (x >= a.nin) s_line[c * LP + PADI(x)] = make_float2(0.f, 0.f);
+    }
+    if (SRC == 0) {
+        constexpr int ITERS = (TB * (N / 2) + THREADS - 1) / THREADS;  // int64 pairs
+        const int pairs = a.nin / 2;
+        longlong2 buf[ITERS];
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int q = threadIdx.x + it * THREADS;
+            const int c = q / pairs, j = q - c * pairs;
+            buf[it] = make_longlong2(0, 0);
+            if (c < TB && row0 + c < a.rows) {
+                longlong2* p = reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + c) * a.nin) + j;
+                buf[it] = *p;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int q = threadIdx.x + it * THREADS;
+            const int c = q / pairs, j = q - c * pairs;
+            if (c < TB) {
+                const float sc = 1.0f / (float)SD_FIX_ONE;
+                s_line[c * LP + PADI(2 * j)] = make_float2((float)buf[it].x * sc, 0.f);
+                s_line[c * LP + PADI(2 * j + 1)] = make_float2((float)buf[it].y * sc, 0.f);
+                if (a.clearSrc && row0 + c < a.rows)
+                    reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + c) * a.nin)[j] = make_longlong2(0, 0);
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < TB * a.nin; i += THREADS) {
+            const int c = i / a.nin, x = i - c * a.nin;
+            const int row = row0 + c;
+            float val = 0.f;
+            if (row < a.rows) {
                 const int y = row % a.S, z = row / a.S;
                 const int S = a.S, h = S / 2;
                 const int dx = x <= h ? x : S - x;
@@ -289,17 +328,24 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, con
                 const float r = sqrtf((float)(dx * dx + dy * dy + dz * dz)) + a.soft;
                 val = -1.0f / r;
             }
+            s_line[c * LP + PADI(x)] = make_float2(val, 0.f);
         }
-        s_line[c * LP + x] = make_float2(val, 0.f);
     }
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
     Plan<N>::template run<false>(s_line + c * LP, s_tw, t);
-    constexpr int NK = N / 2 + 1;
-    for (int i = threadIdx.x; i < TB * NK; i += nthreads) {
-        const int cc = i / NK, k = i - cc * NK;
-        const int row = row0 + cc;
-        if (row < a.rows) a.out[(size_t)row * a.P + k] = s_line[cc * LP + k];
+    // store k = 0 .. N/2 as complex pairs (the pair's second half at N/2 + 1
+    // lands in the row padding)
+    constexpr int KP2 = N / 4 + 1;
+    constexpr int OITERS = (TB * KP2 + THREADS - 1) / THREADS;
+#pragma unroll
+    for (int it = 0; it < OITERS; ++it) {
+        const int q = threadIdx.x + it * THREADS;
+        const int cc = q / KP2, j = q - cc * KP2;
+        if (cc < TB && row0 + cc < a.rows) {
+            const float2 u = s_line[cc * LP + PADI(2 * j)], v = s_line[cc * LP + PADI(2 * j + 1)];
+            reinterpret_cast<float4*>(a.out + (size_t)(row0 + cc) * a.P)[j] = make_float4(u.x, u.y, v.x, v.y);
+        }
     }
 }
 
@@ -309,39 +355,62 @@ struct RowsInvArgs {
     float* out;
     int rows;
     int P;
-    int nout;  // real values kept per row (row stride of out)
+    int nout;  // real values kept per row (row stride of out), multiple of 4
 };
 
 template <int N>
 __global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
-    constexpr int LP = N + 1;
+    constexpr int LP = Line<N>::LP;
+    constexpr int THREADS = TB * Plan<N>::T;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
     const int row0 = blockIdx.x * TB;
-    const int nthreads = TB * Plan<N>::T;
-    constexpr int NK = N / 2 + 1;
-    for (int i = threadIdx.x; i < TB * NK; i += nthreads) {
-        const int c = i / NK, k = i - c * NK;
-        const int row = row0 + c;
-        float2 v = make_float2(0.f, 0.f);
-        if (row < a.rows) v = a.in[(size_t)row * a.P + k];
-        s_line[c * LP + k] = v;
-        if (k > 0 && k < N / 2) s_line[c * LP + N - k] = make_float2(v.x, -v.y);
+    constexpr int KP2 = N / 4 + 1;
+    constexpr int ITERS = (TB * KP2 + THREADS - 1) / THREADS;
+    float4 buf[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int q = threadIdx.x + it * THREADS;
+        const int c = q / KP2, j = q - c * KP2;
+        buf[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c < TB && row0 + c < a.rows)
+            buf[it] = reinterpret_cast<const float4*>(a.in + (size_t)(row0 + c) * a.P)[j];
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int q = threadIdx.x + it * THREADS;
+        const int c = q / KP2, j = q - c * KP2;
+        if (c < TB) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = 2 * j + h;
+                if (k > N / 2) continue;
+                const float2 v = h ? make_float2(buf[it].z, buf[it].w) : make_float2(buf[it].x, buf[it].y);
+                s_line[c * LP + PADI(k)] = v;
+                if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(v.x, -v.y);
+            }
+        }
     }
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
     Plan<N>::template run<true>(s_line + c * LP, s_tw, t);
-    for (int i = threadIdx.x; i < TB * a.nout; i += nthreads) {
-        const int cc = i / a.nout, x = i - cc * a.nout;
+    const int q4 = a.nout / 4;
+    for (int i = threadIdx.x; i < TB * q4; i += THREADS) {
+        const int cc = i / q4, j = i - cc * q4;
         const int row = row0 + cc;
-        if (row < a.rows) a.out[(size_t)row * a.nout + x] = s_line[cc * LP + x].x;
+        if (row < a.rows) {
+            const float2* l = s_line + cc * LP;
+            reinterpret_cast<float4*>(a.out + (size_t)row * a.nout)[j] =
+                make_float4(l[PADI(4 * j)].x, l[PADI(4 * j + 1)].x, l[PADI(4 * j + 2)].x, l[PADI(4 * j + 3)].x);
+        }
     }
 }
 
 // ---- column passes ---------------------------------------------------------
 // Column b of slab a: element n at in[a*inA + n*inN + b]; output k at
-// out[a*outA + k*outN + b]. A block owns TB adjacent columns of one slab.
+// out[a*outA + k*outN + b]. A block owns TB adjacent columns of one slab; each
+// thread moves column pairs (16 bytes) of TB/2-wide row segments.
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_Z_ISO = 2, COL_Z_PER = 3, COL_GREEN = 4 };
 
 struct ColArgs {
@@ -363,20 +432,44 @@ struct ColArgs {
 
 template <int N, int MODE>
 __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float2* __restrict__ tw) {
-    constexpr int LP = N + 1;
+    constexpr int LP = Line<N>::LP;
+    constexpr int THREADS = TB * Plan<N>::T;
+    constexpr int PAIRS = TB / 2;
+    constexpr int RPI = THREADS / PAIRS;  // rows per staging iteration
+    constexpr int ITERS = (N + RPI - 1) / RPI;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
     const int slab = blockIdx.y;
     if (MODE == COL_GREEN && slab > N / 2) return;  // only ky <= S/2 is stored
     load_twiddles<N>(s_tw, tw);
     const int b0 = blockIdx.x * TB;
-    const int nthreads = TB * Plan<N>::T;
-    const float2* in = a.in + (size_t)slab * a.inA + b0;
-    for (int i = threadIdx.x; i < TB * N; i += nthreads) {
-        const int c = i % TB, n = i / TB;
-        float2 v = make_float2(0.f, 0.f);
-        if (n < a.nin && b0 + c < a.nb) v = in[(size_t)n * a.inN + c];
-        s_line[c * LP + n] = v;
+    const int pc = threadIdx.x % PAIRS, r0 = threadIdx.x / PAIRS;
+    const int col = 2 * pc;
+    const bool ok0 = b0 + col < a.nb, ok1 = b0 + col + 1 < a.nb;
+    {
+        const float2* in = a.in + (size_t)slab * a.inA + b0 + col;
+        float4 buf[ITERS];
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int n = r0 + it * RPI;
+            buf[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (n < a.nin && ok0) {
+                if (ok1) {
+                    buf[it] = *reinterpret_cast<const float4*>(in + (size_t)n * a.inN);
+                } else {
+                    const float2 x = in[(size_t)n * a.inN];
+                    buf[it] = make_float4(x.x, x.y, 0.f, 0.f);
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int n = r0 + it * RPI;
+            if (n < N) {
+                s_line[col * LP + PADI(n)] = make_float2(buf[it].x, buf[it].y);
+                s_line[(col + 1) * LP + PADI(n)] = make_float2(buf[it].z, buf[it].w);
+            }
+        }
     }
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
@@ -388,11 +481,11 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float
     }
     if (MODE == COL_GREEN) {
         const int h = N / 2;
-        for (int i = threadIdx.x; i < TB * (h + 1); i += nthreads) {
+        for (int i = threadIdx.x; i < TB * (h + 1); i += THREADS) {
             const int cc = i % TB, kz = i / TB;
             if (b0 + cc < a.nb)
                 a.greenOut[((size_t)kz * (h + 1) + slab) * a.P + b0 + cc] =
-                    s_line[cc * LP + kz].x * a.greenNorm;
+                    s_line[cc * LP + PADI(kz)].x * a.greenNorm;
         }
         return;
     }
@@ -400,36 +493,55 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float
         const int h = N / 2;
         const int ky = slab;
         const int kyf = ky <= h ? ky : N - ky;
-        for (int i = threadIdx.x; i < TB * N; i += nthreads) {
-            const int cc = i % TB, kz = i / TB;
-            const int kx = b0 + cc;
-            if (kx >= a.nb) continue;
-            float m;
+        const int kx0 = b0 + col;
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int kz = r0 + it * RPI;
+            if (kz >= N) continue;
+            float m0, m1;
             if (MODE == COL_Z_ISO) {
                 const int kzf = kz <= h ? kz : N - kz;
-                m = a.green[((size_t)kzf * (h + 1) + kyf) * a.P + kx];
+                // the table row has pitch P > nb, so the pair never leaves it
+                const float2 g = *reinterpret_cast<const float2*>(a.green + ((size_t)kzf * (h + 1) + kyf) * a.P + kx0);
+                m0 = g.x;
+                m1 = g.y;
             } else {
                 // kPoissonPeriodic, Sim.cu:528-557
                 const int sky = ky <= h ? ky : ky - N;
                 const int skz = kz <= h ? kz : kz - N;
-                const float k2 = (float)(kx * kx + sky * sky + skz * skz);
-                if (k2 < 1e-6f) {
-                    m = 0.f;
-                } else {
-                    m = a.perScale / k2;
-                    if (a.perSoft > 0.f) m *= expf(-0.5f * k2 * a.perSoft);
+                float mm[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int kx = kx0 + e;
+                    const float k2 = (float)(kx * kx + sky * sky + skz * skz);
+                    if (k2 < 1e-6f) {
+                        mm[e] = 0.f;
+                    } else {
+                        mm[e] = a.perScale / k2;
+                        if (a.perSoft > 0.f) mm[e] *= expf(-0.5f * k2 * a.perSoft);
+                    }
                 }
+                m0 = mm[0];
+                m1 = mm[1];
             }
-            float2 v = s_line[cc * LP + kz];
-            s_line[cc * LP + kz] = make_float2(v.x * m, v.y * m);
+            const float2 u = s_line[col * LP + PADI(kz)], v = s_line[(col + 1) * LP + PADI(kz)];
+            s_line[col * LP + PADI(kz)] = make_float2(u.x * m0, u.y * m0);
+            s_line[(col + 1) * LP + PADI(kz)] = make_float2(v.x * m1, v.y * m1);
         }
         __syncthreads();
         Plan<N>::template run<true>(line, s_tw, t);
     }
-    float2* out = a.out + (size_t)slab * a.outA + b0;
-    for (int i = threadIdx.x; i < TB * a.nout; i += nthreads) {
-        const int cc = i % TB, k = i / TB;
-        if (b0 + cc < a.nb) out[(size_t)k * a.outN + cc] = s_line[cc * LP + k];
+    float2* out = a.out + (size_t)slab * a.outA + b0 + col;
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = r0 + it * RPI;
+        if (k < a.nout && ok0) {
+            const float2 u = s_line[col * LP + PADI(k)], v = s_line[(col + 1) * LP + PADI(k)];
+            if (ok1)
+                *reinterpret_cast<float4*>(out + (size_t)k * a.outN) = make_float4(u.x, u.y, v.x, v.y);
+            else
+                out[(size_t)k * a.outN] = u;
+        }
     }
 }
 

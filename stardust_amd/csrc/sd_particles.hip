@@ -67,22 +67,24 @@ __device__ __forceinline__ Cic cic_setup(float4 p, float fG) {
 }
 
 // The 2^24 mass units of a particle split exactly along x, y, z (w1 = rint(f w),
-// w0 = w - w1), so the eight weights always sum to 2^24.
-__device__ __forceinline__ void fixed_weights(const Cic& c, long long w[8]) {
-    long long wx[2], wxy[4];
-    wx[1] = (long long)rintf(c.fx * (float)SD_FIX_ONE);
-    wx[0] = SD_FIX_ONE - wx[1];
+// w0 = w - w1), so the eight weights always sum to 2^24. Every partial weight
+// is <= 2^24, so 32-bit integers hold them exactly and the float <-> int
+// conversions are single instructions.
+__device__ __forceinline__ void fixed_weights(const Cic& c, uint32_t w[8]) {
+    int wx[2], wxy[4];
+    wx[1] = (int)rintf(c.fx * (float)SD_FIX_ONE);
+    wx[0] = (int)SD_FIX_ONE - wx[1];
 #pragma unroll
     for (int ox = 0; ox < 2; ++ox) {
-        const long long y1 = (long long)rintf(c.fy * (float)wx[ox]);
+        const int y1 = (int)rintf(c.fy * (float)wx[ox]);
         wxy[ox + 2] = y1;
         wxy[ox] = wx[ox] - y1;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const long long z1 = (long long)rintf(c.fz * (float)wxy[j]);
-        w[j + 4] = z1;
-        w[j] = wxy[j] - z1;
+        const int z1 = (int)rintf(c.fz * (float)wxy[j]);
+        w[j + 4] = (uint32_t)z1;
+        w[j] = (uint32_t)(wxy[j] - z1);
     }
 }
 
@@ -104,15 +106,25 @@ constexpr int DEP_CAP = 2048;  // LDS tile cells (16 KiB)
 constexpr int KP = 4;          // particles per thread in integrate / deposit
 constexpr int BP = PT * KP;    // particles per block
 
-// Segmented scan over the wavefront; returns true on the lane that holds a
-// run's total in v[8].
-__device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, long long v[8]) {
+// DPP lane shift of a 32-bit value (lanes without a source read 0).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_src(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, true);
+}
+
+// Segmented inclusive scan over the wavefront; returns true on the lane that
+// holds a run's total in v[8]. A run lies inside one wavefront, so its sums
+// are <= 64 * 2^24 = 2^30 and fit 32 bits. The scan is Hillis-Steele on DPP:
+// row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast 15 (rows 1, 3)
+// and row_bcast 31 (rows 2, 3); a lane adds a source only if the source lies
+// in its own run (source lane >= the run's first lane).
+__device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, uint32_t v[8]) {
     const int lane = threadIdx.x & 63;
     if (alive) {
         fixed_weights(c, v);
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = 0;
+        for (int k = 0; k < 8; ++k) v[k] = 0u;
     }
     const int ix = alive ? c.ix : INT_MIN, iy = alive ? c.iy : INT_MIN, iz = alive ? c.iz : INT_MIN;
     const int pix = __shfl_up(ix, 1, 64), piy = __shfl_up(iy, 1, 64), piz = __shfl_up(iz, 1, 64);
@@ -121,19 +133,25 @@ __device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, long long v[
     const uint64_t le = lane == 63 ? headMask : (headMask & ((2ull << lane) - 1ull));
     const int start = 63 - __clzll(le);
     const bool tail = lane == 63 || ((headMask >> (lane + 1)) & 1ull);
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const bool take = lane - off >= start;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const long long y = __shfl_up(v[k], off, 64);
-            if (take) v[k] += y;
-        }
+#define SD_DPP_STEP(CTRL, ROWMASK, TAKE)                          \
+    {                                                             \
+        const bool take_ = (TAKE);                                \
+        _Pragma("unroll") for (int k = 0; k < 8; ++k) {           \
+            const uint32_t y_ = dpp_src<CTRL, ROWMASK>(v[k]);     \
+            if (take_) v[k] += y_;                                \
+        }                                                         \
     }
+    SD_DPP_STEP(0x111, 0xF, (lane & 15) >= 1 && lane - 1 >= start)
+    SD_DPP_STEP(0x112, 0xF, (lane & 15) >= 2 && lane - 2 >= start)
+    SD_DPP_STEP(0x114, 0xF, (lane & 15) >= 4 && lane - 4 >= start)
+    SD_DPP_STEP(0x118, 0xF, (lane & 15) >= 8 && lane - 8 >= start)
+    SD_DPP_STEP(0x142, 0xA, ((lane >> 4) & 1) && (lane & ~15) - 1 >= start)
+    SD_DPP_STEP(0x143, 0xC, lane >= 32 && 31 >= start)
+#undef SD_DPP_STEP
     return alive && tail;
 }
 
-__device__ __forceinline__ void add_global8(const Cic& c, const long long v[8], int G, int periodic,
+__device__ __forceinline__ void add_global8(const Cic& c, const uint32_t v[8], int G, int periodic,
                                             long long* rho) {
     int xs[2], ys[2], zs[2];
     xs[0] = wrap_or_clamp(c.ix, G, periodic);
@@ -145,7 +163,7 @@ __device__ __forceinline__ void add_global8(const Cic& c, const long long v[8], 
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        if (v[k] != 0) {
+        if (v[k] != 0u) {
             const size_t idx = ((size_t)zs[oz] * G + ys[oy]) * G + xs[ox];
             atomicAdd((unsigned long long*)(rho + idx), (unsigned long long)v[k]);
         }
@@ -197,7 +215,7 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
     if (vol > DEP_CAP) {
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            long long v[8];
+            uint32_t v[8];
             if (wave_runs(alive[k], c[k], v)) add_global8(c[k], v, G, periodic, rho);
         }
         return;
@@ -207,13 +225,13 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-        long long v[8];
+        uint32_t v[8];
         if (wave_runs(alive[k], c[k], v)) {
             const int bx = c[k].ix - lo[0], by = c[k].iy - lo[1], bz = c[k].iz - lo[2];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int ox = q & 1, oy = (q >> 1) & 1, oz = (q >> 2) & 1;
-                if (v[q] != 0)
+                if (v[q] != 0u)
                     atomicAdd(&s_tile[((bz + oz) * ndy + (by + oy)) * ndx + (bx + ox)], (unsigned long long)v[q]);
             }
         }
@@ -335,24 +353,42 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
     float vm = 0.f;
     bool depAlive[KP];
     Cic depC[KP];
+    // phase 1: every particle load of this thread in flight together
+    float4 P[KP], V[KP];
+    bool act[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int i = blockIdx.x * BP + k * PT + threadIdx.x;
+        act[k] = i < a.n;
+        P[k] = act[k] ? a.pos[i] : make_float4(-1.f, -1.f, -1.f, 0.f);
+        V[k] = act[k] ? a.vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // phase 2: the CIC gathers (8 x KP loads, no stores in between)
+    const float fG = (float)a.G;
+    float3 ACC[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        ACC[k] = make_float3(0.f, 0.f, 0.f);
+        if (P[k].x >= 0.f) ACC[k] = sample_acc(a.accG, cic_setup(P[k], fG), a.G, a.periodic);
+    }
+    const float kick = a.sc->kick, drift = a.sc->dt;
+    // phase 3: kick, drift, boundaries, stores
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
         depAlive[k] = false;
         depC[k].ix = depC[k].iy = depC[k].iz = 0;
         depC[k].fx = depC[k].fy = depC[k].fz = 0.f;
         const int i = blockIdx.x * BP + k * PT + threadIdx.x;
-        if (i >= a.n) continue;
-        float4 p = a.pos[i];
+        if (!act[k]) continue;
+        float4 p = P[k];
         if (p.x >= 0.f) {
-            const float kick = a.sc->kick, drift = a.sc->dt;
-            float4 v = a.vel[i];
+            float4 v = V[k];
             if (a.applyMeanSub) {
                 v.x -= a.sc->meanSub[0];
                 v.y -= a.sc->meanSub[1];
                 v.z -= a.sc->meanSub[2];
             }
-            const float fG = (float)a.G;
-            float3 acc = sample_acc(a.accG, cic_setup(p, fG), a.G, a.periodic);
+            float3 acc = ACC[k];
             if (a.accContact) {
                 const float4 c = a.accContact[i];
                 acc.x += c.x;
