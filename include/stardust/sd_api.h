@@ -200,6 +200,29 @@ int sd_particle_accel(sd_sim* s, float* acc4, int n);
  * acc3[3*nt] = gm * sum_j (x_j - x_i)/(r^2 + eps^2)^(3/2), box units. */
 int sd_direct_sum(sd_sim* s, const int32_t* targets, int nt, double gm, double eps, double* acc3);
 
+/* ---- learned sub-cell correction (no reference counterpart) --------------
+ * MLP 30 -> 128 -> 128 -> 128 -> 3 (GELU, tanh form), BF16 weights, FP32
+ * accumulation on the matrix cores. Inputs per particle (sd_features): log1p
+ * of the particle counts at the 27 mesh nodes around its nearest node, then
+ * 2 * (p G - 0.5 - node) per axis. Output: the direct-sum minus PM
+ * acceleration divided by (a_unit * outScale), a_unit = gravity * G / N.
+ * Parameters use the layout W1 [128 x 30] | W2 | W3 [128 x 128] | W4 [3 x 128]
+ * (row-major out x in), b1 | b2 | b3 | b4. */
+typedef struct sd_mlp sd_mlp;
+int sd_mlp_param_counts(int* nW, int* nB);
+sd_mlp* sd_mlp_create(unsigned seed);
+void sd_mlp_destroy(sd_mlp* m);
+int sd_mlp_set_params(sd_mlp* m, const float* W, const float* b);
+int sd_mlp_get_params(sd_mlp* m, float* W, float* b);
+int sd_mlp_forward(sd_mlp* m, const float* X, int n, float* Y);           /* X [n x 30], Y [n x 3] */
+int sd_mlp_grad(sd_mlp* m, const float* X, const float* T, int n, double* loss, float* gW,
+                float* gb);                                               /* loss 0.5/n sum |Y - T|^2 */
+int sd_mlp_train(sd_mlp* m, const float* X, const float* T, int n, int epochs, int batch, float lr,
+                 unsigned seed, float* lossPerEpoch);                     /* Adam, shuffled batches */
+int sd_features(sd_sim* s, const int32_t* targets, int nt, float* X);    /* X [nt x 30] */
+int sd_attach_mlp(sd_sim* s, sd_mlp* m, float outScale);  /* m = NULL detaches; steps add the correction */
+int sd_mlp_correction(sd_sim* s, float* acc4, int n);      /* correction at current positions, box units */
+
 /* ---- kernel timing (no reference counterpart) ----------------------------
  * Brackets every launch of one named kernel with HIP events on the handle's
  * stream; sd_kernel_time_ms returns total milliseconds and launch count. */

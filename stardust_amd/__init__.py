@@ -170,6 +170,17 @@ def _declare(L):
         "sd_sort_now": (C.c_int, [P, P, P]),
         "sd_particle_accel": (C.c_int, [P, P, C.c_int]),
         "sd_direct_sum": (C.c_int, [P, P, C.c_int, C.c_double, C.c_double, P]),
+        "sd_mlp_param_counts": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "sd_mlp_create": (P, [C.c_uint]),
+        "sd_mlp_destroy": (None, [P]),
+        "sd_mlp_set_params": (C.c_int, [P, P, P]),
+        "sd_mlp_get_params": (C.c_int, [P, P, P]),
+        "sd_mlp_forward": (C.c_int, [P, P, C.c_int, P]),
+        "sd_mlp_grad": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_double), P, P]),
+        "sd_mlp_train": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, P]),
+        "sd_features": (C.c_int, [P, P, C.c_int, P]),
+        "sd_attach_mlp": (C.c_int, [P, P, C.c_float]),
+        "sd_mlp_correction": (C.c_int, [P, P, C.c_int]),
         "sd_kernel_timer_enable": (C.c_int, [P, C.c_char_p]),
         "sd_kernel_time_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     }
@@ -374,6 +385,23 @@ class Sim:
                "sd_direct_sum")
         return out
 
+    def features(self, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        out = np.empty((t.shape[0], 30), np.float32)
+        _check(lib().sd_features(self._h, _ptr(t), t.shape[0], _ptr(out)), "sd_features")
+        return out
+
+    def attach_mlp(self, mlp: "Mlp | None", out_scale: float = 1.0):
+        _check(lib().sd_attach_mlp(self._h, mlp._h if mlp is not None else None, float(out_scale)),
+               "sd_attach_mlp")
+        self._mlp = mlp  # keep it alive while attached
+
+    def mlp_correction(self, n: int | None = None):
+        n = self.particle_count() if n is None else n
+        out = np.empty((n, 4), np.float32)
+        _check(lib().sd_mlp_correction(self._h, _ptr(out), n), "sd_mlp_correction")
+        return out
+
     def kernel_timer(self, name: str | None):
         _check(lib().sd_kernel_timer_enable(self._h, (name or "").encode()), "kernel_timer")
 
@@ -381,6 +409,65 @@ class Sim:
         ms, n = C.c_double(), C.c_longlong()
         _check(lib().sd_kernel_time_ms(self._h, C.byref(ms), C.byref(n)), "kernel_time_ms")
         return ms.value, n.value
+
+
+class Mlp:
+    """The learned correction network (sd_mlp): 30 -> 128 -> 128 -> 128 -> 3."""
+
+    def __init__(self, seed: int = 1):
+        self._h = lib().sd_mlp_create(int(seed))
+        if not self._h:
+            raise StardustError("sd_mlp_create failed: " + fail_message())
+        nw, nb = C.c_int(), C.c_int()
+        lib().sd_mlp_param_counts(C.byref(nw), C.byref(nb))
+        self.nW, self.nB = nw.value, nb.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        W = np.empty(self.nW, np.float32)
+        b = np.empty(self.nB, np.float32)
+        _check(lib().sd_mlp_get_params(self._h, _ptr(W), _ptr(b)), "sd_mlp_get_params")
+        return W, b
+
+    def set_params(self, W, b):
+        W = np.ascontiguousarray(W, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        assert W.size == self.nW and b.size == self.nB
+        _check(lib().sd_mlp_set_params(self._h, _ptr(W), _ptr(b)), "sd_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], 3), np.float32)
+        _check(lib().sd_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y)), "sd_mlp_forward")
+        return Y
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        gW = np.empty(self.nW, np.float32)
+        gb = np.empty(self.nB, np.float32)
+        loss = C.c_double()
+        _check(lib().sd_mlp_grad(self._h, _ptr(X), _ptr(T), X.shape[0], C.byref(loss), _ptr(gW),
+                                 _ptr(gb)), "sd_mlp_grad")
+        return loss.value, gW, gb
+
+    def train(self, X, T, epochs, batch=8192, lr=1e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        losses = np.zeros(max(epochs, 1), np.float32)
+        _check(lib().sd_mlp_train(self._h, _ptr(X), _ptr(T), X.shape[0], int(epochs), int(batch),
+                                  float(lr), int(seed), _ptr(losses)), "sd_mlp_train")
+        return losses[:epochs]
 
 
 def failed() -> bool:

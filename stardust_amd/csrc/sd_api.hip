@@ -16,6 +16,7 @@
 
 #include "../../include/stardust/sd_api.h"
 #include "sd_internal.h"
+#include "sd_mlp.h"
 
 using namespace sd;
 
@@ -105,6 +106,11 @@ struct sd_sim {
     bool primed = false;     // sc->dt / sc->kick are set for the next integrate
     long long stepCount = 0;
 
+    // learned correction
+    struct sd_mlp* mlp = nullptr;
+    float mlpOutScale = 0.f;
+    float4* accCorr = nullptr;
+
     // timings
     hipEvent_t evA = nullptr, ev1 = nullptr, evB = nullptr;
     sd_timings tm{};
@@ -193,7 +199,7 @@ static void free_buffers(sd_sim* s) {
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
-    dfree(s->cellStart); dfree(s->cellEnd); dfree(s->partials); dfree(s->sc);
+    dfree(s->cellStart); dfree(s->cellEnd); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
     s->cellTableCells = 0;
     if (s->scHost) {
         (void)hipHostFree(s->scHost);
@@ -468,6 +474,77 @@ static bool contact_pass(sd_sim* s) {
     return true;
 }
 
+// ---- learned correction -------------------------------------------------------
+struct sd_mlp {
+    MlpDev d{};
+    hipStream_t st = nullptr;
+    float* dX = nullptr;  // staging for host-provided inputs
+    float* dT = nullptr;
+    int32_t* dIdx = nullptr;
+    int cap = 0;
+    int adamStep = 0;
+};
+
+static bool mlp_alloc_batch(sd_mlp* m, int rows) {
+    MlpDev& d = m->d;
+    const int nPad = mlp_pad_rows(rows);
+    if (nPad <= d.maxBatch) return true;
+    auto F = [](auto*& p) { dfree(p); };
+    F(d.Xb); F(d.Xt); F(d.Y); F(d.dYb); F(d.dYt); F(d.dYf); F(d.lossPart);
+    for (int l = 0; l < 3; ++l) { F(d.U[l]); F(d.Hb[l]); F(d.Ht[l]); F(d.Df[l]); F(d.Db[l]); F(d.Dt[l]); }
+    const size_t R = (size_t)nPad;
+    bool ok = dalloc(d.Xb, R * MLP_INP) && dalloc(d.Xt, R * MLP_INP) && dalloc(d.Y, R * MLP_OUTP) &&
+              dalloc(d.dYb, R * MLP_OUTP) && dalloc(d.dYt, R * MLP_OUTP) && dalloc(d.dYf, R * MLP_OUTP) &&
+              dalloc(d.lossPart, (R * MLP_OUTP + 255) / 256);
+    for (int l = 0; l < 3 && ok; ++l)
+        ok = dalloc(d.U[l], R * MLP_H) && dalloc(d.Hb[l], R * MLP_H) && dalloc(d.Ht[l], R * MLP_H) &&
+             dalloc(d.Df[l], R * MLP_H) && dalloc(d.Db[l], R * MLP_H) && dalloc(d.Dt[l], R * MLP_H);
+    d.maxBatch = ok ? nPad : 0;
+    return ok;
+}
+
+static bool mlp_stage(sd_mlp* m, int rows) {
+    if (rows <= m->cap) return true;
+    dfree(m->dX); dfree(m->dT); dfree(m->dIdx);
+    const bool ok = dalloc(m->dX, (size_t)rows * MLP_IN) && dalloc(m->dT, (size_t)rows * MLP_OUT) &&
+                    dalloc(m->dIdx, (size_t)rows);
+    m->cap = ok ? rows : 0;
+    return ok;
+}
+
+// Runs the network on every particle of the current positions (needs the
+// fixed-point density of those positions) into s->accCorr.
+static bool mlp_correction(sd_sim* s) {
+    if (!s->mlp) return false;
+    if (!s->accCorr && !dalloc(s->accCorr, (size_t)s->allocN)) return false;
+    const bool hadRho = s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED;
+    if (!hadRho) {
+        // rhoFix is zero whenever it is not valid (fixed-point invariant)
+        if (s->modes.depositMode == SD_DEPOSIT_FLOAT) launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+        TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
+    }
+    MlpInferArgs a{};
+    a.pos = s->pos;
+    a.n = s->allocN;
+    a.rho = s->rhoFix;
+    a.G = s->G;
+    a.periodic = s->periodic;
+    const MlpDev& d = s->mlp->d;
+    a.W1p = d.W1p; a.W2 = d.W2; a.W3 = d.W3; a.W4p = d.W4p; a.b = d.b;
+    a.scale = s->mlpOutScale * s->cfg.gravity * (float)s->G / (float)(s->allocN > 0 ? s->allocN : 1);
+    a.out = s->accCorr;
+    int cus = sd_device_multiprocessors();
+    TIMED(s, "mlp", launch_mlp_infer(a, cus > 0 ? cus : 256, s->st));
+    if (!hadRho) {
+        if (s->modes.depositMode == SD_DEPOSIT_FIXED && !s->meshValid) {
+            s->rhoValid = true;
+        } else {
+            launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+        }
+    }
+    return true;
+}
+
 static void prime_scalars(sd_sim* s) {
     launch_vmax(s->pos, s->vel, s->allocN, s->partials, s->st);
     launch_finalize(s->partials, s->nparts, 2 | 4, dt_policy(s), s->sc, s->st);
@@ -481,6 +558,7 @@ static void enqueue_step(sd_sim* s) {
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
                          !s->cfg.contactEnabled;
     if (sortDue) sort_particles(s);
+    const bool corr = mlp_correction(s);
     if (fused) {
         ensure_mesh(s);
     } else {
@@ -499,6 +577,7 @@ static void enqueue_step(sd_sim* s) {
     a.periodic = s->periodic;
     a.c2 = light_speed_sq(s->cfg.lengthScale, s->cfg.timeUnitScale);
     a.accContact = contact ? s->accContact : nullptr;
+    a.accCorr = corr ? s->accCorr : nullptr;
     a.softBoundR = s->periodic ? 0.f : fmaxf(s->cfg.softBoundR, 0.f);
     a.darkEnergy = s->cfg.darkEnergy;
     a.sc = s->sc;
@@ -559,6 +638,7 @@ __global__ void k_restart_integrator(StepScalars* sc) {
 }
 
 static void synced_velocity(sd_sim* s);
+static bool query_density(sd_sim* s);
 
 static int check_ready(const sd_sim* s) {
     if (g_failed) return SD_ERR_FAILED;
@@ -1286,6 +1366,212 @@ int sd_kernel_time_ms(sd_sim* s, double* totalMs, long long* launches) {
     if (totalMs) *totalMs = s->timer.totalMs;
     if (launches) *launches = s->timer.launches;
     return SD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned correction
+// ---------------------------------------------------------------------------
+int sd_mlp_param_counts(int* nW, int* nB) {
+    if (nW) *nW = MLP_NW;
+    if (nB) *nB = MLP_NB;
+    return SD_OK;
+}
+
+sd_mlp* sd_mlp_create(unsigned seed) {
+    if (g_failed) return nullptr;
+    sd_mlp* m = new sd_mlp();
+    MlpDev& d = m->d;
+    HK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+    bool ok = dalloc(d.W, MLP_NW) && dalloc(d.b, MLP_NB) && dalloc(d.gW, MLP_NW) && dalloc(d.gb, MLP_NB) &&
+              dalloc(d.mW, MLP_NW) && dalloc(d.vW, MLP_NW) && dalloc(d.mb, MLP_NB) && dalloc(d.vb, MLP_NB) &&
+              dalloc(d.b4p, MLP_OUTP) && dalloc(d.gb4p, MLP_OUTP) && dalloc(d.W1p, MLP_H * MLP_INP) &&
+              dalloc(d.W2, MLP_H * MLP_H) && dalloc(d.W3, MLP_H * MLP_H) && dalloc(d.W4p, MLP_OUTP * MLP_H) &&
+              dalloc(d.W2t, MLP_H * MLP_H) && dalloc(d.W3t, MLP_H * MLP_H) && dalloc(d.W4pt, MLP_OUTP * MLP_H) &&
+              dalloc(d.part, 16 * MLP_H * MLP_H);
+    if (!ok) {
+        sd_mlp_destroy(m);
+        return nullptr;
+    }
+    // deterministic init: uniform(-sqrt(6/fan_in), +) from a 64-bit LCG, zero biases
+    std::vector<float> W(MLP_NW), b(MLP_NB, 0.f);
+    unsigned long long st = 0x9E3779B97F4A7C15ull ^ (unsigned long long)seed;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int fan[4] = {MLP_IN, MLP_H, MLP_H, MLP_H};
+    const int cnt[4] = {MLP_H * MLP_IN, MLP_H * MLP_H, MLP_H * MLP_H, MLP_OUT * MLP_H};
+    int off = 0;
+    for (int l = 0; l < 4; ++l) {
+        const double a = std::sqrt(6.0 / fan[l]) * (l == 3 ? 0.1 : 1.0);
+        for (int i = 0; i < cnt[l]; ++i) W[off + i] = (float)((2.0 * rnd() - 1.0) * a);
+        off += cnt[l];
+    }
+    sd_mlp_set_params(m, W.data(), b.data());
+    return m;
+}
+
+void sd_mlp_destroy(sd_mlp* m) {
+    if (!m) return;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    MlpDev& d = m->d;
+    auto F = [](auto*& p) { dfree(p); };
+    F(d.W); F(d.b); F(d.gW); F(d.gb); F(d.mW); F(d.vW); F(d.mb); F(d.vb); F(d.b4p); F(d.gb4p);
+    F(d.W1p); F(d.W2); F(d.W3); F(d.W4p); F(d.W2t); F(d.W3t); F(d.W4pt); F(d.part);
+    F(d.Xb); F(d.Xt); F(d.Y); F(d.dYb); F(d.dYt); F(d.dYf); F(d.lossPart);
+    for (int l = 0; l < 3; ++l) { F(d.U[l]); F(d.Hb[l]); F(d.Ht[l]); F(d.Df[l]); F(d.Db[l]); F(d.Dt[l]); }
+    F(m->dX); F(m->dT); F(m->dIdx);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+}
+
+int sd_mlp_set_params(sd_mlp* m, const float* W, const float* b) {
+    if (!m || !W || !b) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    MlpDev& d = m->d;
+    HK(hipMemcpy(d.W, W, sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+    HK(hipMemcpy(d.b, b, sizeof(float) * MLP_NB, hipMemcpyHostToDevice));
+    std::vector<float> z(MLP_NW, 0.f);
+    HK(hipMemcpy(d.mW, z.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+    HK(hipMemcpy(d.vW, z.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+    HK(hipMemcpy(d.mb, z.data(), sizeof(float) * MLP_NB, hipMemcpyHostToDevice));
+    HK(hipMemcpy(d.vb, z.data(), sizeof(float) * MLP_NB, hipMemcpyHostToDevice));
+    HK(hipMemcpy(d.b4p, z.data(), sizeof(float) * MLP_OUTP, hipMemcpyHostToDevice));
+    m->adamStep = 0;
+    mlp_pack(d, m->st);
+    HK(hipStreamSynchronize(m->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_mlp_get_params(sd_mlp* m, float* W, float* b) {
+    if (!m) return SD_ERR_ARG;
+    HK(hipStreamSynchronize(m->st));
+    if (W) HK(hipMemcpy(W, m->d.W, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost));
+    if (b) HK(hipMemcpy(b, m->d.b, sizeof(float) * MLP_NB, hipMemcpyDeviceToHost));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_mlp_forward(sd_mlp* m, const float* X, int n, float* Y) {
+    if (!m || n <= 0 || !X || !Y) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    if (!mlp_alloc_batch(m, n) || !mlp_stage(m, n)) return SD_ERR_DEVICE;
+    HK(hipMemcpy(m->dX, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice));
+    mlp_forward(m->d, m->dX, n, m->st);
+    std::vector<float> y((size_t)mlp_pad_rows(n) * MLP_OUTP);
+    HK(hipMemcpyAsync(y.data(), m->d.Y, sizeof(float) * y.size(), hipMemcpyDeviceToHost, m->st));
+    HK(hipStreamSynchronize(m->st));
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < MLP_OUT; ++c) Y[(size_t)i * MLP_OUT + c] = y[(size_t)i * MLP_OUTP + c];
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+static double mlp_read_loss(sd_mlp* m) {
+    std::vector<double> lp((size_t)m->d.lossBlocks);
+    HK(hipMemcpyAsync(lp.data(), m->d.lossPart, sizeof(double) * lp.size(), hipMemcpyDeviceToHost, m->st));
+    HK(hipStreamSynchronize(m->st));
+    double L = 0.0;
+    for (double v : lp) L += v;
+    return L;
+}
+
+int sd_mlp_grad(sd_mlp* m, const float* X, const float* T, int n, double* loss, float* gW, float* gb) {
+    if (!m || n <= 0 || !X || !T) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    if (!mlp_alloc_batch(m, n) || !mlp_stage(m, n)) return SD_ERR_DEVICE;
+    HK(hipMemcpy(m->dX, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(m->dT, T, sizeof(float) * MLP_OUT * (size_t)n, hipMemcpyHostToDevice));
+    mlp_batch_grad(m->d, m->dX, m->dT, n, m->st);
+    HK(hipMemcpyAsync(m->d.gb + 3 * MLP_H, m->d.gb4p, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, m->st));
+    if (loss) *loss = mlp_read_loss(m);
+    if (gW) HK(hipMemcpyAsync(gW, m->d.gW, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost, m->st));
+    if (gb) HK(hipMemcpyAsync(gb, m->d.gb, sizeof(float) * MLP_NB, hipMemcpyDeviceToHost, m->st));
+    HK(hipStreamSynchronize(m->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_mlp_train(sd_mlp* m, const float* X, const float* T, int n, int epochs, int batch, float lr,
+                 unsigned seed, float* lossOut) {
+    if (!m || n <= 0 || !X || !T || epochs < 0 || batch <= 0) return SD_ERR_ARG;
+    if (g_failed) return SD_ERR_FAILED;
+    batch = batch > n ? n : batch;
+    if (!mlp_alloc_batch(m, batch) || !mlp_stage(m, n)) return SD_ERR_DEVICE;
+    float *bx = nullptr, *bt = nullptr;
+    if (!dalloc(bx, (size_t)batch * MLP_IN) || !dalloc(bt, (size_t)batch * MLP_OUT)) {
+        dfree(bx);
+        dfree(bt);
+        return SD_ERR_DEVICE;
+    }
+    HK(hipMemcpy(m->dX, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(m->dT, T, sizeof(float) * MLP_OUT * (size_t)n, hipMemcpyHostToDevice));
+    std::vector<int32_t> perm((size_t)n);
+    unsigned long long rs = 0xD1B54A32D192ED03ull ^ seed;
+    const int nb = n / batch;
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        for (int i = n - 1; i > 0; --i) {  // Fisher-Yates with a 64-bit LCG
+            rs = rs * 6364136223846793005ull + 1442695040888963407ull;
+            const int j = (int)((rs >> 33) % (unsigned long long)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        HK(hipMemcpyAsync(m->dIdx, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, m->st));
+        double epochLoss = 0.0;
+        for (int bI = 0; bI < nb; ++bI) {
+            launch_gather_rows(m->dX, m->dIdx + (size_t)bI * batch, batch, MLP_IN, bx, m->st);
+            launch_gather_rows(m->dT, m->dIdx + (size_t)bI * batch, batch, MLP_OUT, bt, m->st);
+            mlp_batch_grad(m->d, bx, bt, batch, m->st);
+            mlp_adam(m->d, lr, ++m->adamStep, m->st);
+            mlp_pack(m->d, m->st);
+            if (bI == nb - 1) epochLoss = mlp_read_loss(m);
+        }
+        if (lossOut) lossOut[e] = (float)epochLoss;
+        HK(hipStreamSynchronize(m->st));
+    }
+    dfree(bx);
+    dfree(bt);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_features(sd_sim* s, const int32_t* targets, int nt, float* X) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (nt <= 0 || !targets || !X) return SD_ERR_ARG;
+    for (int t = 0; t < nt; ++t)
+        if (targets[t] < 0 || targets[t] >= s->allocN) return arg_error("sd_features: target out of range");
+    const bool clearAfter = query_density(s);
+    int32_t* dt = nullptr;
+    float* dx = nullptr;
+    if (!dalloc(dt, (size_t)nt) || !dalloc(dx, (size_t)nt * MLP_IN)) {
+        dfree(dt);
+        dfree(dx);
+        return SD_ERR_DEVICE;
+    }
+    HK(hipMemcpyAsync(dt, targets, sizeof(int32_t) * nt, hipMemcpyHostToDevice, s->st));
+    launch_features(s->pos, dt, nt, s->rhoFix, s->G, s->periodic, dx, s->st);
+    HK(hipMemcpyAsync(X, dx, sizeof(float) * MLP_IN * (size_t)nt, hipMemcpyDeviceToHost, s->st));
+    if (clearAfter) launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    HK(hipStreamSynchronize(s->st));
+    dfree(dt);
+    dfree(dx);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_attach_mlp(sd_sim* s, sd_mlp* m, float outScale) {
+    if (!s) return SD_ERR_ARG;
+    if (m) HK(hipStreamSynchronize(m->st));
+    s->mlp = m;
+    s->mlpOutScale = outScale;
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_mlp_correction(sd_sim* s, float* acc4, int n) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (!s->mlp) return SD_ERR_STATE;
+    if (!acc4 || n < 0 || n > s->allocN) return SD_ERR_ARG;
+    mlp_correction(s);
+    HK(hipMemcpyAsync(acc4, s->accCorr, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, s->st));
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? SD_ERR_FAILED : SD_OK;
 }
 
 // ---------------------------------------------------------------------------

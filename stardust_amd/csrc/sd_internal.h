@@ -69,6 +69,7 @@ struct IntegrateArgs {
     int periodic;
     float c2;
     const float4* accContact;
+    const float4* accCorr;   // learned correction (nullptr = off)
     float softBoundR;
     float darkEnergy;
     StepScalars* sc;

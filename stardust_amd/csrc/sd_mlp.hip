@@ -1,0 +1,494 @@
+// sd_mlp.hip -- learned sub-cell force correction (no reference counterpart).
+//
+// MLP 30 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form), BF16 weights and
+// activations, FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_32x32x16_bf16). Inputs per particle: the 27 CIC-node densities
+// around it (log1p of particles per cell) and its offset from the nearest node
+// (x2); output: the direct-sum minus PM acceleration in units of
+// a_unit * outScale, a_unit = gravity * G / N (one particle at one cell).
+//
+// MFMA operand layout used throughout (32x32x16): lane l holds row (l % 32) of
+// A and column (l % 32) of B with 8 consecutive K values starting at
+// 8 * (l / 32); accumulator element j of lane l is row 8*(j/4) + 4*(l/32) +
+// (j % 4), column l % 32.
+#include <climits>
+
+#include "sd_device.h"
+#include "sd_internal.h"
+#include "sd_mlp.h"
+
+namespace sd {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float gelu_f(float u) {
+    const float c = 0.7978845608028654f;
+    const float x = c * (u + 0.044715f * u * u * u);
+    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * x));
+    return 0.5f * u * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_d(float u) {
+    const float c = 0.7978845608028654f;
+    const float x = c * (u + 0.044715f * u * u * u);
+    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * x));
+    return 0.5f * (1.0f + t) + 0.5f * u * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * u * u);
+}
+
+__device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---- generic NT GEMM: C[M x N] = A[M x K] * B^T, A row-major (lda), Bt = B^T
+// row-major [N x K] (ldb); K % 16 == 0, M % 32 == 0, N % 32 == 0. A block is
+// 2 x 2 waves of 32 x 32 tiles; grid.z splits K into equal slices.
+struct GemmArgs {
+    const __bf16* A;
+    const __bf16* Bt;
+    int M, N, K, lda, ldb;
+    int ksplit;          // K slice per grid.z
+    // epilogue
+    float* C;            // fp32 out [M x N] (+ z * M * N for split-K partials)
+    const float* bias;   // [N]
+    float* U;            // pre-activation out (fp32) [M x N]
+    __bf16* H;           // activation out (bf16) [M x ldh]
+    __bf16* Ht;          // activation out transposed (bf16) [N x ldht]
+    int ldh, ldht;
+    const float* Uin;    // pre-activation in for the GELU derivative
+    float* Df;           // fp32 delta out [M x N]
+};
+
+enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_DGELU = 3 };
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int m0 = blockIdx.y * 64 + (w >> 1) * 32, n0 = blockIdx.x * 64 + (w & 1) * 32;
+    if (m0 >= g.M || n0 >= g.N) return;  // whole wave (no block-level sync below)
+    const int k0 = blockIdx.z * g.ksplit, k1 = min(g.K, k0 + g.ksplit);
+    f32x16 acc = {};
+    const __bf16* ap = g.A + (size_t)(m0 + (l & 31)) * g.lda + 8 * (l >> 5);
+    const __bf16* bp = g.Bt + (size_t)(n0 + (l & 31)) * g.ldb + 8 * (l >> 5);
+    for (int k = k0; k < k1; k += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(ap + k);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + k);
+        acc = mfma32(a, b, acc);
+    }
+    const int col = n0 + (l & 31);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int row = m0 + 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+        const size_t o = (size_t)row * g.N + col;
+        if (EPI == EPI_STORE) {
+            g.C[(size_t)blockIdx.z * g.M * g.N + o] = acc[j];
+        } else if (EPI == EPI_BIAS) {
+            g.C[o] = acc[j] + g.bias[col];
+        } else if (EPI == EPI_BIAS_GELU) {
+            const float u = acc[j] + g.bias[col];
+            g.U[o] = u;
+            const __bf16 h = (__bf16)gelu_f(u);
+            g.H[(size_t)row * g.ldh + col] = h;
+            g.Ht[(size_t)col * g.ldht + row] = h;
+        } else {  // EPI_DGELU: delta = acc * gelu'(U)
+            const float d = acc[j] * gelu_d(g.Uin[o]);
+            g.Df[o] = d;
+            g.H[(size_t)row * g.ldh + col] = (__bf16)d;
+            g.Ht[(size_t)col * g.ldht + row] = (__bf16)d;
+        }
+    }
+}
+
+static void gemm(const GemmArgs& g, int epi, int splits, hipStream_t st) {
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits);
+    switch (epi) {
+        case EPI_STORE: hipLaunchKernelGGL(k_gemm_nt<EPI_STORE>, grid, dim3(256), 0, st, g); break;
+        case EPI_BIAS_GELU: hipLaunchKernelGGL(k_gemm_nt<EPI_BIAS_GELU>, grid, dim3(256), 0, st, g); break;
+        case EPI_BIAS: hipLaunchKernelGGL(k_gemm_nt<EPI_BIAS>, grid, dim3(256), 0, st, g); break;
+        default: hipLaunchKernelGGL(k_gemm_nt<EPI_DGELU>, grid, dim3(256), 0, st, g); break;
+    }
+}
+
+// ---- small elementwise kernels ------------------------------------------------
+// FP32 master parameters -> padded BF16 copies used by the matrix cores:
+// W1p [128 x 32] (inputs padded), W2, W3 [128 x 128], W4p [32 x 128] (outputs
+// padded with zero rows) and transposes W2t, W3t, W4pt [128 x 32].
+__global__ void k_mlp_pack(const float* __restrict__ W, MlpDev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* W1 = W;
+    const float* W2 = W1 + MLP_H * MLP_IN;
+    const float* W3 = W2 + MLP_H * MLP_H;
+    const float* W4 = W3 + MLP_H * MLP_H;
+    if (i < MLP_H * MLP_INP) {  // W1p
+        const int o = i / MLP_INP, k = i % MLP_INP;
+        d.W1p[i] = (__bf16)(k < MLP_IN ? W1[o * MLP_IN + k] : 0.f);
+    }
+    if (i < MLP_H * MLP_H) {
+        const int o = i / MLP_H, k = i % MLP_H;
+        d.W2[i] = (__bf16)W2[i];
+        d.W3[i] = (__bf16)W3[i];
+        d.W2t[k * MLP_H + o] = (__bf16)W2[i];
+        d.W3t[k * MLP_H + o] = (__bf16)W3[i];
+    }
+    if (i < MLP_OUTP * MLP_H) {  // W4p [32 x 128], W4pt [128 x 32]
+        const int o = i / MLP_H, k = i % MLP_H;
+        const float v = o < MLP_OUT ? W4[o * MLP_H + k] : 0.f;
+        d.W4p[i] = (__bf16)v;
+        d.W4pt[k * MLP_OUTP + o] = (__bf16)v;
+    }
+}
+
+// X [n x 30] fp32 -> Xb [nPad x 32] bf16 and Xt [32 x nPad]; rows >= n are zero.
+__global__ void k_mlp_input(const float* __restrict__ X, int n, int nPad, __bf16* Xb, __bf16* Xt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad * MLP_INP) return;
+    const int r = i / MLP_INP, k = i % MLP_INP;
+    const float v = (r < n && k < MLP_IN) ? X[(size_t)r * MLP_IN + k] : 0.f;
+    Xb[i] = (__bf16)v;
+    Xt[(size_t)k * nPad + r] = (__bf16)v;
+}
+
+// dY = (Y - T) / n on the 3 live outputs (padded to 32 columns), loss terms.
+__global__ void k_mlp_dloss(const float* __restrict__ Y, const float* __restrict__ T, int n, int nPad, float invn,
+                            __bf16* dY, __bf16* dYt, float* dYf, double* lossPart) {
+    __shared__ double s[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double lsum = 0.0;
+    if (i < nPad * MLP_OUTP) {
+        const int r = i / MLP_OUTP, c = i % MLP_OUTP;
+        float d = 0.f;
+        if (r < n && c < MLP_OUT) {
+            const float e = Y[(size_t)r * MLP_OUTP + c] - T[(size_t)r * MLP_OUT + c];
+            lsum = 0.5 * (double)e * (double)e * (double)invn;
+            d = e * invn;
+        }
+        dY[i] = (__bf16)d;
+        dYt[(size_t)c * nPad + r] = (__bf16)d;
+        dYf[i] = d;
+    }
+    s[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = s[0];
+}
+
+// Column sums of an fp32 [rows x cols] matrix (bias gradient), fixed order.
+__global__ void k_colsum(const float* __restrict__ D, int rows, int cols, float* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    double s = 0.0;
+    for (int r = 0; r < rows; ++r) s += D[(size_t)r * cols + c];
+    out[c] = (float)s;
+}
+
+// Sum of split-K partials [splits x M x N] into out (fixed order); optional
+// row / column crop to the unpadded weight shape (out is [Mo x No], row-major,
+// taking columns < No of partial rows < Mo).
+__global__ void k_reduce_splits(const float* __restrict__ part, int splits, int M, int N, int Mo, int No,
+                                float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Mo * No) return;
+    const int r = i / No, c = i % No;
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) s += part[((size_t)z * M + r) * N + c];
+    out[i] = s;
+}
+
+// Adam on the FP32 master parameters.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---- features -----------------------------------------------------------------
+// Nearest node c = clamp(floor(p G), 0, G-1) (the node closest to p G - 0.5);
+// features = log1p(particles per cell) at c + {-1,0,1}^3 (x fastest, edge
+// clamp / periodic wrap) and 2 (p G - 0.5 - c).
+__device__ __forceinline__ void particle_features(float4 p, const long long* __restrict__ rho, int G,
+                                                  int periodic, float* f) {
+    const float fG = (float)G;
+    const int cx = min(max((int)floorf(p.x * fG), 0), G - 1);
+    const int cy = min(max((int)floorf(p.y * fG), 0), G - 1);
+    const int cz = min(max((int)floorf(p.z * fG), 0), G - 1);
+    int q = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = wrap_or_clamp(cx + dx, G, periodic), y = wrap_or_clamp(cy + dy, G, periodic),
+                          z = wrap_or_clamp(cz + dz, G, periodic);
+                const long long r = rho[((size_t)z * G + y) * G + x];
+                f[q++] = log1pf((float)r * (1.0f / (float)SD_FIX_ONE));
+            }
+    f[27] = 2.0f * (p.x * fG - 0.5f - (float)cx);
+    f[28] = 2.0f * (p.y * fG - 0.5f - (float)cy);
+    f[29] = 2.0f * (p.z * fG - 0.5f - (float)cz);
+}
+
+__global__ void k_features(const float4* __restrict__ pos, const int32_t* __restrict__ targets, int nt,
+                           const long long* __restrict__ rho, int G, int periodic, float* __restrict__ X) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    float f[MLP_IN];
+    particle_features(pos[targets[t]], rho, G, periodic, f);
+    for (int k = 0; k < MLP_IN; ++k) X[(size_t)t * MLP_IN + k] = f[k];
+}
+
+// ---- fused inference ------------------------------------------------------------
+// Persistent blocks of 4 waves; the padded BF16 weights (W1p, W2, W3, W4p;
+// 80 KiB) live in LDS for the block's lifetime; each wave loops over tiles of
+// 32 particles: features -> 4 MFMA layers with the activations in a per-wave
+// LDS tile (32 x 136 bf16, in place: each layer finishes all of its output
+// tiles in registers before overwriting its input) -> correction written as
+// a float4 per particle (inactive particles get 0).
+// LDS row pitches (bf16): 8 extra elements shift consecutive rows by 4 or 20
+// banks, so the 32 rows a wavefront reads in one MFMA operand load spread
+// over all banks.
+constexpr int ACT_LD = MLP_H + 8;
+constexpr int W1_LD = MLP_INP + 8;
+constexpr int WH_LD = MLP_H + 8;
+
+template <int K, int LDW, int NT>
+__device__ __forceinline__ void lds_layer(const __bf16* __restrict__ act, const __bf16* __restrict__ Wl,
+                                          f32x16* acc, int l) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+#pragma unroll
+    for (int k = 0; k < K; k += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(act + (l & 31) * ACT_LD + k + 8 * (l >> 5));
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wl + (t * 32 + (l & 31)) * LDW + k + 8 * (l >> 5));
+            acc[t] = mfma32(a, b, acc[t]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(MlpInferArgs a) {
+    extern __shared__ unsigned char smem[];
+    __bf16* sW1 = reinterpret_cast<__bf16*>(smem);
+    __bf16* sW2 = sW1 + MLP_H * W1_LD;
+    __bf16* sW3 = sW2 + MLP_H * WH_LD;
+    __bf16* sW4 = sW3 + MLP_H * WH_LD;
+    __bf16* sAct = sW4 + MLP_OUTP * WH_LD;
+    float* sB = reinterpret_cast<float*>(sAct + 4 * 32 * ACT_LD);  // b1 | b2 | b3 | b4 (padded to 32)
+    // weights into padded LDS rows, 16 bytes at a time
+    for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
+        const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
+        *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W1p)[i];
+    }
+    for (int i = threadIdx.x; i < MLP_H * (MLP_H / 8); i += blockDim.x) {
+        const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
+        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W2)[i];
+        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W3)[i];
+    }
+    for (int i = threadIdx.x; i < MLP_OUTP * (MLP_H / 8); i += blockDim.x) {
+        const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
+        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W4p)[i];
+    }
+    for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x)
+        sB[i] = (i < 3 * MLP_H + MLP_OUT) ? a.b[i] : 0.f;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __bf16* act = sAct + w * 32 * ACT_LD;
+    const int ntiles = (a.n + 31) / 32;
+    for (int tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+        // features: lane l fills row l % 32, half (l / 32) of the 32 columns
+        const int r = l & 31, half = l >> 5;
+        const int i = tile * 32 + r;
+        float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
+        if (i < a.n) p = a.pos[i];
+        const bool alive = p.x >= 0.f;
+        {
+            float f[MLP_IN];
+            if (alive) {
+                particle_features(p, a.rho, a.G, a.periodic, f);
+            } else {
+#pragma unroll
+                for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int c = half * 16 + k;
+                act[r * ACT_LD + c] = (__bf16)(c < MLP_IN ? f[c] : 0.f);
+            }
+        }
+        f32x16 acc[4];
+        // layer 1 (K = 32), then layers 2 and 3 (K = 128); each finishes all four
+        // output tiles in registers before overwriting its input rows
+        lds_layer<MLP_INP, W1_LD, 4>(act, sW1, acc, l);
+#pragma unroll
+        for (int layer = 0; layer < 3; ++layer) {
+            if (layer == 1) lds_layer<MLP_H, WH_LD, 4>(act, sW2, acc, l);
+            if (layer == 2) lds_layer<MLP_H, WH_LD, 4>(act, sW3, acc, l);
+            const float* bl = sB + layer * MLP_H;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int row = 8 * (j >> 2) + 4 * (l >> 5) + (j & 3), col = t * 32 + (l & 31);
+                    act[row * ACT_LD + col] = (__bf16)gelu_f(acc[t][j] + bl[col]);
+                }
+        }
+        // layer 4 (one output tile, 3 live columns)
+        lds_layer<MLP_H, WH_LD, 1>(act, sW4, acc, l);
+        const int col = l & 31;
+        if (col <= MLP_OUT) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
+                const int pi = tile * 32 + row;
+                if (pi < a.n) {
+                    float y = 0.f;
+                    if (col < MLP_OUT && a.pos[pi].x >= 0.f) y = (acc[0][j] + sB[3 * MLP_H + col]) * a.scale;
+                    reinterpret_cast<float*>(a.out + pi)[col] = y;
+                }
+            }
+        }
+    }
+}
+
+size_t mlp_infer_smem() {
+    return sizeof(__bf16) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD + 4 * 32 * ACT_LD) +
+           sizeof(float) * (3 * MLP_H + MLP_OUTP);
+}
+
+void launch_mlp_infer(const MlpInferArgs& a, int blocks, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)mlp_infer_smem());
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), mlp_infer_smem(), st, a);
+}
+
+__global__ void k_gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx, int n, int width,
+                              float* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * width) return;
+    const int r = i / width, c = i % width;
+    dst[i] = src[(size_t)idx[r] * width + c];
+}
+
+void launch_gather_rows(const float* src, const int32_t* idx, int n, int width, float* dst, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_rows, dim3((n * width + 255) / 256), dim3(256), 0, st, src, idx, n, width, dst);
+}
+
+void launch_features(const float4* pos, const int32_t* targets, int nt, const long long* rho, int G, int periodic,
+                     float* X, hipStream_t st) {
+    if (nt > 0)
+        hipLaunchKernelGGL(k_features, dim3((nt + 255) / 256), dim3(256), 0, st, pos, targets, nt, rho, G, periodic, X);
+}
+
+// ---- training step ----------------------------------------------------------------
+// One batch: forward (storing U and H, H^T), loss gradient, backward with the
+// GELU derivative fused into the data-gradient GEMMs, weight gradients as
+// split-K GEMMs over the batch, bias gradients as column sums. Gradients land
+// in gW / gb in the oracle's layout (FP32).
+int mlp_pad_rows(int n) { return (n + 255) / 256 * 256; }
+
+void mlp_forward(MlpDev& d, const float* X, int n, hipStream_t st) {
+    const int nPad = mlp_pad_rows(n);
+    hipLaunchKernelGGL(k_mlp_input, dim3((nPad * MLP_INP + 255) / 256), dim3(256), 0, st, X, n, nPad, d.Xb, d.Xt);
+    // forward
+    GemmArgs g{};
+    g.M = nPad;
+    // layer 1
+    g.A = d.Xb; g.lda = MLP_INP; g.Bt = d.W1p; g.ldb = MLP_INP; g.N = MLP_H; g.K = MLP_INP; g.ksplit = MLP_INP;
+    g.bias = d.b; g.U = d.U[0]; g.H = d.Hb[0]; g.Ht = d.Ht[0]; g.ldh = MLP_H; g.ldht = nPad;
+    gemm(g, EPI_BIAS_GELU, 1, st);
+    // layers 2, 3
+    for (int l = 1; l < 3; ++l) {
+        g.A = d.Hb[l - 1]; g.lda = MLP_H; g.Bt = l == 1 ? d.W2 : d.W3; g.ldb = MLP_H; g.K = MLP_H; g.ksplit = MLP_H;
+        g.bias = d.b + l * MLP_H; g.U = d.U[l]; g.H = d.Hb[l]; g.Ht = d.Ht[l];
+        gemm(g, EPI_BIAS_GELU, 1, st);
+    }
+    // layer 4 -> Y [nPad x 32]
+    {
+        GemmArgs o{};
+        o.M = nPad; o.A = d.Hb[2]; o.lda = MLP_H; o.Bt = d.W4p; o.ldb = MLP_H; o.N = MLP_OUTP; o.K = MLP_H;
+        o.ksplit = MLP_H; o.bias = d.b4p; o.C = d.Y;
+        gemm(o, EPI_BIAS, 1, st);
+    }
+}
+
+void mlp_batch_grad(MlpDev& d, const float* X, const float* T, int n, hipStream_t st) {
+    const int nPad = mlp_pad_rows(n);
+    const float invn = 1.0f / (float)(n > 0 ? n : 1);
+    mlp_forward(d, X, n, st);
+    const int lb = (nPad * MLP_OUTP + 255) / 256;
+    hipLaunchKernelGGL(k_mlp_dloss, dim3(lb), dim3(256), 0, st, d.Y, T, n, nPad, invn, d.dYb, d.dYt, d.dYf,
+                       d.lossPart);
+    d.lossBlocks = lb;
+    // backward. delta of layer 4 = dY; dH3 = dY * W4p, then dU3 = dH3 * gelu'(U3)
+    const int splits = nPad >= 4096 ? 16 : 1;
+    const int ks = nPad / splits;
+    // weight gradient of layer 4: dW4 = dY^T [32 x nPad] * H3 [nPad x 128]
+    {
+        GemmArgs w{};
+        w.A = d.dYt; w.lda = nPad; w.Bt = d.Ht[2]; w.ldb = nPad; w.M = MLP_OUTP; w.N = MLP_H; w.K = nPad;
+        w.ksplit = ks; w.C = d.part;
+        gemm(w, EPI_STORE, splits, st);
+        hipLaunchKernelGGL(k_reduce_splits, dim3((MLP_OUT * MLP_H + 255) / 256), dim3(256), 0, st, d.part, splits,
+                           MLP_OUTP, MLP_H, MLP_OUT, MLP_H, d.gW + MLP_H * MLP_IN + 2 * MLP_H * MLP_H);
+        hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, st, d.dYf, nPad, MLP_OUTP, d.gb4p);
+    }
+    // dU3 = (dY * W4p) .* gelu'(U3)  (A = dY [nPad x 32], Bt = W4pt [128 x 32])
+    {
+        GemmArgs b{};
+        b.M = nPad; b.A = d.dYb; b.lda = MLP_OUTP; b.Bt = d.W4pt; b.ldb = MLP_OUTP; b.N = MLP_H; b.K = MLP_OUTP;
+        b.ksplit = MLP_OUTP; b.Uin = d.U[2]; b.Df = d.Df[2]; b.H = d.Db[2]; b.Ht = d.Dt[2]; b.ldh = MLP_H;
+        b.ldht = nPad;
+        gemm(b, EPI_DGELU, 1, st);
+    }
+    for (int l = 2; l >= 0; --l) {
+        // weight gradient of layer l: dW = dU_l^T [128 x nPad] * H_{l-1} [nPad x in]
+        const bool first = l == 0;
+        const int in = first ? MLP_INP : MLP_H;
+        GemmArgs w{};
+        w.A = d.Dt[l]; w.lda = nPad; w.Bt = first ? d.Xt : d.Ht[l - 1]; w.ldb = nPad; w.M = MLP_H; w.N = in;
+        w.K = nPad; w.ksplit = ks; w.C = d.part;
+        gemm(w, EPI_STORE, splits, st);
+        float* gWl = d.gW + (l == 0 ? 0 : MLP_H * MLP_IN + (l - 1) * MLP_H * MLP_H);
+        const int inLive = first ? MLP_IN : MLP_H;
+        hipLaunchKernelGGL(k_reduce_splits, dim3((MLP_H * inLive + 255) / 256), dim3(256), 0, st, d.part, splits,
+                           MLP_H, in, MLP_H, inLive, gWl);
+        hipLaunchKernelGGL(k_colsum, dim3((MLP_H + 63) / 64), dim3(64), 0, st, d.Df[l], nPad, MLP_H,
+                           d.gb + l * MLP_H);
+        if (l == 0) break;
+        // dU_{l-1} = (dU_l * W_l) .* gelu'(U_{l-1})  (Bt = W_l^T [in x out])
+        GemmArgs b{};
+        b.M = nPad; b.A = d.Db[l]; b.lda = MLP_H; b.Bt = l == 1 ? d.W2t : d.W3t; b.ldb = MLP_H; b.N = MLP_H;
+        b.K = MLP_H; b.ksplit = MLP_H; b.Uin = d.U[l - 1]; b.Df = d.Df[l - 1]; b.H = d.Db[l - 1];
+        b.Ht = d.Dt[l - 1]; b.ldh = MLP_H; b.ldht = nPad;
+        gemm(b, EPI_DGELU, 1, st);
+    }
+}
+
+void mlp_pack(MlpDev& d, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_H * MLP_H + 255) / 256), dim3(256), 0, st, d.W, d);
+    // padded output bias b4p
+    hipMemcpyAsync(d.b4p, d.b + 3 * MLP_H, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
+}
+
+void mlp_adam(MlpDev& d, float lr, int t, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    // the output-layer bias gradient lives in gb4p (padded); copy its 3 values
+    hipMemcpyAsync(d.gb + 3 * MLP_H, d.gb4p, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_adam, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, d.W, d.gW, d.mW, d.vW, MLP_NW, lr, b1,
+                       b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(MLP_NB), 0, st, d.b, d.gb, d.mb, d.vb, MLP_NB, lr, b1, b2, eps, c1, c2);
+}
+
+}  // namespace sd

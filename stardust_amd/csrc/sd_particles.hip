@@ -395,6 +395,12 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
                 acc.y += c.y;
                 acc.z += c.z;
             }
+            if (a.accCorr) {
+                const float4 c = a.accCorr[i];
+                acc.x += c.x;
+                acc.y += c.y;
+                acc.z += c.z;
+            }
             if (a.darkEnergy != 0.f) {
                 float k = 1.0f;
                 if (a.softBoundR > 0.f) {

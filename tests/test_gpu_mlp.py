@@ -1,0 +1,142 @@
+"""GPU tests of the learned correction: BF16 MFMA forward / backward against
+the FP64 oracle, the fused per-particle inference against the feature +
+forward path, and training on the GPU. Tolerances are those of BF16 weights
+and activations (8-bit mantissa) with FP32 accumulation."""
+import sys
+
+import numpy as np
+import pytest
+
+import stardust_amd as sd
+
+pytestmark = pytest.mark.gpu
+H = 128
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    nW = 30 * H + 2 * H * H + 3 * H
+    fan = np.concatenate([np.full(30 * H, 30), np.full(2 * H * H, H), np.full(3 * H, H)])
+    W = (rng.uniform(-1, 1, nW) * np.sqrt(6.0 / fan) * scale).astype(np.float32)
+    b = (rng.normal(0, 0.1, 3 * H + 3)).astype(np.float32)
+    return W, b
+
+
+def bf16_round(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def test_mlp_forward_vs_oracle(gpu, oracle):
+    _say("mlp forward")
+    W, b = params(1)
+    m = sd.Mlp(seed=1)
+    m.set_params(W, b)
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (1000, 30)).astype(np.float32)
+    Y = m.forward(X)
+    # FP64 oracle on the BF16-rounded weights and inputs
+    Yr = oracle.mlp_forward(bf16_round(W).astype(np.float64), b.astype(np.float64), H,
+                            bf16_round(X).astype(np.float64))
+    err = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    _say("  rel max err", err)
+    assert err < 3e-2
+    assert np.corrcoef(Y.ravel(), Yr.ravel())[0, 1] > 0.999
+
+
+def test_mlp_gradients_vs_oracle(gpu, oracle):
+    _say("mlp grad")
+    W, b = params(3)
+    m = sd.Mlp(seed=1)
+    m.set_params(W, b)
+    rng = np.random.default_rng(4)
+    n = 2048
+    X = rng.normal(0, 1, (n, 30)).astype(np.float32)
+    T = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    loss, gW, gb = m.grad(X, T)
+    lr, gWr, gbr = oracle.mlp_loss_grad(bf16_round(W).astype(np.float64), b.astype(np.float64), H,
+                                        bf16_round(X).astype(np.float64), T.astype(np.float64))
+    assert loss == pytest.approx(lr, rel=2e-2)
+    offs = [0, 30 * H, 30 * H + H * H, 30 * H + 2 * H * H, 30 * H + 2 * H * H + 3 * H]
+    for l in range(4):
+        g, r = gW[offs[l]:offs[l + 1]], gWr[offs[l]:offs[l + 1]]
+        cos = float(np.dot(g, r) / (np.linalg.norm(g) * np.linalg.norm(r)))
+        rel = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+        _say(f"  layer {l}: cos {cos:.5f} rel {rel:.4f}")
+        assert cos > 0.995 and rel < 0.08
+    rel_b = float(np.linalg.norm(gb - gbr) / np.linalg.norm(gbr))
+    assert rel_b < 0.08
+
+
+def test_mlp_training_reduces_loss(gpu):
+    _say("mlp train")
+    rng = np.random.default_rng(5)
+    n = 32768
+    X = rng.normal(0, 1, (n, 30)).astype(np.float32)
+    A = rng.normal(0, 0.3, (30, 3))
+    T = (np.tanh(X @ A) + 0.1 * X[:, :3]).astype(np.float32)
+    m = sd.Mlp(seed=7)
+    l0, _, _ = m.grad(X[:8192], T[:8192])
+    losses = m.train(X, T, epochs=8, batch=4096, lr=2e-3, seed=3)
+    l1, _, _ = m.grad(X[:8192], T[:8192])
+    _say("  loss", l0, "->", l1, losses)
+    assert l1 < 0.2 * l0
+    assert np.all(np.isfinite(losses))
+
+
+def test_features_and_fused_inference(gpu, oracle):
+    """Features match a numpy restatement on the oracle density; the fused
+    per-particle kernel matches features -> forward -> scale."""
+    _say("mlp features/infer")
+    n, G = 20000, 32
+    rng = np.random.default_rng(6)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.1, 0.9, (n, 3))
+    pos[::50] = (-1, -1, -1, 0)
+    vel = np.zeros_like(pos)
+    gravity = 0.7
+    s = sd.Sim(sd.default_config(particleCount=n, gridSize=G, gravity=gravity, softeningCells=1.0),
+               sd.default_modes())
+    s.set_particles(pos, vel)
+    targets = np.arange(0, n, 7, dtype=np.int32)
+    targets = targets[pos[targets, 0] >= 0]
+    X = s.features(targets)
+    rho = oracle.deposit_fixed(pos, G).astype(np.float64) / oracle.FIX_ONE
+    p = pos[targets, :3]
+    c = np.clip(np.floor(p * np.float32(G)).astype(int), 0, G - 1)
+    ref = np.zeros((targets.size, 30))
+    q = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                x = np.clip(c[:, 0] + dx, 0, G - 1)
+                y = np.clip(c[:, 1] + dy, 0, G - 1)
+                z = np.clip(c[:, 2] + dz, 0, G - 1)
+                ref[:, q] = np.log1p(rho[z, y, x])
+                q += 1
+    ref[:, 27:] = 2.0 * (p.astype(np.float64) * G - 0.5 - c)
+    assert np.abs(X - ref).max() < 1e-5
+    m = sd.Mlp(seed=11)
+    W, b = params(12, scale=0.5)
+    m.set_params(W, b)
+    out_scale = 0.37
+    s.attach_mlp(m, out_scale)
+    corr = s.mlp_correction()
+    a_unit = np.float32(gravity) * G / n
+    Y = m.forward(X) * out_scale * a_unit
+    err = np.abs(corr[targets, :3] - Y).max() / np.abs(Y).max()
+    _say("  fused vs path rel err", err)
+    assert err < 2e-2
+    assert np.all(corr[pos[:, 0] < 0] == 0)
+    # an attached network changes the step (and detaching restores it)
+    s.set_fixed_dt(1e-3)
+    s.run(2)
+    s.attach_mlp(None)
+    s.close()
