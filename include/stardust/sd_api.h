@@ -221,6 +221,10 @@ int sd_mlp_train(sd_mlp* m, const float* X, const float* T, int n, int epochs, i
                  unsigned seed, float* lossPerEpoch);                     /* Adam, shuffled batches */
 int sd_features(sd_sim* s, const int32_t* targets, int nt, float* X);    /* X [nt x 30] */
 int sd_attach_mlp(sd_sim* s, sd_mlp* m, float outScale);  /* m = NULL detaches; steps add the correction */
+/* Recompute the correction every `steps` steps (default 1); it is always
+ * recomputed after a sort or sd_set_particles (the per-particle array follows
+ * the particle order). */
+int sd_set_mlp_interval(sd_sim* s, int steps);
 int sd_mlp_correction(sd_sim* s, float* acc4, int n);      /* correction at current positions, box units */
 
 /* ---- kernel timing (no reference counterpart) ----------------------------

@@ -181,6 +181,7 @@ def _declare(L):
         "sd_features": (C.c_int, [P, P, C.c_int, P]),
         "sd_attach_mlp": (C.c_int, [P, P, C.c_float]),
         "sd_mlp_correction": (C.c_int, [P, P, C.c_int]),
+        "sd_set_mlp_interval": (C.c_int, [P, C.c_int]),
         "sd_kernel_timer_enable": (C.c_int, [P, C.c_char_p]),
         "sd_kernel_time_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     }
@@ -395,6 +396,9 @@ class Sim:
         _check(lib().sd_attach_mlp(self._h, mlp._h if mlp is not None else None, float(out_scale)),
                "sd_attach_mlp")
         self._mlp = mlp  # keep it alive while attached
+
+    def set_mlp_interval(self, steps: int):
+        _check(lib().sd_set_mlp_interval(self._h, int(steps)), "sd_set_mlp_interval")
 
     def mlp_correction(self, n: int | None = None):
         n = self.particle_count() if n is None else n

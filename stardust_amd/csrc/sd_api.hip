@@ -110,6 +110,9 @@ struct sd_sim {
     struct sd_mlp* mlp = nullptr;
     float mlpOutScale = 0.f;
     float4* accCorr = nullptr;
+    int mlpInterval = 1;     // steps between recomputations of the correction
+    bool corrValid = false;  // accCorr matches the current particle order
+    int corrAge = 0;         // steps since it was computed
 
     // timings
     hipEvent_t evA = nullptr, ev1 = nullptr, evB = nullptr;
@@ -394,6 +397,7 @@ static void sort_particles(sd_sim* s, bool forContact = false) {
     std::swap(s->pos, s->posTmp);
     std::swap(s->vel, s->velTmp);
     s->keysSorted = true;
+    s->corrValid = false;  // per-particle correction no longer in particle order
 }
 
 static DtPolicy dt_policy(const sd_sim* s) {
@@ -530,7 +534,7 @@ static bool mlp_correction(sd_sim* s) {
     a.G = s->G;
     a.periodic = s->periodic;
     const MlpDev& d = s->mlp->d;
-    a.W1p = d.W1p; a.W2 = d.W2; a.W3 = d.W3; a.W4p = d.W4p; a.b = d.b;
+    a.W1p = d.W1p; a.W2q = d.W2q; a.W3q = d.W3q; a.W4q = d.W4q; a.b = d.b; a.bq = d.bq;
     a.scale = s->mlpOutScale * s->cfg.gravity * (float)s->G / (float)(s->allocN > 0 ? s->allocN : 1);
     a.out = s->accCorr;
     int cus = sd_device_multiprocessors();
@@ -558,7 +562,16 @@ static void enqueue_step(sd_sim* s) {
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
                          !s->cfg.contactEnabled;
     if (sortDue) sort_particles(s);
-    const bool corr = mlp_correction(s);
+    bool corr = false;
+    if (s->mlp) {
+        if (!s->corrValid || s->corrAge >= s->mlpInterval) {
+            corr = mlp_correction(s);
+            s->corrValid = corr;
+            s->corrAge = 0;
+        } else {
+            corr = true;
+        }
+    }
     if (fused) {
         ensure_mesh(s);
     } else {
@@ -613,6 +626,7 @@ static void enqueue_step(sd_sim* s) {
     }
     s->meshValid = false;
     s->keysSorted = false;
+    s->corrAge++;
     s->stepCount++;
     HK(hipEventRecord(s->evB, s->st));
 }
@@ -1225,6 +1239,7 @@ int sd_set_particles(sd_sim* s, const float* pos4, const float* vel4, int n) {
     s->active = n;
     launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
     s->rhoValid = s->meshValid = s->keysSorted = false;
+    s->corrValid = false;
     // keep simTime, restart the integrator state
     read_scalars(s);
     StepScalars z{};
@@ -1387,7 +1402,8 @@ sd_mlp* sd_mlp_create(unsigned seed) {
               dalloc(d.b4p, MLP_OUTP) && dalloc(d.gb4p, MLP_OUTP) && dalloc(d.W1p, MLP_H * MLP_INP) &&
               dalloc(d.W2, MLP_H * MLP_H) && dalloc(d.W3, MLP_H * MLP_H) && dalloc(d.W4p, MLP_OUTP * MLP_H) &&
               dalloc(d.W2t, MLP_H * MLP_H) && dalloc(d.W3t, MLP_H * MLP_H) && dalloc(d.W4pt, MLP_OUTP * MLP_H) &&
-              dalloc(d.part, 16 * MLP_H * MLP_H);
+              dalloc(d.part, 16 * MLP_H * MLP_H) && dalloc(d.W2q, MLP_H * MLP_H) &&
+              dalloc(d.W3q, MLP_H * MLP_H) && dalloc(d.W4q, MLP_OUTP * MLP_H) && dalloc(d.bq, 3 * MLP_H);
     if (!ok) {
         sd_mlp_destroy(m);
         return nullptr;
@@ -1418,6 +1434,7 @@ void sd_mlp_destroy(sd_mlp* m) {
     auto F = [](auto*& p) { dfree(p); };
     F(d.W); F(d.b); F(d.gW); F(d.gb); F(d.mW); F(d.vW); F(d.mb); F(d.vb); F(d.b4p); F(d.gb4p);
     F(d.W1p); F(d.W2); F(d.W3); F(d.W4p); F(d.W2t); F(d.W3t); F(d.W4pt); F(d.part);
+    F(d.W2q); F(d.W3q); F(d.W4q); F(d.bq);
     F(d.Xb); F(d.Xt); F(d.Y); F(d.dYb); F(d.dYt); F(d.dYf); F(d.lossPart);
     for (int l = 0; l < 3; ++l) { F(d.U[l]); F(d.Hb[l]); F(d.Ht[l]); F(d.Df[l]); F(d.Db[l]); F(d.Dt[l]); }
     F(m->dX); F(m->dT); F(m->dIdx);
@@ -1560,7 +1577,14 @@ int sd_attach_mlp(sd_sim* s, sd_mlp* m, float outScale) {
     if (m) HK(hipStreamSynchronize(m->st));
     s->mlp = m;
     s->mlpOutScale = outScale;
+    s->corrValid = false;
     return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_set_mlp_interval(sd_sim* s, int steps) {
+    if (!s || steps < 1) return SD_ERR_ARG;
+    s->mlpInterval = steps;
+    return SD_OK;
 }
 
 int sd_mlp_correction(sd_sim* s, float* acc4, int n) {
@@ -1569,6 +1593,8 @@ int sd_mlp_correction(sd_sim* s, float* acc4, int n) {
     if (!s->mlp) return SD_ERR_STATE;
     if (!acc4 || n < 0 || n > s->allocN) return SD_ERR_ARG;
     mlp_correction(s);
+    s->corrValid = true;
+    s->corrAge = 0;
     HK(hipMemcpyAsync(acc4, s->accCorr, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, s->st));
     HK(hipStreamSynchronize(s->st));
     return g_failed ? SD_ERR_FAILED : SD_OK;

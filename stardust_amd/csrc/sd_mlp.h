@@ -16,6 +16,8 @@ struct MlpDev {
     float *W, *b, *gW, *gb, *mW, *vW, *mb, *vb;
     float *b4p, *gb4p;  // output bias / gradient padded to 32
     __bf16 *W1p, *W2, *W3, *W4p, *W2t, *W3t, *W4pt;
+    __bf16 *W2q, *W3q, *W4q;  // inference copies, input order permuted (perm_k)
+    float* bq;                // biases of layers 1-3 in that order
     int maxBatch;       // padded rows the work arrays hold
     __bf16 *Xb, *Xt;
     float* U[3];
@@ -36,8 +38,9 @@ struct MlpInferArgs {
     const long long* rho;  // fixed-point density of the current positions
     int G;
     int periodic;
-    const __bf16 *W1p, *W2, *W3, *W4p;
-    const float* b;        // FP32 biases (oracle layout)
+    const __bf16 *W1p, *W2q, *W3q, *W4q;
+    const float* b;        // FP32 biases (oracle layout; the output bias is read from here)
+    const float* bq;       // biases of layers 1-3 in permuted order
     float scale;           // output scale: outScale * a_unit
     float4* out;           // correction (box units), w = 0
 };

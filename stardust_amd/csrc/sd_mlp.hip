@@ -22,17 +22,18 @@ namespace sd {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// GELU, tanh form: 0.5 u (1 + tanh(x)) = u / (1 + exp(-2x)),
+// x = sqrt(2/pi) (u + 0.044715 u^3); one exp and one reciprocal, no divide.
 __device__ __forceinline__ float gelu_f(float u) {
-    const float c = 0.7978845608028654f;
-    const float x = c * (u + 0.044715f * u * u * u);
-    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * x));
-    return 0.5f * u * (1.0f + t);
+    const float x = 0.7978845608028654f * fmaf(0.044715f * u, u * u, u);
+    return u * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * x));
 }
 __device__ __forceinline__ float gelu_d(float u) {
     const float c = 0.7978845608028654f;
-    const float x = c * (u + 0.044715f * u * u * u);
-    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * x));
-    return 0.5f * (1.0f + t) + 0.5f * u * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * u * u);
+    const float x = c * fmaf(0.044715f * u, u * u, u);
+    const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * x));  // 0.5 (1 + tanh x)
+    const float t = 2.0f * sg - 1.0f;
+    return sg + 0.5f * u * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * u * u);
 }
 
 __device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
@@ -108,6 +109,15 @@ static void gemm(const GemmArgs& g, int epi, int splits, hipStream_t st) {
     }
 }
 
+// Neuron held at position q = 16 s + 8 g + e of the register-resident
+// activation vector: accumulator element j = 8 (s % 2) + e of out-tile s / 2
+// on lane group g is row 8 (j / 4) + 4 g + (j % 4) of that tile.
+__host__ __device__ __forceinline__ int perm_k(int q) {
+    const int s = q >> 4, g = (q >> 3) & 1, e = q & 7;
+    const int t = s >> 1, h = s & 1;
+    return 32 * t + 8 * (2 * h + (e >> 2)) + 4 * g + (e & 3);
+}
+
 // ---- small elementwise kernels ------------------------------------------------
 // FP32 master parameters -> padded BF16 copies used by the matrix cores:
 // W1p [128 x 32] (inputs padded), W2, W3 [128 x 128], W4p [32 x 128] (outputs
@@ -134,6 +144,23 @@ __global__ void k_mlp_pack(const float* __restrict__ W, MlpDev d) {
         const float v = o < MLP_OUT ? W4[o * MLP_H + k] : 0.f;
         d.W4p[i] = (__bf16)v;
         d.W4pt[k * MLP_OUTP + o] = (__bf16)v;
+    }
+    // Inference copies with the input (column) order of layers 2-4 permuted to
+    // the accumulator order of the layer before: column q = 16 s + 8 g + e holds
+    // neuron perm_k(q) (see k_mlp_infer); biases of layers 1-3 in that order.
+    if (i < MLP_H * MLP_H) {
+        const int o = i / MLP_H, q = i % MLP_H;
+        const int src = perm_k(q);
+        d.W2q[i] = (__bf16)W2[o * MLP_H + src];
+        d.W3q[i] = (__bf16)W3[o * MLP_H + src];
+    }
+    if (i < MLP_OUTP * MLP_H) {
+        const int o = i / MLP_H, q = i % MLP_H;
+        d.W4q[i] = (__bf16)(o < MLP_OUT ? W4[o * MLP_H + perm_k(q)] : 0.f);
+    }
+    if (i < 3 * MLP_H) {
+        const int l = i / MLP_H, q = i % MLP_H;
+        d.bq[i] = d.b[l * MLP_H + perm_k(q)];
     }
 }
 
@@ -212,7 +239,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 // ---- features -----------------------------------------------------------------
 // Nearest node c = clamp(floor(p G), 0, G-1) (the node closest to p G - 0.5);
-// features = log1p(particles per cell) at c + {-1,0,1}^3 (x fastest, edge
+// features = log(1 + particles per cell) at c + {-1,0,1}^3 (x fastest, edge
 // clamp / periodic wrap) and 2 (p G - 0.5 - c).
 __device__ __forceinline__ void particle_features(float4 p, const long long* __restrict__ rho, int G,
                                                   int periodic, float* f) {
@@ -227,7 +254,7 @@ __device__ __forceinline__ void particle_features(float4 p, const long long* __r
                 const int x = wrap_or_clamp(cx + dx, G, periodic), y = wrap_or_clamp(cy + dy, G, periodic),
                           z = wrap_or_clamp(cz + dz, G, periodic);
                 const long long r = rho[((size_t)z * G + y) * G + x];
-                f[q++] = log1pf((float)r * (1.0f / (float)SD_FIX_ONE));
+                f[q++] = __logf(1.0f + (float)r * (1.0f / (float)SD_FIX_ONE));
             }
     f[27] = 2.0f * (p.x * fG - 0.5f - (float)cx);
     f[28] = 2.0f * (p.y * fG - 0.5f - (float)cy);
@@ -244,121 +271,125 @@ __global__ void k_features(const float4* __restrict__ pos, const int32_t* __rest
 }
 
 // ---- fused inference ------------------------------------------------------------
-// Persistent blocks of 4 waves; the padded BF16 weights (W1p, W2, W3, W4p;
-// 80 KiB) live in LDS for the block's lifetime; each wave loops over tiles of
-// 32 particles: features -> 4 MFMA layers with the activations in a per-wave
-// LDS tile (32 x 136 bf16, in place: each layer finishes all of its output
-// tiles in registers before overwriting its input) -> correction written as
+// Persistent blocks of 8 waves; the BF16 weights (W1p and the permuted W2q,
+// W3q, W4q; padded rows) and biases live in LDS for the block's lifetime.
+// Each wave advances tiles of 32 particles: Y^T = W X^T with the
+// weights as the A operand (from LDS) and
+// the activations as the B operand, which never leave registers: accumulator
+// elements 8h .. 8h+7 of out-tile t are, after bias and GELU, exactly the B
+// fragment of k-step 2t + h of the next layer because that layer's weight
+// columns are stored in the same (permuted) order. Output: the correction as
 // a float4 per particle (inactive particles get 0).
-// LDS row pitches (bf16): 8 extra elements shift consecutive rows by 4 or 20
-// banks, so the 32 rows a wavefront reads in one MFMA operand load spread
-// over all banks.
-constexpr int ACT_LD = MLP_H + 8;
-constexpr int W1_LD = MLP_INP + 8;
+constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (bf16); +8 spreads rows over banks
 constexpr int WH_LD = MLP_H + 8;
+constexpr int INF_WAVES = 12;
 
 template <int K, int LDW, int NT>
-__device__ __forceinline__ void lds_layer(const __bf16* __restrict__ act, const __bf16* __restrict__ Wl,
-                                          f32x16* acc, int l) {
+__device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const bf16x8* B, f32x16* acc, int l) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
 #pragma unroll
-    for (int k = 0; k < K; k += 16) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(act + (l & 31) * ACT_LD + k + 8 * (l >> 5));
+    for (int s = 0; s < K / 16; ++s) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wl + (t * 32 + (l & 31)) * LDW + k + 8 * (l >> 5));
-            acc[t] = mfma32(a, b, acc[t]);
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(Wl + (t * 32 + (l & 31)) * LDW + 16 * s + 8 * (l >> 5));
+            acc[t] = mfma32(a, B[s], acc[t]);
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpInferArgs a) {
+__device__ __forceinline__ void mlp_activate(const f32x16* acc, const float* __restrict__ bq, bf16x8* B, int l) {
+    const int g = l >> 5;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const float4 b0 = *reinterpret_cast<const float4*>(bq + 16 * s + 8 * g);
+        const float4 b1 = *reinterpret_cast<const float4*>(bq + 16 * s + 8 * g + 4);
+        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (__bf16)gelu_f(acc[s >> 1][8 * (s & 1) + e] + bb[e]);
+        B[s] = v;
+    }
+}
+
+// Features of one lane's half (k = 8 g .. 8 g + 7 and 16 + 8 g .. 16 + 8 g + 7).
+__device__ __forceinline__ void feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
+                                              int periodic, int g, bf16x8* B) {
+    float f[MLP_IN];
+    if (alive) {
+        particle_features(p, rho, G, periodic, f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
+    }
+    // static indices only (a lane-dependent index would put f in scratch)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k0 = 16 * s + e, k1 = 16 * s + 8 + e;
+            const float f0 = k0 < MLP_IN ? f[k0] : 0.f;
+            const float f1 = k1 < MLP_IN ? f[k1] : 0.f;
+            v[e] = (__bf16)(g ? f1 : f0);
+        }
+        B[s] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     extern __shared__ unsigned char smem[];
     __bf16* sW1 = reinterpret_cast<__bf16*>(smem);
     __bf16* sW2 = sW1 + MLP_H * W1_LD;
     __bf16* sW3 = sW2 + MLP_H * WH_LD;
     __bf16* sW4 = sW3 + MLP_H * WH_LD;
-    __bf16* sAct = sW4 + MLP_OUTP * WH_LD;
-    float* sB = reinterpret_cast<float*>(sAct + 4 * 32 * ACT_LD);  // b1 | b2 | b3 | b4 (padded to 32)
-    // weights into padded LDS rows, 16 bytes at a time
+    float* sB = reinterpret_cast<float*>(sW4 + MLP_OUTP * WH_LD);  // b1q | b2q | b3q | b4 (padded to 32)
     for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
         const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
         *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W1p)[i];
     }
     for (int i = threadIdx.x; i < MLP_H * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W2)[i];
-        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W3)[i];
+        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W2q)[i];
+        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W3q)[i];
     }
     for (int i = threadIdx.x; i < MLP_OUTP * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W4p)[i];
+        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W4q)[i];
     }
     for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x)
-        sB[i] = (i < 3 * MLP_H + MLP_OUT) ? a.b[i] : 0.f;
+        sB[i] = i < 3 * MLP_H ? a.bq[i] : (i - 3 * MLP_H < MLP_OUT ? a.b[i] : 0.f);
     __syncthreads();
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __bf16* act = sAct + w * 32 * ACT_LD;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5;
     const int ntiles = (a.n + 31) / 32;
-    for (int tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
-        // features: lane l fills row l % 32, half (l / 32) of the 32 columns
-        const int r = l & 31, half = l >> 5;
-        const int i = tile * 32 + r;
+    for (int tile = blockIdx.x * INF_WAVES + w; tile < ntiles; tile += gridDim.x * INF_WAVES) {
+        // keep the weight-fragment LDS reads inside the loop: hoisted out of it
+        // they would need 320 registers per lane
+        __asm__ volatile("" ::: "memory");
+        const int i = tile * 32 + (l & 31);
         float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
         if (i < a.n) p = a.pos[i];
-        const bool alive = p.x >= 0.f;
-        {
-            float f[MLP_IN];
-            if (alive) {
-                particle_features(p, a.rho, a.G, a.periodic, f);
-            } else {
-#pragma unroll
-                for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
-            }
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int c = half * 16 + k;
-                act[r * ACT_LD + c] = (__bf16)(c < MLP_IN ? f[c] : 0.f);
-            }
-        }
+        bf16x8 B[8];
         f32x16 acc[4];
-        // layer 1 (K = 32), then layers 2 and 3 (K = 128); each finishes all four
-        // output tiles in registers before overwriting its input rows
-        lds_layer<MLP_INP, W1_LD, 4>(act, sW1, acc, l);
-#pragma unroll
-        for (int layer = 0; layer < 3; ++layer) {
-            if (layer == 1) lds_layer<MLP_H, WH_LD, 4>(act, sW2, acc, l);
-            if (layer == 2) lds_layer<MLP_H, WH_LD, 4>(act, sW3, acc, l);
-            const float* bl = sB + layer * MLP_H;
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int row = 8 * (j >> 2) + 4 * (l >> 5) + (j & 3), col = t * 32 + (l & 31);
-                    act[row * ACT_LD + col] = (__bf16)gelu_f(acc[t][j] + bl[col]);
-                }
-        }
-        // layer 4 (one output tile, 3 live columns)
-        lds_layer<MLP_H, WH_LD, 1>(act, sW4, acc, l);
-        const int col = l & 31;
-        if (col <= MLP_OUT) {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int row = 8 * (j >> 2) + 4 * (l >> 5) + (j & 3);
-                const int pi = tile * 32 + row;
-                if (pi < a.n) {
-                    float y = 0.f;
-                    if (col < MLP_OUT && a.pos[pi].x >= 0.f) y = (acc[0][j] + sB[3 * MLP_H + col]) * a.scale;
-                    reinterpret_cast<float*>(a.out + pi)[col] = y;
-                }
-            }
+        feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
+        mlp_layer<MLP_INP, W1_LD, 4>(sW1, B, acc, l);
+        mlp_activate(acc, sB, B, l);
+        mlp_layer<MLP_H, WH_LD, 4>(sW2, B, acc, l);
+        mlp_activate(acc, sB + MLP_H, B, l);
+        mlp_layer<MLP_H, WH_LD, 4>(sW3, B, acc, l);
+        mlp_activate(acc, sB + 2 * MLP_H, B, l);
+        mlp_layer<MLP_H, WH_LD, 1>(sW4, B, acc, l);
+        // outputs 0..2 are accumulator rows 0..2 (elements 0..2 on lanes 0..31)
+        if (g == 0 && i < a.n) {
+            const float* b4 = sB + 3 * MLP_H;
+            const float sc = p.x >= 0.f ? a.scale : 0.f;
+            a.out[i] = make_float4((acc[0][0] + b4[0]) * sc, (acc[0][1] + b4[1]) * sc, (acc[0][2] + b4[2]) * sc, 0.f);
         }
     }
 }
 
 size_t mlp_infer_smem() {
-    return sizeof(__bf16) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD + 4 * 32 * ACT_LD) +
+    return sizeof(__bf16) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD) +
            sizeof(float) * (3 * MLP_H + MLP_OUTP);
 }
 
@@ -369,7 +400,7 @@ void launch_mlp_infer(const MlpInferArgs& a, int blocks, hipStream_t st) {
                                   (int)mlp_infer_smem());
         attr = true;
     }
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), mlp_infer_smem(), st, a);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), mlp_infer_smem(), st, a);
 }
 
 __global__ void k_gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx, int n, int width,
