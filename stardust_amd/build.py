@@ -1,4 +1,6 @@
-"""Builds the gfx950 shared library ``libstardust_amd.so`` in-tree with hipcc.
+"""Builds the gfx950 shared library ``libstardust_amd.so`` in-tree with hipcc,
+then the C++ ``Sim`` shim over it (``libstardust_sim.so``, g++) and the C++
+regression-check binary ``tests/cpp/sim_tests``.
 
 Usage: ``python -m stardust_amd.build`` (or ``stardust_amd.build.build()``).
 """
@@ -14,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstardust_amd.so")
+SIMLIB = os.path.join(HERE, "libstardust_sim.so")
+SIM_TESTS = os.path.join(HERE, "..", "tests", "cpp", "sim_tests")
 
 SOURCES = ["sd_fft.hip", "sd_sort.hip", "sd_particles.hip", "sd_api.hip", "sd_mlp.hip", "sd_batch.hip"]
 # The per-particle and API kernels restate the reference's arithmetic with
@@ -57,9 +61,29 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp()
     if verbose:
-        print(f"[stardust_amd] built {LIB}")
+        print(f"[stardust_amd] built {LIB}, {SIMLIB}")
     return LIB
+
+
+def _gxx(args: list[str]) -> None:
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra"] + args, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed:\n{r.stdout}\n{r.stderr}")
+
+
+def build_cpp() -> None:
+    """Host-only C++: the Sim shim links the HIP library by file name and finds
+    it beside itself ($ORIGIN); the test binary finds both in stardust_amd/."""
+    _gxx(["-shared", "-fPIC", os.path.join(HERE, "cpp", "Sim.cpp"), "-o", SIMLIB + ".tmp",
+          "-L" + HERE, "-lstardust_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(SIMLIB + ".tmp", SIMLIB)
+    src = os.path.join(HERE, "..", "tests", "cpp", "sim_tests.cpp")
+    if os.path.exists(src):
+        _gxx([src, "-o", SIM_TESTS + ".tmp", "-L" + HERE, "-lstardust_sim", "-lstardust_amd",
+              "-Wl,-rpath,$ORIGIN/../../stardust_amd"])
+        os.replace(SIM_TESTS + ".tmp", SIM_TESTS)
 
 
 if __name__ == "__main__":
