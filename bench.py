@@ -51,7 +51,74 @@ def cpu_baseline(cfg, n_steps, threads):
     return n * n_steps / dt, dt
 
 
+def copy_calibration(mib):
+    """Plain device-to-device copies of `mib` MiB (the runtime's copy kernel),
+    so a counter run sees a kernel of known traffic beside k_integrate."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), nbytes) or hip.hipMalloc(C.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(a, 1, nbytes)
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, nbytes, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in the copy calibration")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {mib} MiB device-to-device")
+
+
+def _counter_rows(path, counter):
+    import csv
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            key = (r["Dispatch_Id"], r["Kernel_Name"])
+            out[key] = out.get(key, 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def traffic_summary(fetch_csv, write_csv, mib, out_path, command):
+    """HBM bytes per k_integrate launch from two counter runs (FETCH_SIZE and
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for the copy kernel of `mib` MiB (reads mib, writes mib)."""
+    res = {}
+    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = _counter_rows(path, name)
+        integ = [v for (d, k), v in rows.items() if "k_integrate" in k]
+        # the calibration copies are the last three copy-kernel dispatches
+        copy = [v for (d, k), v in sorted(rows.items(), key=lambda kv: int(kv[0][0])) if "copyBuffer" in k][-3:]
+        if not integ or len(copy) < 3:
+            raise RuntimeError(f"{path}: no k_integrate or calibration copy dispatches for {name}")
+        integ_kib = sorted(integ)[len(integ) // 2]
+        copy_kib = sorted(copy)[1]
+        factor = (mib * 1024.0) / copy_kib
+        res[name] = {"k_integrate_kib_median": integ_kib, "copy_kib_median": copy_kib,
+                     "copy_kib_last3": copy, "scale": factor, "bytes": integ_kib * 1024.0 * factor,
+                     "dispatches": len(integ)}
+    total = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
+    doc = {"kernel": "sd::k_integrate", "workload": "two-disk 4,194,304 particles, G=128",
+           "copy_calibration_mib": mib, "counters": res, "hbm_bytes_per_launch": round(total),
+           "command": command}
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1)
+    print(json.dumps(doc, indent=1))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
+        # offline: bench.py --traffic-summary FETCH.csv WRITE.csv MIB OUT.json "COMMAND"
+        fetch_csv, write_csv, mib, out_path, command = sys.argv[2:7]
+        traffic_summary(fetch_csv, write_csv, int(mib), out_path, command)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
@@ -60,6 +127,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="integrate", help="kernel timed for the roofline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not provided",
@@ -142,6 +211,8 @@ def main():
     }
     print(json.dumps(out), flush=True)
     sim.close()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
 
 if __name__ == "__main__":

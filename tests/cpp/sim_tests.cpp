@@ -4,6 +4,7 @@
 // header no longer has (SpiralDisk); here the same scenes are set up through
 // Sim::setParticles with a deterministic disk / cloud generator. Thresholds
 // are the reference's. Exit 2 when no device is visible, as the reference.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -127,9 +128,12 @@ static void testGravityResponds() {
         SimConfig cfg = plain(300000, 128);
         cfg.gravity = gravity;
         sim.init(cfg);
-        SimConfig seedCfg = cfg;
-        seedCfg.gravity = 2.5f;  // same initial disk in both runs
-        seed_disk(sim, seedCfg, 2);
+        // the same cold disk (no rotation) in both runs: without gravity it
+        // stays put, with gravity it contracts
+        std::vector<float> p, v;
+        spiral_disk(cfg.particleCount, sim.gridSize(), 2.5f, 2, p, v);
+        std::fill(v.begin(), v.end(), 0.f);
+        sim.setParticles(p.data(), v.data(), cfg.particleCount);
         for (int i = 0; i < 900; ++i) sim.step();
         return sim.measureMaxDensity();
     };
