@@ -227,6 +227,35 @@ int sd_attach_mlp(sd_sim* s, sd_mlp* m, float outScale);  /* m = NULL detaches; 
 int sd_set_mlp_interval(sd_sim* s, int steps);
 int sd_mlp_correction(sd_sim* s, float* acc4, int n);      /* correction at current positions, box units */
 
+/* ---- batched scenes (no reference counterpart: one scene per Sim) --------
+ * Many independent simulations stepped together: every kernel of the fused
+ * step covers all scenes in one launch (meshes grouped by size), and a run of
+ * `steps` steps is recorded into one HIP graph and launched once. Each scene
+ * is an ordinary sd_sim (sd_batch_scene) whose particles live in the batch's
+ * arrays: set its particles / fixed dt through the sd_* calls, read it the
+ * same way. A batched scene evolves bit-identically to the same scene stepped
+ * alone with exactGrid = 1. Scenes must share the sort interval and step
+ * count, use the fused pipeline and fixed-point deposit, and have no contact
+ * or learned correction. The batch owns its scenes (do not sd_destroy them). */
+typedef struct sd_batch sd_batch;
+sd_batch* sd_batch_create(void);
+void sd_batch_destroy(sd_batch* b);
+int sd_batch_add_scene(sd_batch* b, const sd_config* cfg);  /* scene index, or < 0 */
+int sd_batch_init(sd_batch* b);                             /* allocates every scene */
+int sd_batch_scene_count(const sd_batch* b);
+sd_sim* sd_batch_scene(sd_batch* b, int i);
+long long sd_batch_total_particles(const sd_batch* b);
+/* Enqueue `steps` steps of every scene as one graph launch (recorded on first
+ * use, replayed while steps, snapshotEvery and the scenes' configuration are
+ * unchanged); positions are copied to snapshot k after step (k+1)*snapshotEvery
+ * (0 = none). Asynchronous; scene calls are ordered after it. */
+int sd_batch_run(sd_batch* b, int steps, int snapshotEvery);
+int sd_batch_synchronize(sd_batch* b);
+float sd_batch_last_run_ms(sd_batch* b);                    /* device time of the last run */
+long long sd_batch_graph_nodes(const sd_batch* b);          /* nodes of the recorded graph */
+int sd_batch_snapshot_count(const sd_batch* b);
+int sd_batch_get_snapshot(sd_batch* b, int snap, int scene, float* pos4, int n);
+
 /* ---- kernel timing (no reference counterpart) ----------------------------
  * Brackets every launch of one named kernel with HIP events on the handle's
  * stream; sd_kernel_time_ms returns total milliseconds and launch count. */

@@ -182,6 +182,19 @@ def _declare(L):
         "sd_attach_mlp": (C.c_int, [P, P, C.c_float]),
         "sd_mlp_correction": (C.c_int, [P, P, C.c_int]),
         "sd_set_mlp_interval": (C.c_int, [P, C.c_int]),
+        "sd_batch_create": (P, []),
+        "sd_batch_destroy": (None, [P]),
+        "sd_batch_add_scene": (C.c_int, [P, C.POINTER(Config)]),
+        "sd_batch_init": (C.c_int, [P]),
+        "sd_batch_scene_count": (C.c_int, [P]),
+        "sd_batch_scene": (P, [P, C.c_int]),
+        "sd_batch_total_particles": (C.c_longlong, [P]),
+        "sd_batch_run": (C.c_int, [P, C.c_int, C.c_int]),
+        "sd_batch_synchronize": (C.c_int, [P]),
+        "sd_batch_last_run_ms": (C.c_float, [P]),
+        "sd_batch_graph_nodes": (C.c_longlong, [P]),
+        "sd_batch_snapshot_count": (C.c_int, [P]),
+        "sd_batch_get_snapshot": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int]),
         "sd_kernel_timer_enable": (C.c_int, [P, C.c_char_p]),
         "sd_kernel_time_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     }
@@ -242,9 +255,18 @@ class Sim:
         if config is not None:
             self.init(config)
 
+    @classmethod
+    def _view(cls, handle, owner):
+        """A Sim over a handle owned elsewhere (a batch scene): never destroyed here."""
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owner = owner
+        return self
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().sd_destroy(self._h)
+            if getattr(self, "_owner", None) is None:
+                lib().sd_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -485,3 +507,61 @@ def fail_message() -> str:
 
 def clear_failure():
     lib().sd_clear_failure()
+
+
+class Batch:
+    """Many independent scenes stepped together (one launch per kernel for all
+    scenes, a whole run recorded into one HIP graph). ``scene(i)`` is a
+    :class:`Sim` view of scene i for particle I/O and measures."""
+
+    def __init__(self, configs):
+        self._h = lib().sd_batch_create()
+        if not self._h:
+            raise StardustError("sd_batch_create failed")
+        for c in configs:
+            rc = lib().sd_batch_add_scene(self._h, C.byref(c))
+            if rc < 0:
+                _check(rc, "sd_batch_add_scene")
+        _check(lib().sd_batch_init(self._h), "sd_batch_init")
+        self.scenes = [Sim._view(lib().sd_batch_scene(self._h, i), self) for i in range(len(configs))]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for s in self.scenes:
+                s._h = None
+            lib().sd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scene(self, i) -> Sim:
+        return self.scenes[i]
+
+    @property
+    def total_particles(self) -> int:
+        return int(lib().sd_batch_total_particles(self._h))
+
+    def run(self, steps: int, snapshot_every: int = 0):
+        _check(lib().sd_batch_run(self._h, int(steps), int(snapshot_every)), "sd_batch_run")
+
+    def synchronize(self):
+        _check(lib().sd_batch_synchronize(self._h), "sd_batch_synchronize")
+
+    def last_run_ms(self) -> float:
+        return float(lib().sd_batch_last_run_ms(self._h))
+
+    def graph_nodes(self) -> int:
+        return int(lib().sd_batch_graph_nodes(self._h))
+
+    def snapshot_count(self) -> int:
+        return int(lib().sd_batch_snapshot_count(self._h))
+
+    def snapshot(self, snap: int, scene: int):
+        n = self.scenes[scene].particle_count()
+        out = np.empty((n, 4), np.float32)
+        _check(lib().sd_batch_get_snapshot(self._h, snap, scene, out.ctypes.data, n), "sd_batch_get_snapshot")
+        return out

@@ -17,6 +17,7 @@
 #include "../../include/stardust/sd_api.h"
 #include "sd_internal.h"
 #include "sd_mlp.h"
+#include "sd_sim.h"
 
 using namespace sd;
 
@@ -26,7 +27,7 @@ using namespace sd;
 static bool g_failed = false;
 static char g_failMsg[256] = {0};
 
-static void mark_failure(const char* fmt, ...) {
+void sd_mark_failure(const char* fmt, ...) {
     if (g_failed) return;
     g_failed = true;
     va_list ap;
@@ -36,89 +37,17 @@ static void mark_failure(const char* fmt, ...) {
     fprintf(stderr, "[stardust_amd] failure: %s\n", g_failMsg);
 }
 
-#define HK(x)                                                                              \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess) {                                                            \
-            mark_failure("%s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__);         \
-        }                                                                                  \
-    } while (0)
+bool sd_failure_set() { return g_failed; }
 
-static int arg_error(const char* what) {
+int sd_arg_error(const char* what) {
     // argument errors are reported but are not sticky
     snprintf(g_failMsg, sizeof(g_failMsg), "%s", what);
     fprintf(stderr, "[stardust_amd] invalid argument: %s\n", what);
     return SD_ERR_ARG;
 }
 
-// ---------------------------------------------------------------------------
-// the handle
-// ---------------------------------------------------------------------------
-struct KernelTimer {
-    std::string name;
-    std::vector<hipEvent_t> pool;  // start/stop pairs
-    int used = 0;
-    double totalMs = 0.0;
-    long long launches = 0;
-};
-
-struct sd_sim {
-    sd_config cfg{};
-    sd_modes modes{};
-    bool ready = false;
-    int G = 0, S = 0, P = 0, periodic = 0;
-    int allocN = 0;  // slots (mass per particle = gravity / allocN)
-    int active = 0;  // slots in use (prefix), for state files
-    float fixedDt = 0.f;
-    hipStream_t st = nullptr;
-
-    // particles
-    float4 *pos = nullptr, *vel = nullptr, *posTmp = nullptr, *velTmp = nullptr;
-    uint32_t *keysA = nullptr, *keysB = nullptr;
-    int32_t *idxA = nullptr, *idxB = nullptr;
-    uint32_t *hist = nullptr, *chunk = nullptr;
-    SortBuffers sb{};
-    bool keysSorted = false;
-
-    // mesh
-    long long* rhoFix = nullptr;
-    float* rhoF = nullptr;
-    float2 *A = nullptr, *B = nullptr;
-    float* green = nullptr;
-    float* pot = nullptr;
-    float4* accG = nullptr;
-    float2* tw = nullptr;
-    float4* accContact = nullptr;
-    int32_t *cellStart = nullptr, *cellEnd = nullptr;
-    long long cellTableCells = 0;
-    MeshPlan plan{};
-
-    // reductions
-    ReducePartial* partials = nullptr;
-    int nparts = 0;     // one-particle-per-thread reductions
-    int npartsInt = 0;  // k_integrate partials
-    StepScalars* sc = nullptr;
-    StepScalars* scHost = nullptr;  // pinned mirror for reads
-
-    // state
-    bool rhoValid = false;   // rhoFix holds the deposit of the current positions
-    bool meshValid = false;  // accG matches the current positions
-    bool primed = false;     // sc->dt / sc->kick are set for the next integrate
-    long long stepCount = 0;
-
-    // learned correction
-    struct sd_mlp* mlp = nullptr;
-    float mlpOutScale = 0.f;
-    float4* accCorr = nullptr;
-    int mlpInterval = 1;     // steps between recomputations of the correction
-    bool corrValid = false;  // accCorr matches the current particle order
-    int corrAge = 0;         // steps since it was computed
-
-    // timings
-    hipEvent_t evA = nullptr, ev1 = nullptr, evB = nullptr;
-    sd_timings tm{};
-    KernelTimer timer;
-};
+#define mark_failure sd_mark_failure
+#define arg_error sd_arg_error
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -197,7 +126,9 @@ static void timer_harvest(sd_sim* s) {
     } while (0)
 
 static void free_buffers(sd_sim* s) {
-    dfree(s->pos); dfree(s->vel); dfree(s->posTmp); dfree(s->velTmp);
+    if (!s->borrowed) {
+        dfree(s->pos); dfree(s->vel); dfree(s->posTmp); dfree(s->velTmp);
+    }
     dfree(s->keysA); dfree(s->keysB); dfree(s->idxA); dfree(s->idxB);
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
@@ -286,7 +217,8 @@ static int allocate(sd_sim* s) {
     const int G = s->G, S = s->S, P = s->P;
     const size_t N = (size_t)(s->allocN > 0 ? s->allocN : 1);
     const size_t cells = (size_t)G * G * G;
-    bool ok = dalloc(s->pos, N) && dalloc(s->vel, N) && dalloc(s->posTmp, N) && dalloc(s->velTmp, N) &&
+    bool ok = (s->borrowed || (dalloc(s->pos, N) && dalloc(s->vel, N) && dalloc(s->posTmp, N) &&
+                               dalloc(s->velTmp, N))) &&
               dalloc(s->keysA, N) && dalloc(s->keysB, N) && dalloc(s->idxA, N) && dalloc(s->idxB, N);
     const int ntiles = sort_tiles_for((int)N);
     ok = ok && dalloc(s->hist, (size_t)256 * ntiles) && dalloc(s->chunk, (size_t)sort_chunks_for((int)N) + 1);
@@ -394,8 +326,14 @@ static void sort_particles(sd_sim* s, bool forContact = false) {
         radix_sort_pairs(s->sb, n, key_bits(s), s->st);
         launch_reorder(s->pos, s->vel, s->sb.idx, s->posTmp, s->velTmp, n, s->st);
     });
-    std::swap(s->pos, s->posTmp);
-    std::swap(s->vel, s->velTmp);
+    if (s->borrowed) {
+        // a batch owns the arrays: keep their addresses, copy the result back
+        launch_copy_f4(s->posTmp, s->pos, n, s->st);
+        launch_copy_f4(s->velTmp, s->vel, n, s->st);
+    } else {
+        std::swap(s->pos, s->posTmp);
+        std::swap(s->vel, s->velTmp);
+    }
     s->keysSorted = true;
     s->corrValid = false;  // per-particle correction no longer in particle order
 }
@@ -660,6 +598,27 @@ static int check_ready(const sd_sim* s) {
     return SD_OK;
 }
 
+int sd_check_ready(const sd_sim* s) { return check_ready(s); }
+DtPolicy sd_dt_policy(const sd_sim* s) { return dt_policy(s); }
+float sd_light_speed_sq(float lengthScale, float timeUnitScale) {
+    return light_speed_sq(lengthScale, timeUnitScale);
+}
+
+void sd_prepare_fused(sd_sim* s) {
+    ensure_rho(s);
+    if (!s->primed) prime_scalars(s);
+}
+
+void sd_note_fused_steps(sd_sim* s, int steps) {
+    s->rhoValid = true;
+    s->meshValid = false;
+    s->primed = true;
+    s->keysSorted = false;
+    s->corrValid = false;
+    s->corrAge += steps;
+    s->stepCount += steps;
+}
+
 // ---------------------------------------------------------------------------
 // defaults / statics
 // ---------------------------------------------------------------------------
@@ -825,7 +784,10 @@ int sd_set_modes(sd_sim* s, const sd_modes* m) {
         // hand the state over synchronised: pending momentum correction and the
         // leapfrog half kick applied, integrator restarted
         synced_velocity(s);
-        std::swap(s->vel, s->velTmp);
+        if (s->borrowed)
+            launch_copy_f4(s->velTmp, s->vel, s->allocN, s->st);
+        else
+            std::swap(s->vel, s->velTmp);
         hipLaunchKernelGGL(k_restart_integrator, dim3(1), dim3(1), 0, s->st, s->sc);
         launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
         s->rhoValid = false;
@@ -863,6 +825,8 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
                             c->particleCount != s->allocN ||
                             (!s->periodic && c->softeningCells != s->cfg.softeningCells) ||
                             (s->periodic && c->softeningCells != s->cfg.softeningCells);
+    if (structural && s->ready && s->borrowed)
+        return arg_error("a scene of a batch keeps its particle count, grid and boundary");
     s->cfg = *c;
     s->cfg.gridSize = G;
     if (!structural) return SD_OK;

@@ -33,6 +33,19 @@ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
     return morton_spread10(x) | (morton_spread10(y) << 1) | (morton_spread10(z) << 2);
 }
 
+// Job of a block in a batched launch: the j with prefix[j] <= b < prefix[j + 1]
+// (prefix[0] = 0, nondecreasing; empty jobs are skipped). The search runs on
+// uniform values, so the loads are scalar.
+__device__ __forceinline__ int find_job(const int* __restrict__ prefix, int njobs, int b) {
+    int lo = 0, hi = njobs;  // prefix[lo] <= b < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Wavefront-wide reductions (64 lanes on CDNA).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -276,13 +276,13 @@ struct RowsFwdArgs {
 };
 
 template <int N, int SRC>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2* __restrict__ tw, int bx) {
     constexpr int LP = Line<N>::LP;
     constexpr int THREADS = TB * Plan<N>::T;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
-    const int row0 = blockIdx.x * TB;
+    const int row0 = bx * TB;
     // zero the padded half of every line
     for (int i = threadIdx.x; i < TB * N; i += THREADS) {
         const int c = i / N, x = i - c * N;
@@ -349,6 +349,20 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, con
     }
 }
 
+template <int N, int SRC>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
+    rows_fwd_body<N, SRC>(a, tw, blockIdx.x);
+}
+
+// One launch over the X pass of many meshes of the same FFT side: blockIdx.y
+// selects the mesh (every job has the same row count).
+template <int N>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd_batch(const RowsFwdArgs* __restrict__ jobs,
+                                                                    const float2* __restrict__ tw) {
+    const RowsFwdArgs a = jobs[blockIdx.y];
+    rows_fwd_body<N, 0>(a, tw, blockIdx.x);
+}
+
 // ---- X' pass: rows, half spectrum -> real output --------------------------
 struct RowsInvArgs {
     const float2* in;
@@ -359,13 +373,13 @@ struct RowsInvArgs {
 };
 
 template <int N>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2* __restrict__ tw, int bx) {
     constexpr int LP = Line<N>::LP;
     constexpr int THREADS = TB * Plan<N>::T;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
-    const int row0 = blockIdx.x * TB;
+    const int row0 = bx * TB;
     constexpr int KP2 = N / 4 + 1;
     constexpr int ITERS = (TB * KP2 + THREADS - 1) / THREADS;
     float4 buf[ITERS];
@@ -407,6 +421,18 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, con
     }
 }
 
+template <int N>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
+    rows_inv_body<N>(a, tw, blockIdx.x);
+}
+
+template <int N>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv_batch(const RowsInvArgs* __restrict__ jobs,
+                                                                    const float2* __restrict__ tw) {
+    const RowsInvArgs a = jobs[blockIdx.y];
+    rows_inv_body<N>(a, tw, blockIdx.x);
+}
+
 // ---- column passes ---------------------------------------------------------
 // Column b of slab a: element n at in[a*inA + n*inN + b]; output k at
 // out[a*outA + k*outN + b]. A block owns TB adjacent columns of one slab; each
@@ -431,7 +457,7 @@ struct ColArgs {
 };
 
 template <int N, int MODE>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float2* __restrict__ tw) {
+__device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
     constexpr int LP = Line<N>::LP;
     constexpr int THREADS = TB * Plan<N>::T;
     constexpr int PAIRS = TB / 2;
@@ -439,10 +465,9 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float
     constexpr int ITERS = (N + RPI - 1) / RPI;
     __shared__ float2 s_line[TB * LP];
     __shared__ float2 s_tw[N];
-    const int slab = blockIdx.y;
     if (MODE == COL_GREEN && slab > N / 2) return;  // only ky <= S/2 is stored
     load_twiddles<N>(s_tw, tw);
-    const int b0 = blockIdx.x * TB;
+    const int b0 = bx * TB;
     const int pc = threadIdx.x % PAIRS, r0 = threadIdx.x / PAIRS;
     const int col = 2 * pc;
     const bool ok0 = b0 + col < a.nb, ok1 = b0 + col + 1 < a.nb;
@@ -545,13 +570,35 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float
     }
 }
 
+template <int N, int MODE>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float2* __restrict__ tw) {
+    cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y);
+}
+
+// Column pass of many meshes of the same FFT side and mesh size: blockIdx.z
+// selects the mesh.
+template <int N, int MODE>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_cols_batch(const ColArgs* __restrict__ jobs,
+                                                                const float2* __restrict__ tw) {
+    const ColArgs a = jobs[blockIdx.z];
+    cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y);
+}
+
+struct GradJob {
+    const float* pot;
+    float4* acc;
+    int G;
+    float h;
+    int periodic;
+    int pad;
+};
+
 // ---- gradient: kGridAccel, Sim.cu:589-611 ----------------------------------
 // a = -(pot[+1] - pot[-1]) * (0.5 * G * potScale); isolated clamps the stencil,
 // periodic wraps. pot and acc are G^3 [z][y][x].
-__global__ void __launch_bounds__(256) k_grid_accel(const float* __restrict__ pot, float4* __restrict__ acc,
-                                                    int G, float h, int periodic) {
+__device__ __forceinline__ void grid_accel_body(const float* __restrict__ pot, float4* __restrict__ acc, int G,
+                                                float h, int periodic, long long i) {
     const long long cells = (long long)G * G * G;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cells) return;
     const int x = (int)(i % G), y = (int)((i / G) % G), z = (int)(i / ((long long)G * G));
     int xm, xp, ym, yp, zm, zp;
@@ -571,11 +618,24 @@ __global__ void __launch_bounds__(256) k_grid_accel(const float* __restrict__ po
     acc[i] = make_float4(ax, ay, az, 0.f);
 }
 
+__global__ void __launch_bounds__(256) k_grid_accel(const float* __restrict__ pot, float4* __restrict__ acc,
+                                                    int G, float h, int periodic) {
+    grid_accel_body(pot, acc, G, h, periodic, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Gradient of many meshes of the same size: blockIdx.y selects the mesh.
+__global__ void __launch_bounds__(256) k_grid_accel_batch(const GradJob* __restrict__ jobs) {
+    const GradJob j = jobs[blockIdx.y];
+    grid_accel_body(j.pot, j.acc, j.G, j.h, j.periodic, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 }  // namespace sd
 
 // ---------------------------------------------------------------------------
 // Host launchers (C linkage inside the library; declared in sd_internal.h)
 // ---------------------------------------------------------------------------
+#include <vector>
+
 #include "sd_internal.h"
 
 namespace sd {
@@ -728,6 +788,120 @@ void mesh_grid_accel(const float* pot, float4* acc, int G, float potScale, int p
     const float h = 0.5f * (float)G * potScale;
     hipLaunchKernelGGL(k_grid_accel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, pot,
                        acc, G, h, periodic);
+}
+
+// ---------------------------------------------------------------------------
+// Mesh groups: meshes of one size (G, S, boundary) solved together, one launch
+// per pass for the whole group. Each job is exactly the single-mesh solve
+// (same kernels, same arguments), so every mesh's result is bit-identical to
+// its own mesh_rows_fwd + mesh_solve_rest + mesh_grid_accel.
+// ---------------------------------------------------------------------------
+struct MeshGroup {
+    int G, S, P, periodic, n;
+    const float2* tw;
+    RowsFwdArgs* xf;
+    ColArgs *yf, *z, *yi;
+    RowsInvArgs* xi;
+    GradJob* grad;
+    void* block;  // one device allocation holding every table
+};
+
+MeshGroup* mesh_group_create(const MeshGroupJob* jobs, int n) {
+    if (n <= 0) return nullptr;
+    const MeshPlan& m0 = jobs[0].plan;
+    for (int j = 1; j < n; ++j) {
+        const MeshPlan& m = jobs[j].plan;
+        if (m.G != m0.G || m.S != m0.S || m.P != m0.P || m.periodic != m0.periodic) return nullptr;
+    }
+    if (!fft_length_supported(m0.S)) return nullptr;
+    const int G = m0.G, S = m0.S, P = m0.P, NK = S / 2 + 1;
+    std::vector<RowsFwdArgs> xf(n);
+    std::vector<ColArgs> yf(n), z(n), yi(n);
+    std::vector<RowsInvArgs> xi(n);
+    std::vector<GradJob> gr(n);
+    for (int j = 0; j < n; ++j) {
+        const MeshGroupJob& J = jobs[j];
+        const MeshPlan& m = J.plan;
+        RowsFwdArgs& a = xf[j];
+        a = RowsFwdArgs{};
+        a.src = J.rhoFix; a.out = J.A; a.rows = G * G; a.nin = G; a.P = P; a.clearSrc = 1;
+        ColArgs& b = yf[j];
+        b = ColArgs{};
+        b.in = J.A; b.out = J.B;
+        b.inA = (long long)G * P; b.inN = P; b.outA = (long long)S * P; b.outN = P;
+        b.nb = NK; b.nin = G; b.nout = S; b.P = P; b.S = S;
+        ColArgs& c = z[j];
+        c = ColArgs{};
+        c.in = J.B; c.out = J.B;
+        c.inA = P; c.inN = (long long)S * P; c.outA = P; c.outN = (long long)S * P;
+        c.nb = NK; c.nin = G; c.nout = G; c.P = P; c.S = S;
+        c.green = J.green; c.perScale = m.perScale; c.perSoft = m.perSoft;
+        ColArgs& d = yi[j];
+        d = ColArgs{};
+        d.in = J.B; d.out = J.A;
+        d.inA = (long long)S * P; d.inN = P; d.outA = (long long)G * P; d.outN = P;
+        d.nb = NK; d.nin = S; d.nout = G; d.P = P; d.S = S;
+        RowsInvArgs& e = xi[j];
+        e = RowsInvArgs{};
+        e.in = J.A; e.out = J.pot; e.rows = G * G; e.P = P; e.nout = G;
+        gr[j] = GradJob{J.pot, J.accG, G, 0.5f * (float)G * J.potScale, m.periodic, 0};
+    }
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t oXf = 0, oYf = oXf + al(sizeof(RowsFwdArgs) * n), oZ = oYf + al(sizeof(ColArgs) * n),
+                 oYi = oZ + al(sizeof(ColArgs) * n), oXi = oYi + al(sizeof(ColArgs) * n),
+                 oGr = oXi + al(sizeof(RowsInvArgs) * n), total = oGr + al(sizeof(GradJob) * n);
+    char* d = nullptr;
+    if (hipMalloc((void**)&d, total) != hipSuccess) return nullptr;
+    bool ok = hipMemcpy(d + oXf, xf.data(), sizeof(RowsFwdArgs) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + oYf, yf.data(), sizeof(ColArgs) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + oZ, z.data(), sizeof(ColArgs) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + oYi, yi.data(), sizeof(ColArgs) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + oXi, xi.data(), sizeof(RowsInvArgs) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d + oGr, gr.data(), sizeof(GradJob) * n, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    MeshGroup* g = new MeshGroup();
+    g->G = G; g->S = S; g->P = P; g->periodic = m0.periodic; g->n = n; g->tw = m0.tw;
+    g->xf = (RowsFwdArgs*)(d + oXf);
+    g->yf = (ColArgs*)(d + oYf);
+    g->z = (ColArgs*)(d + oZ);
+    g->yi = (ColArgs*)(d + oYi);
+    g->xi = (RowsInvArgs*)(d + oXi);
+    g->grad = (GradJob*)(d + oGr);
+    g->block = d;
+    return g;
+}
+
+void mesh_group_destroy(MeshGroup* g) {
+    if (!g) return;
+    (void)hipFree(g->block);
+    delete g;
+}
+
+template <int N>
+static void group_solve(const MeshGroup& g, hipStream_t st) {
+    const dim3 blk(TB * Plan<N>::T);
+    const int NK = g.S / 2 + 1, rowBlocks = (g.G * g.G + TB - 1) / TB, colBlocks = (NK + TB - 1) / TB;
+    hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(rowBlocks, g.n), blk, 0, st, g.xf, g.tw);
+    hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
+    if (g.periodic)
+        hipLaunchKernelGGL((k_cols_batch<N, COL_Z_PER>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
+    else
+        hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
+    hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
+    hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(rowBlocks, g.n), blk, 0, st, g.xi, g.tw);
+    const long long cells = (long long)g.G * g.G * g.G;
+    hipLaunchKernelGGL(k_grid_accel_batch, dim3((unsigned)((cells + 255) / 256), g.n), dim3(256), 0, st, g.grad);
+}
+
+int mesh_group_solve(const MeshGroup* g, hipStream_t st) {
+    if (!g) return -1;
+#define CALL(NN) group_solve<NN>(*g, st)
+    SD_DISPATCH_N(g->S, CALL)
+#undef CALL
+    return 0;
 }
 
 }  // namespace sd

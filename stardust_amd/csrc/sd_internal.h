@@ -29,6 +29,23 @@ int mesh_build_green(const MeshPlan& m, float softCells, float2* tmpA, float2* t
 void mesh_grid_accel(const float* pot, float4* acc, int G, float potScale, int periodic,
                      hipStream_t st);
 
+// Meshes of one size (G, S, boundary) solved with one launch per pass. Each job
+// is one mesh's buffers; the solve consumes (clears) rhoFix and writes accG.
+struct MeshGroupJob {
+    MeshPlan plan;
+    long long* rhoFix;
+    float2* A;
+    float2* B;
+    const float* green;
+    float* pot;
+    float4* accG;
+    float potScale;
+};
+struct MeshGroup;
+MeshGroup* mesh_group_create(const MeshGroupJob* jobs, int n);  // nullptr if the jobs differ in size
+void mesh_group_destroy(MeshGroup* g);
+int mesh_group_solve(const MeshGroup* g, hipStream_t st);
+
 // ---- keys / sort ------------------------------------------------------------
 struct SortBuffers {
     uint32_t* keys;     // n
@@ -41,6 +58,18 @@ struct SortBuffers {
 };
 void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
                       hipStream_t st);
+// Keys of many particle sets held in one array: job j covers particles
+// base .. base + n - 1 and its keys get `tag` above the cell key bits, so a
+// stable sort keeps every set in place and orders each set as its own sort would.
+struct KeyJob {
+    int base;
+    int n;
+    int G;
+    int mode;
+    uint32_t tag;  // job index << key bits
+};
+void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* prefix, int njobs,
+                            int totalBlocks, uint32_t* keys, int32_t* idx, hipStream_t st);
 int sort_tiles_for(int n);
 int sort_chunks_for(int n);
 // Stable LSD sort of (keys, idx) by the low `bits` bits. The result is in
@@ -99,6 +128,20 @@ struct DtPolicy {
 void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol,
                      StepScalars* sc, hipStream_t st);
 void launch_subtract_mean(const float4* pos, float4* vel, int n, StepScalars* sc, hipStream_t st);
+
+// Many simulations in one launch: job j owns integrate blocks
+// prefix[j] .. prefix[j + 1] - 1 (prefix[njobs] = totalBlocks).
+void launch_integrate_batch(const IntegrateArgs* jobs, const int* prefix, int njobs, int totalBlocks,
+                            hipStream_t st);
+struct FinalizeJob {
+    const ReducePartial* partials;
+    int nparts;
+    int mode;
+    DtPolicy pol;
+    StepScalars* sc;
+};
+void launch_finalize_batch(const FinalizeJob* jobs, int njobs, hipStream_t st);
+void launch_copy_f4(const float4* src, float4* dst, long long n, hipStream_t st);
 void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G,
                           int periodic, const StepScalars* sc, float4* velOut, hipStream_t st);
 

@@ -308,7 +308,7 @@ __device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, co
 
 // Block reduction of (sum v, count, max |v|) into one partial per block.
 __device__ __forceinline__ void block_partial(double sx, double sy, double sz, double cnt, float vmax,
-                                              ReducePartial* out) {
+                                              ReducePartial* out, int blk) {
     __shared__ double s_d[PT / 64][4];
     __shared__ float s_f[PT / 64];
     sx = wave_sum(sx);
@@ -337,7 +337,7 @@ __device__ __forceinline__ void block_partial(double sx, double sy, double sz, d
             r.vmax = fmaxf(r.vmax, s_f[k]);
         }
         r.pad[0] = r.pad[1] = r.pad[2] = 0.f;
-        out[blockIdx.x] = r;
+        out[blk] = r;
     }
 }
 
@@ -348,7 +348,7 @@ __device__ __forceinline__ float speed3(float x, float y, float z) {
 // kIntegrate, Sim.cu:685-906 (plain PM: no black holes, no pressure). Each
 // thread advances KP particles (i = block * BP + k * PT + thread, coalesced for
 // every k); the new positions feed the block deposit when depositTo is set.
-__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
+__device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) {
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
     bool depAlive[KP];
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
     bool act[KP];
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-        const int i = blockIdx.x * BP + k * PT + threadIdx.x;
+        const int i = blk * BP + k * PT + threadIdx.x;
         act[k] = i < a.n;
         P[k] = act[k] ? a.pos[i] : make_float4(-1.f, -1.f, -1.f, 0.f);
         V[k] = act[k] ? a.vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
         depAlive[k] = false;
         depC[k].ix = depC[k].iy = depC[k].iz = 0;
         depC[k].fx = depC[k].fy = depC[k].fz = 0.f;
-        const int i = blockIdx.x * BP + k * PT + threadIdx.x;
+        const int i = blk * BP + k * PT + threadIdx.x;
         if (!act[k]) continue;
         float4 p = P[k];
         if (p.x >= 0.f) {
@@ -491,7 +491,19 @@ __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
         }
     }
     if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
-    if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials);
+    if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials, blk);
+}
+
+__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) { integrate_body(a, blockIdx.x); }
+
+// Integrate of many simulations in one launch: block b belongs to job j with
+// prefix[j] <= b < prefix[j + 1] and runs that job's block b - prefix[j], so
+// every job sees exactly its own single-launch decomposition.
+__global__ void __launch_bounds__(PT) k_integrate_batch(const IntegrateArgs* __restrict__ jobs,
+                                                        const int* __restrict__ prefix, int njobs) {
+    const int j = find_job(prefix, njobs, (int)blockIdx.x);
+    const IntegrateArgs a = jobs[j];
+    integrate_body(a, (int)blockIdx.x - prefix[j]);
 }
 
 int integrate_blocks(int n) { return (n + BP - 1) / BP; }
@@ -511,7 +523,7 @@ __global__ void __launch_bounds__(PT) k_vmax(const float4* __restrict__ pos, con
         const float4 v = vel[i];
         vm = speed3(v.x, v.y, v.z);
     }
-    block_partial(0.0, 0.0, 0.0, 0.0, vm, partials);
+    block_partial(0.0, 0.0, 0.0, 0.0, vm, partials, blockIdx.x);
 }
 
 // kMomentumAccum (Sim.cu:2252-2268) as block partials.
@@ -528,7 +540,7 @@ __global__ void __launch_bounds__(PT) k_momentum(const float4* __restrict__ pos,
             cnt = 1.0;
         }
     }
-    block_partial(sx, sy, sz, cnt, 0.f, partials);
+    block_partial(sx, sy, sz, cnt, 0.f, partials, blockIdx.x);
 }
 
 void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* partials, hipStream_t st) {
@@ -540,8 +552,8 @@ void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial*
 
 // Deterministic reduction (fixed summation order) + dt policy.
 constexpr int FIN = 1024;
-__global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
-                                                  DtPolicy pol, StepScalars* sc) {
+__device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ parts, int np, int mode,
+                                              const DtPolicy& pol, StepScalars* sc) {
     __shared__ double s_d[FIN][4];
     __shared__ float s_f[FIN];
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
@@ -603,9 +615,42 @@ __global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restric
     }
 }
 
+__global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
+                                                  DtPolicy pol, StepScalars* sc) {
+    finalize_body(parts, np, mode, pol, sc);
+}
+
+// One block per simulation.
+__global__ void __launch_bounds__(FIN) k_finalize_batch(const FinalizeJob* __restrict__ jobs) {
+    const FinalizeJob j = jobs[blockIdx.x];
+    finalize_body(j.partials, j.nparts, j.mode, j.pol, j.sc);
+}
+
 void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol, StepScalars* sc,
                      hipStream_t st) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN), 0, st, partials, nparts, mode, pol, sc);
+}
+
+void launch_integrate_batch(const IntegrateArgs* jobs, const int* prefix, int njobs, int totalBlocks,
+                            hipStream_t st) {
+    if (totalBlocks <= 0) return;
+    hipLaunchKernelGGL(k_integrate_batch, dim3(totalBlocks), dim3(PT), 0, st, jobs, prefix, njobs);
+}
+
+void launch_finalize_batch(const FinalizeJob* jobs, int njobs, hipStream_t st) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_finalize_batch, dim3(njobs), dim3(FIN), 0, st, jobs);
+}
+
+__global__ void __launch_bounds__(256) k_copy_f4(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                 long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void launch_copy_f4(const float4* src, float4* dst, long long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
 }
 
 // kSubtractMeanVelDev (Sim.cu:2282-2291); the pending mean is cleared after use

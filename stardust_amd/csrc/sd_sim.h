@@ -1,0 +1,107 @@
+// sd_sim.h -- the simulation handle behind the C ABI and the library-internal
+// helpers the batch driver shares with it. Not installed; include/stardust/sd_api.h
+// is the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/stardust/sd_api.h"
+#include "sd_internal.h"
+
+// Sticky failure state (Sim.cu:52-75): the first device error is kept until
+// sd_clear_failure; argument errors are reported but not sticky.
+#define SD_INTERNAL __attribute__((visibility("hidden")))
+SD_INTERNAL void sd_mark_failure(const char* fmt, ...);
+SD_INTERNAL bool sd_failure_set();
+SD_INTERNAL int sd_arg_error(const char* what);
+
+#define HK(x)                                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            sd_mark_failure("%s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__);      \
+        }                                                                                  \
+    } while (0)
+
+struct KernelTimer {
+    std::string name;
+    std::vector<hipEvent_t> pool;  // start/stop pairs
+    int used = 0;
+    double totalMs = 0.0;
+    long long launches = 0;
+};
+
+struct sd_sim {
+    sd_config cfg{};
+    sd_modes modes{};
+    bool ready = false;
+    int G = 0, S = 0, P = 0, periodic = 0;
+    int allocN = 0;  // slots (mass per particle = gravity / allocN)
+    int active = 0;  // slots in use (prefix), for state files
+    float fixedDt = 0.f;
+    hipStream_t st = nullptr;
+
+    // particles (borrowed: the four arrays are slices of a batch's storage,
+    // which owns them; their addresses never change)
+    float4 *pos = nullptr, *vel = nullptr, *posTmp = nullptr, *velTmp = nullptr;
+    bool borrowed = false;
+    uint32_t *keysA = nullptr, *keysB = nullptr;
+    int32_t *idxA = nullptr, *idxB = nullptr;
+    uint32_t *hist = nullptr, *chunk = nullptr;
+    sd::SortBuffers sb{};
+    bool keysSorted = false;
+
+    // mesh
+    long long* rhoFix = nullptr;
+    float* rhoF = nullptr;
+    float2 *A = nullptr, *B = nullptr;
+    float* green = nullptr;
+    float* pot = nullptr;
+    float4* accG = nullptr;
+    float2* tw = nullptr;
+    float4* accContact = nullptr;
+    int32_t *cellStart = nullptr, *cellEnd = nullptr;
+    long long cellTableCells = 0;
+    sd::MeshPlan plan{};
+
+    // reductions
+    sd::ReducePartial* partials = nullptr;
+    int nparts = 0;     // one-particle-per-thread reductions
+    int npartsInt = 0;  // k_integrate partials
+    sd::StepScalars* sc = nullptr;
+    sd::StepScalars* scHost = nullptr;  // pinned mirror for reads
+
+    // state
+    bool rhoValid = false;   // rhoFix holds the deposit of the current positions
+    bool meshValid = false;  // accG matches the current positions
+    bool primed = false;     // sc->dt / sc->kick are set for the next integrate
+    long long stepCount = 0;
+
+    // learned correction
+    struct sd_mlp* mlp = nullptr;
+    float mlpOutScale = 0.f;
+    float4* accCorr = nullptr;
+    int mlpInterval = 1;     // steps between recomputations of the correction
+    bool corrValid = false;  // accCorr matches the current particle order
+    int corrAge = 0;         // steps since it was computed
+
+    // timings
+    hipEvent_t evA = nullptr, ev1 = nullptr, evB = nullptr;
+    sd_timings tm{};
+    KernelTimer timer;
+};
+
+
+// Shared with the batch driver.
+SD_INTERNAL int sd_check_ready(const sd_sim* s);
+SD_INTERNAL sd::DtPolicy sd_dt_policy(const sd_sim* s);
+SD_INTERNAL float sd_light_speed_sq(float lengthScale, float timeUnitScale);
+// Brings a simulation to the state a fused step starts from (density of the
+// current positions deposited, dt and kick of the next step set); enqueued on
+// the simulation's stream.
+SD_INTERNAL void sd_prepare_fused(sd_sim* s);
+// Host bookkeeping after `steps` fused steps were run for s by other code.
+SD_INTERNAL void sd_note_fused_steps(sd_sim* s, int steps);

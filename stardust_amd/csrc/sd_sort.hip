@@ -55,6 +55,41 @@ __global__ void __launch_bounds__(256) k_cell_keys(const float4* __restrict__ po
     keys[i] = k + 1u;
 }
 
+// Keys of many particle sets in one launch (see KeyJob): the key of set j is
+// its own k_cell_keys key with the set's tag above it; idx is the global index.
+__global__ void __launch_bounds__(256) k_cell_keys_batch(const float4* __restrict__ pos,
+                                                         const KeyJob* __restrict__ jobs,
+                                                         const int* __restrict__ prefix, int njobs,
+                                                         uint32_t* __restrict__ keys,
+                                                         int32_t* __restrict__ idx) {
+    const int j = find_job(prefix, njobs, (int)blockIdx.x);
+    const KeyJob J = jobs[j];
+    const int li = ((int)blockIdx.x - prefix[j]) * 256 + (int)threadIdx.x;
+    if (li >= J.n) return;
+    const int i = J.base + li;
+    const float4 p = pos[i];
+    idx[i] = i;
+    if (p.x < 0.f) {
+        keys[i] = J.tag;
+        return;
+    }
+    const int G = J.G;
+    const float fG = (float)G;
+    int cx, cy, cz;
+    if (J.mode == 2) {
+        cx = min(max((int)floorf(fmaf(p.x, fG, -0.5f)), 0), G - 1);
+        cy = min(max((int)floorf(fmaf(p.y, fG, -0.5f)), 0), G - 1);
+        cz = min(max((int)floorf(fmaf(p.z, fG, -0.5f)), 0), G - 1);
+    } else {
+        cx = min(max((int)(p.x * fG), 0), G - 1);
+        cy = min(max((int)(p.y * fG), 0), G - 1);
+        cz = min(max((int)(p.z * fG), 0), G - 1);
+    }
+    const uint32_t k = J.mode ? morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)
+                              : (uint32_t)((cz * G + cy) * G + cx);
+    keys[i] = J.tag | (k + 1u);
+}
+
 // Lanes of this wavefront whose digit equals mine (8-bit digits).
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     uint64_t m = active;
@@ -250,6 +285,12 @@ void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys,
                       hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pos, n, G, mode, keys, idx);
+}
+
+void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* prefix, int njobs,
+                            int totalBlocks, uint32_t* keys, int32_t* idx, hipStream_t st) {
+    if (totalBlocks <= 0) return;
+    hipLaunchKernelGGL(k_cell_keys_batch, dim3(totalBlocks), dim3(256), 0, st, pos, jobs, prefix, njobs, keys, idx);
 }
 
 int sort_tiles_for(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }

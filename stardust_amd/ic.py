@@ -149,3 +149,26 @@ def zeldovich(n_side=256, seed=1, rms_cells=0.3, G_mesh=512, index=-2.0):
     p4[:, :3] -= np.floor(p4[:, :3])
     v4 = np.zeros((n, 4), np.float32)
     return p4, v4
+
+
+def config3(n_scenes=64, seed=3, n_range=(2 ** 15, 2 ** 20), meshes=(64, 96, 128)):
+    """Batched ensemble: ``n_scenes`` independent disks as config 0, particle
+    counts N_i = round(exp(U[ln lo, ln hi])), meshes G_i drawn uniformly from
+    ``meshes`` (S = 2 G), disk seed = scene index. Returns a list of
+    (n, G, seed) triples; ``disk_scene`` builds each."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    out = []
+    for i in range(n_scenes):
+        n = int(round(math.exp(rng.uniform(lo, hi))))
+        G = int(meshes[int(rng.integers(len(meshes)))])
+        out.append((n, G, i))
+    return out
+
+
+def disk_scene(n, G, seed, L=12.0):
+    """One config-0 disk of n particles on a G mesh (box side L, dt 1e-3)."""
+    pos, vel = exp_disk(n, seed=seed)
+    p4, v4 = to_box(pos, vel, (0.0, 0.0, 0.0), L)
+    return dict(pos=p4, vel=v4, G=G, L=L, M=1.0, dt=1e-3,
+                gravity=gravity_isolated(G, 1.0, L), soft=softening_cells(G))
