@@ -28,11 +28,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def integrate_bytes(n_live, G):
+def integrate_bytes(n_live, G, contact=False):
     """Algorithmic bytes of one k_integrate launch: pos + vel read and written
     (64 B per particle) + the acceleration grid read once (16 B per cell) + the
-    fixed-point density written once (8 B per cell)."""
-    return 64.0 * n_live + 24.0 * G ** 3
+    fixed-point density written once (8 B per cell) (+ the contact acceleration,
+    16 B per particle, when contact is on)."""
+    return (80.0 if contact else 64.0) * n_live + 24.0 * G ** 3
 
 
 def cpu_baseline(cfg, n_steps, threads):
@@ -127,6 +128,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="integrate", help="kernel timed for the roofline")
+    ap.add_argument("--workload", choices=("two_disk", "cosmic_web"), default="two_disk",
+                    help="two_disk: the headline 4M-particle G=128 step; cosmic_web: 16.7M particles, "
+                         "periodic G=512 with contact")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -140,14 +144,36 @@ def main():
     from stardust_amd import ic
 
     log("building initial conditions")
-    cfg = ic.config1(G=args.grid)
-    n = cfg["pos"].shape[0]
-    G = cfg["G"]
-    conf = sd.default_config(particleCount=n, gridSize=G, gravity=cfg["gravity"],
-                             softeningCells=cfg["soft"], sortInterval=cfg["sort_interval"])
-    sim = sd.Sim(conf, sd.default_modes())
-    sim.set_fixed_dt(cfg["dt"])
-    sim.set_particles(cfg["pos"], cfg["vel"])
+    contact = args.workload == "cosmic_web"
+    if contact:
+        # 256^3 lattice, Zel'dovich displacements (P(k) ~ k^-2, rms 0.3 cells), v = 0,
+        # periodic G = 512, contact radius 0.2 cells, re-sort every step, CFL dt
+        pos, vel = ic.zeldovich(256, seed=1, rms_cells=0.3, G_mesh=512)
+        cfg = {"pos": pos, "vel": vel, "G": 512}
+        n, G = pos.shape[0], 512
+        conf = sd.default_config(particleCount=n, gridSize=G, boundary=sd.BOUNDARY_PERIODIC,
+                                 gravity=ic.gravity_periodic(G, 1.0, 1.0), softeningCells=3.0, sortInterval=1,
+                                 contactEnabled=1, contactRadiusCells=0.2)
+        sim = sd.Sim(conf, sd.default_modes())
+        sim.set_particles(pos, vel)
+        desc = ("cosmic web: 256^3 lattice (16,777,216 particles) with Zel'dovich displacements "
+                "(P(k)~k^-2, rms 0.3 cells, v=0), periodic G=512, contact radius 0.2 cells, "
+                "re-sort every step, CFL dt")
+        metric = "particle-steps/s (16.7M, periodic 512^3 PM + contact, 1 GPU)"
+        fft_side = G
+    else:
+        cfg = ic.config1(G=args.grid)
+        n = cfg["pos"].shape[0]
+        G = cfg["G"]
+        conf = sd.default_config(particleCount=n, gridSize=G, gravity=cfg["gravity"],
+                                 softeningCells=cfg["soft"], sortInterval=cfg["sort_interval"])
+        sim = sd.Sim(conf, sd.default_modes())
+        sim.set_fixed_dt(cfg["dt"])
+        sim.set_particles(cfg["pos"], cfg["vel"])
+        desc = (f"two-disk encounter, 2x2,097,152 particles, G={G} / S={2 * G} isolated PM, dt=1e-3, "
+                "Morton re-sort every 10 steps")
+        metric = "particle-steps/s (4M, 256^3 PM, 1 GPU)"
+        fft_side = 2 * G
     log(f"N={n} G={G} warmup={args.warmup}")
     sim.run(args.warmup)
     sim.synchronize()
@@ -169,11 +195,11 @@ def main():
     roof = None
     if launches > 0:
         avg_s = kms / launches / 1e3
-        nbytes = integrate_bytes(n_live, G)
+        nbytes = integrate_bytes(n_live, G, contact)
         ach = nbytes / avg_s / 1e9
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic_k_integrate_two_disk_4M_G128.json")
-        if os.path.exists(tpath):
+        if not contact and args.kernel == "integrate" and G == 128 and os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -182,7 +208,7 @@ def main():
                 "algorithmic_bytes_per_launch": nbytes}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not contact:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle reference step, {args.cpu_steps} steps, {threads} threads")
         v, secs = cpu_baseline(cfg, args.cpu_steps, threads)
@@ -191,7 +217,7 @@ def main():
                          f"warm-up step ({secs:.1f} s): FP64 FFT mesh + FP32 particle kernels"}
 
     out = {
-        "metric": "particle-steps/s (4M, 256^3 PM, 1 GPU)",
+        "metric": metric,
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -203,9 +229,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "two-disk encounter, 2x2,097,152 particles, G=128 / S=256 "
-                               "isolated PM, dt=1e-3, Morton re-sort every 10 steps",
-                   "particles": n, "mesh": G, "fft_side": 2 * G, "parallelism": "single GPU"},
+        "config": {"workload": desc, "particles": n, "mesh": G, "fft_side": fft_side,
+                   "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

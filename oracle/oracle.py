@@ -235,6 +235,23 @@ def sample_acc(acc4, p, G, periodic=False):
     return out
 
 
+def particle_accel(params, pos):
+    """Mesh acceleration at each particle (compute_accel, then the CIC gather);
+    zero for inactive slots. Returns float32 [n, 4]."""
+    pos = _p4(pos)
+    acc = compute_accel(params, pos)
+    flat = np.ascontiguousarray(acc, dtype=np.float32).reshape(-1)
+    out = np.zeros((pos.shape[0], 4), np.float32)
+    o3 = np.empty(3, np.float32)
+    L = lib()
+    G, per = params.G, int(params.periodic)
+    for i in range(pos.shape[0]):
+        if pos[i, 0] >= 0.0:
+            L.sdo_sample_acc(flat, float(pos[i, 0]), float(pos[i, 1]), float(pos[i, 2]), G, per, o3)
+            out[i, :3] = o3
+    return out
+
+
 def max_speed(pos, vel):
     pos, vel = _p4(pos), _p4(vel)
     return float(lib().sdo_max_speed(pos, vel, pos.shape[0]))

@@ -22,11 +22,24 @@ namespace sd {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// GELU, tanh form: 0.5 u (1 + tanh(x)) = u / (1 + exp(-2x)),
-// x = sqrt(2/pi) (u + 0.044715 u^3); one exp and one reciprocal, no divide.
+// GELU, tanh form: 0.5 u (1 + T(u)), T(u) = tanh(sqrt(2/pi) (u + 0.044715 u^3)).
+// Evaluated without transcendental instructions (quarter rate on CDNA): for
+// |u| <= 4, T(u) = u R(u^2) with R a degree-6 polynomial (weighted least
+// squares fit to T(u)/u, constrained to T(4) = 1); the clamped input makes
+// T = sign(u) beyond. Written as u (0.5 + uc R(uc^2) / 2): |error| <= 2.2e-4 for
+// every u (FP32 evaluation), below the BF16 rounding of the activations it
+// feeds. 10 full-rate VALU ops.
 __device__ __forceinline__ float gelu_f(float u) {
-    const float x = 0.7978845608028654f * fmaf(0.044715f * u, u * u, u);
-    return u * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * x));
+    const float uc = __builtin_amdgcn_fmed3f(u, -4.0f, 4.0f);
+    const float t = uc * uc;
+    float r = 2.3469173271450927e-08f;  // coefficients of R / 2
+    r = fmaf(r, t, -1.6407460634582094e-06f);
+    r = fmaf(r, t, 4.893512959824875e-05f);
+    r = fmaf(r, t, -0.0008244823548011482f);
+    r = fmaf(r, t, 0.00881875865161419f);
+    r = fmaf(r, t, -0.06459682434797287f);
+    r = fmaf(r, t, 0.39771363139152527f);
+    return u * fmaf(uc, r, 0.5f);
 }
 __device__ __forceinline__ float gelu_d(float u) {
     const float c = 0.7978845608028654f;
@@ -284,10 +297,27 @@ constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (bf16); +8 spreads rows o
 constexpr int WH_LD = MLP_H + 8;
 constexpr int INF_WAVES = 12;
 
+// One layer, Y^T = W X^T: the accumulators start from the layer's biases
+// (bq order: accumulator element j of row tile t is neuron
+// 32 t + 8 (j / 4) + 4 g + (j % 4), stored at bq[32 t + 16 (j / 8) + 8 g + (j % 8)]),
+// so no add is needed afterwards; bias == nullptr starts from zero.
 template <int K, int LDW, int NT>
-__device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const bf16x8* B, f32x16* acc, int l) {
+__device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const bf16x8* B, f32x16* acc, int l,
+                                          const float* __restrict__ bias) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+    for (int t = 0; t < NT; ++t) {
+        if (bias) {
+            const float* bt = bias + 32 * t + 8 * (l >> 5);
+            const float4 p0 = *reinterpret_cast<const float4*>(bt);
+            const float4 p1 = *reinterpret_cast<const float4*>(bt + 4);
+            const float4 p2 = *reinterpret_cast<const float4*>(bt + 16);
+            const float4 p3 = *reinterpret_cast<const float4*>(bt + 20);
+            acc[t] = f32x16{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w,
+                            p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+        } else {
+            acc[t] = f32x16{};
+        }
+    }
 #pragma unroll
     for (int s = 0; s < K / 16; ++s) {
 #pragma unroll
@@ -298,40 +328,63 @@ __device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const b
     }
 }
 
-__device__ __forceinline__ void mlp_activate(const f32x16* acc, const float* __restrict__ bq, bf16x8* B, int l) {
-    const int g = l >> 5;
+// GELU of a layer's accumulators (biases already in them) into the next
+// layer's B fragments: activation (s, e) is accumulator (s / 2, 8 (s % 2) + e).
+__device__ __forceinline__ void mlp_activate(const f32x16* acc, bf16x8* B) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const float4 b0 = *reinterpret_cast<const float4*>(bq + 16 * s + 8 * g);
-        const float4 b1 = *reinterpret_cast<const float4*>(bq + 16 * s + 8 * g + 4);
-        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
         bf16x8 v;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (__bf16)gelu_f(acc[s >> 1][8 * (s & 1) + e] + bb[e]);
+        for (int e = 0; e < 8; ++e) v[e] = (__bf16)gelu_f(acc[s >> 1][8 * (s & 1) + e]);
         B[s] = v;
     }
 }
 
-// Features of one lane's half (k = 8 g .. 8 g + 7 and 16 + 8 g .. 16 + 8 g + 7).
-__device__ __forceinline__ void feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
-                                              int periodic, int g, bf16x8* B) {
-    float f[MLP_IN];
-    if (alive) {
-        particle_features(p, rho, G, periodic, f);
-    } else {
+// Inference features of one lane's half only: lane half g needs k = 16 s + 8 g + e
+// (s = 0, 1; e = 0..7), so g = 0 computes densities 0-7 and 16-23, g = 1 densities
+// 8-15 and 24-26 plus the three offsets (27-29; 30, 31 are zero padding). Same
+// values as particle_features: neighbour node q = 9 dz + 3 dy + dx around the
+// particle's cell, log(1 + count), count = rho / 2^24.
+__device__ __forceinline__ float log1p_count(const long long* __restrict__ rho, int idx) {
+    const long long r = rho[idx];
+    const float c = (float)r * (1.0f / (float)SD_FIX_ONE);
+    return __logf(1.0f + c);
+}
+
+__device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
+                                                    int periodic, int g, bf16x8* B) {
+    const float fG = (float)G;
+    const int cx = min(max((int)floorf(p.x * fG), 0), G - 1);
+    const int cy = min(max((int)floorf(p.y * fG), 0), G - 1);
+    const int cz = min(max((int)floorf(p.z * fG), 0), G - 1);
+    int xs[3], ys[3], zs[3];
 #pragma unroll
-        for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
+    for (int d = 0; d < 3; ++d) {
+        xs[d] = wrap_or_clamp(cx + d - 1, G, periodic);
+        ys[d] = wrap_or_clamp(cy + d - 1, G, periodic) * G;
+        zs[d] = wrap_or_clamp(cz + d - 1, G, periodic) * G * G;
     }
-    // static indices only (a lane-dependent index would put f in scratch)
+    const float off0 = 2.0f * (p.x * fG - 0.5f - (float)cx);
+    const float off1 = 2.0f * (p.y * fG - 0.5f - (float)cy);
+    const float off2 = 2.0f * (p.z * fG - 0.5f - (float)cz);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         bf16x8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int k0 = 16 * s + e, k1 = 16 * s + 8 + e;
-            const float f0 = k0 < MLP_IN ? f[k0] : 0.f;
-            const float f1 = k1 < MLP_IN ? f[k1] : 0.f;
-            v[e] = (__bf16)(g ? f1 : f0);
+            const int q0 = 16 * s + e, q1 = q0 + 8;  // feature of half 0 / half 1 (compile time)
+            float val = 0.f;
+            if (q1 < 27) {
+                const int idx = g ? zs[q1 / 9] + ys[(q1 / 3) % 3] + xs[q1 % 3]
+                                  : zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3];
+                val = alive ? log1p_count(rho, idx) : 0.f;
+            } else {
+                const float o = q1 == 27 ? off0 : (q1 == 28 ? off1 : (q1 == 29 ? off2 : 0.f));
+                float d0 = 0.f;
+                if (alive && !g) d0 = log1p_count(rho, zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3]);
+                val = g ? (alive ? o : 0.f) : d0;
+            }
+            v[e] = (__bf16)val;
         }
         B[s] = v;
     }
@@ -371,14 +424,14 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         if (i < a.n) p = a.pos[i];
         bf16x8 B[8];
         f32x16 acc[4];
-        feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
-        mlp_layer<MLP_INP, W1_LD, 4>(sW1, B, acc, l);
-        mlp_activate(acc, sB, B, l);
-        mlp_layer<MLP_H, WH_LD, 4>(sW2, B, acc, l);
-        mlp_activate(acc, sB + MLP_H, B, l);
-        mlp_layer<MLP_H, WH_LD, 4>(sW3, B, acc, l);
-        mlp_activate(acc, sB + 2 * MLP_H, B, l);
-        mlp_layer<MLP_H, WH_LD, 1>(sW4, B, acc, l);
+        infer_feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
+        mlp_layer<MLP_INP, W1_LD, 4>(sW1, B, acc, l, sB);
+        mlp_activate(acc, B);
+        mlp_layer<MLP_H, WH_LD, 4>(sW2, B, acc, l, sB + MLP_H);
+        mlp_activate(acc, B);
+        mlp_layer<MLP_H, WH_LD, 4>(sW3, B, acc, l, sB + 2 * MLP_H);
+        mlp_activate(acc, B);
+        mlp_layer<MLP_H, WH_LD, 1>(sW4, B, acc, l, nullptr);
         // outputs 0..2 are accumulator rows 0..2 (elements 0..2 on lanes 0..31)
         if (g == 0 && i < a.n) {
             const float* b4 = sB + 3 * MLP_H;
@@ -509,14 +562,14 @@ void mlp_batch_grad(MlpDev& d, const float* X, const float* T, int n, hipStream_
 void mlp_pack(MlpDev& d, hipStream_t st) {
     hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_H * MLP_H + 255) / 256), dim3(256), 0, st, d.W, d);
     // padded output bias b4p
-    hipMemcpyAsync(d.b4p, d.b + 3 * MLP_H, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(d.b4p, d.b + 3 * MLP_H, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
 }
 
 void mlp_adam(MlpDev& d, float lr, int t, hipStream_t st) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     // the output-layer bias gradient lives in gb4p (padded); copy its 3 values
-    hipMemcpyAsync(d.gb + 3 * MLP_H, d.gb4p, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(d.gb + 3 * MLP_H, d.gb4p, sizeof(float) * MLP_OUT, hipMemcpyDeviceToDevice, st);
     hipLaunchKernelGGL(k_adam, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, d.W, d.gW, d.mW, d.vW, MLP_NW, lr, b1,
                        b2, eps, c1, c2);
     hipLaunchKernelGGL(k_adam, dim3(1), dim3(MLP_NB), 0, st, d.b, d.gb, d.mb, d.vb, MLP_NB, lr, b1, b2, eps, c1, c2);

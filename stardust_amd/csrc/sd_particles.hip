@@ -102,9 +102,38 @@ __device__ __forceinline__ void fixed_weights(const Cic& c, uint32_t w[8]) {
 // global grid directly. Same-address global atomics from consecutive waves
 // were the bottleneck of a per-particle deposit; a block of KP particles
 // issues one per touched cell instead.
-constexpr int DEP_CAP = 2048;  // LDS tile cells (16 KiB)
-constexpr int KP = 4;          // particles per thread in integrate / deposit
-constexpr int BP = PT * KP;    // particles per block
+// Tuning (compile-time): LDS tile size, particles per thread, and whether the
+// streamed particle arrays bypass the caches (they are touched once per step).
+#ifndef SD_DEP_CAP
+#define SD_DEP_CAP 2048
+#endif
+#ifndef SD_KP
+#define SD_KP 4
+#endif
+#ifndef SD_STREAM_PV
+#define SD_STREAM_PV 0
+#endif
+constexpr int DEP_CAP = SD_DEP_CAP;  // LDS tile cells (8 B each)
+constexpr int KP = SD_KP;            // particles per thread in integrate / deposit
+constexpr int BP = PT * KP;          // particles per block
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_pv(const float4* p) {
+#if SD_STREAM_PV
+    const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void store_pv(float4* p, float4 v) {
+#if SD_STREAM_PV
+    const f32x4_t w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f32x4_t*>(p));
+#else
+    *p = v;
+#endif
+}
 
 // DPP lane shift of a 32-bit value (lanes without a source read 0).
 template <int CTRL, int ROWMASK>
@@ -360,8 +389,8 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
     for (int k = 0; k < KP; ++k) {
         const int i = blk * BP + k * PT + threadIdx.x;
         act[k] = i < a.n;
-        P[k] = act[k] ? a.pos[i] : make_float4(-1.f, -1.f, -1.f, 0.f);
-        V[k] = act[k] ? a.vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        P[k] = act[k] ? load_pv(a.pos + i) : make_float4(-1.f, -1.f, -1.f, 0.f);
+        V[k] = act[k] ? load_pv(a.vel + i) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     // phase 2: the CIC gathers (8 x KP loads, no stores in between)
     const float fG = (float)a.G;
@@ -477,8 +506,8 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
                 if (p.z < 0.002f) { p.z = 0.002f; v.z = fabsf(v.z) * 0.25f; }
                 if (p.z > 0.998f) { p.z = 0.998f; v.z = -fabsf(v.z) * 0.25f; }
             }
-            a.pos[i] = p;
-            a.vel[i] = v;
+            store_pv(a.pos + i, p);
+            store_pv(a.vel + i, v);
             if (alive) {
                 depAlive[k] = true;
                 depC[k] = cic_setup(p, fG);

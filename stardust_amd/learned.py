@@ -1,9 +1,10 @@
 """Learned sub-cell correction: training data, training and evaluation.
 
 Pipeline (all heavy work on the GPU through the C ABI):
-  1. scenes: exponential disks (and two-disk encounters) of varying particle
-     count and box size, evolved a few PM steps; snapshots are taken at the
-     mesh being corrected (G, softening 1 cell);
+  1. scenes: the batched ensemble (64 disks of 2^15..2^20 particles advanced in
+     one graph launch, snapshots every 20 steps) plus exponential disks and
+     two-disk encounters of 2^18..2^22 particles evolved a few PM steps; each
+     snapshot is re-meshed at the mesh being corrected (G, softening 1 cell);
   2. pairs: for a subsample of particles per snapshot, the 30 features
      (sd_features), the PM acceleration (sd_particle_accel) and the Plummer
      direct sum over all particles (sd_direct_sum, fixed softening eps_box);
@@ -98,6 +99,24 @@ def training_pairs(G=128, scenes=None, n_targets=65536, eps_box=EPS_BOX_DEFAULT,
     return np.concatenate(Xs), np.concatenate(Ts)
 
 
+def training_pairs_batch(G=128, targets_per_snapshot=2048, n_scenes=64, steps=200, every=20, seed=3,
+                         log=print):
+    """Pairs from the batched ensemble: 64 config-0 disks of 2^15..2^20
+    particles on meshes {64, 96, 128}, advanced ``steps`` steps in one graph
+    launch with position snapshots every ``every`` steps; each snapshot is
+    re-meshed at G (softening 1 cell) for features, PM force and direct sum."""
+    from . import batch as BT
+    scenes = ic.config3(n_scenes, seed=seed)
+    b, data = BT.make_batch(scenes, 10, log)
+    b.run(steps, every)
+    b.synchronize()
+    X, T = BT.pairs_from_snapshots(b, data, G_train=G, n_targets=targets_per_snapshot, log=None)
+    b.close()
+    log(f"[learned] {X.shape[0]} pairs from {len(scenes)} batched scenes x {steps // every} snapshots, "
+        f"residual rms {np.sqrt((T.astype(np.float64) ** 2).mean()):.2f}")
+    return X, T
+
+
 def train(X, T, epochs=30, batch=8192, lr=1e-3, seed=1, log=print):
     out_scale = float(np.sqrt((T.astype(np.float64) ** 2).mean()))
     m = Mlp(seed=seed)
@@ -132,6 +151,49 @@ def evaluate(pos, vel, M, L, G_coarse, G_fine, mlp=None, out_scale=1.0, eps_box=
     return res
 
 
+def evaluate_staleness(pos, vel, M, L, G_coarse, G_fine, mlp, out_scale, lags=9, dt=1e-3,
+                       eps_box=EPS_BOX_DEFAULT, n_targets=16384, seed=12345):
+    """Force error of a correction computed ``lag`` steps earlier. The
+    corrected coarse run keeps its first correction (no re-sort, so particle
+    order is fixed); at each lag the error of PM + that stale correction is
+    compared with PM + a fresh correction, plain coarse and plain fine PM, all
+    against the direct sum at the current positions."""
+    n = pos.shape[0]
+    grav_c = ic.gravity_isolated(G_coarse, M, L)
+    grav_f = ic.gravity_isolated(G_fine, M, L)
+    run = _sim_for(pos, vel, G_coarse, grav_c, sort_interval=0)
+    run.set_fixed_dt(dt)
+    run.attach_mlp(mlp, out_scale)
+    run.set_mlp_interval(1 << 30)
+    stale = run.mlp_correction()
+    probe_c = _sim_for(pos, vel, G_coarse, grav_c, sort_interval=0)
+    probe_c.attach_mlp(mlp, out_scale)
+    probe_f = _sim_for(pos, vel, G_fine, grav_f, sort_interval=0)
+    targets = FE.pick_targets(pos, n_targets, seed)
+    gm = FE.direct_gm(grav_c, G_coarse, n)
+    out = []
+    for lag in range(lags + 1):
+        if lag:
+            run.run(1)
+        p, v = run.get_particles()
+        probe_c.set_particles(p, v)
+        probe_f.set_particles(p, v)
+        a_dir = probe_c.direct_sum(targets, gm, eps_box)
+        a_c = probe_c.particle_accel()[targets, :3].astype(np.float64)
+        fresh = probe_c.mlp_correction()[targets, :3]
+        a_f = probe_f.particle_accel()[targets, :3].astype(np.float64)
+        out.append({"lag": lag,
+                    "coarse": FE.rel_rms(a_c, a_dir)[0],
+                    "coarse+mlp_fresh": FE.rel_rms(a_c + fresh, a_dir)[0],
+                    "coarse+mlp_stale": FE.rel_rms(a_c + stale[targets, :3], a_dir)[0],
+                    "fine": FE.rel_rms(a_f, a_dir)[0]})
+    probe_c.attach_mlp(None)
+    run.attach_mlp(None)
+    for s in (run, probe_c, probe_f):
+        s.close()
+    return out
+
+
 def time_steps(pos, vel, M, L, G, mlp=None, out_scale=1.0, steps=20, warmup=5, dt=1e-3, interval=1):
     gravity = ic.gravity_isolated(G, M, L)
     s = _sim_for(pos, vel, G, gravity)
@@ -160,9 +222,20 @@ def main(argv=None):
     ap.add_argument("--targets", type=int, default=65536)
     ap.add_argument("--out", default=None, help="write the JSON report here")
     ap.add_argument("--save-model", default=None, help="write parameters (.npz)")
+    ap.add_argument("--pairs", choices=("batch", "scenes", "both"), default="both",
+                    help="training pairs from the batched ensemble's snapshots (64 disks of 2^15..2^20 "
+                         "particles), from 8 larger scenes (2^18..2^22 particles, disks and pairs), or both; "
+                         "the ensemble alone does not reach the density range of the 4M-particle evaluation")
+    ap.add_argument("--lags", type=int, default=9, help="staleness report up to this many steps")
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    X, T = training_pairs(G=args.coarse, n_targets=args.targets, log=log)
+    parts = []
+    if args.pairs in ("batch", "both"):
+        parts.append(training_pairs_batch(G=args.coarse, log=log))
+    if args.pairs in ("scenes", "both"):
+        parts.append(training_pairs(G=args.coarse, n_targets=args.targets, log=log))
+    X = np.concatenate([x for x, _ in parts])
+    T = np.concatenate([t for _, t in parts])
     mlp, out_scale, losses = train(X, T, epochs=args.epochs, log=log)
     p4, v4, M, L = scene_pair(args.eval_each, seed=1)
     err = evaluate(p4, v4, M, L, args.coarse, args.fine, mlp, out_scale)
@@ -171,6 +244,12 @@ def main(argv=None):
     corr_c, ms_c = time_steps(p4, v4, M, L, args.coarse, mlp, out_scale)
     corr_c5, ms_c5 = time_steps(p4, v4, M, L, args.coarse, mlp, out_scale, interval=5)
     plain_c, ms_pc = time_steps(p4, v4, M, L, args.coarse)
+    by_interval = {}
+    for k in (2, 3, 5, 10):
+        v, ms = time_steps(p4, v4, M, L, args.coarse, mlp, out_scale, interval=k)
+        by_interval[str(k)] = {"particle_steps_per_s": v, "ms_per_step": ms, "speedup_vs_plain_fine": v / plain_f}
+    stale = evaluate_staleness(p4, v4, M, L, args.coarse, args.fine, mlp, out_scale, lags=args.lags)
+    log("[learned] staleness", json.dumps(stale))
     rep = {"errors": err, "train_pairs": int(X.shape[0]), "epochs": args.epochs,
            "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "out_scale": out_scale,
            "eps_box": EPS_BOX_DEFAULT, "eval_particles": int(p4.shape[0]),
@@ -180,7 +259,8 @@ def main(argv=None):
                            "plain_coarse": ms_pc},
            "speedup_coarse_mlp_every5_vs_plain_fine": corr_c5 / plain_f,
            "speedup_coarse_mlp_vs_plain_fine": corr_c / plain_f,
-           "coarse_mlp_error_vs_plain_fine": err["coarse+mlp"]["rel_rms"] / err["fine"]["rel_rms"]}
+           "coarse_mlp_error_vs_plain_fine": err["coarse+mlp"]["rel_rms"] / err["fine"]["rel_rms"],
+           "pairs_source": args.pairs, "refresh_interval": by_interval, "staleness": stale}
     print(json.dumps(rep), flush=True)
     if args.out:
         with open(args.out, "w") as f:

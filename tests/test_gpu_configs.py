@@ -1,0 +1,117 @@
+"""The benchmark configurations as parity cases against the oracle.
+
+* config 0 at its full size (262,144-particle disk, G=64 / S=128 isolated,
+  dt 1e-3, 10 steps): the reference-order pipeline against the oracle's
+  Sim::step, and the product pipeline (fused, leapfrog, Morton keys) against
+  the oracle's kick-drift started half a kick back (the exact relation of the
+  two integrators);
+* config 2 at reduced size (Zel'dovich lattice 32^3 on a periodic G=64 mesh,
+  contact radius 0.2 cells, re-sort every step, CFL dt).
+
+Tolerances: positions 5e-6 box units, velocities 2e-4 of the largest speed
+(FP32 FFT on the GPU against the oracle's FP64 FFT).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import stardust_amd as sd
+from stardust_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _config0():
+    c = ic.config0()
+    pos = c["pos"].copy()
+    pos[:, 3] = np.arange(1, pos.shape[0] + 1, dtype=np.float32)  # identity in w (unused by the step)
+    return c, pos, c["vel"]
+
+
+def test_config0_reference_order_vs_oracle(gpu, oracle):
+    c, pos, vel = _config0()
+    n, G, steps = pos.shape[0], c["G"], c["steps"]
+    _say("config0 reference order", n, G, steps)
+    cfg = sd.default_config(particleCount=n, gridSize=G, gravity=c["gravity"], softeningCells=c["soft"],
+                            sortInterval=40)
+    s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
+                                     keyMode=sd.KEY_ROWMAJOR))
+    s.set_fixed_dt(c["dt"])
+    s.set_particles(pos, c["vel"])
+    s.run(steps)
+    gp, gv = s.get_particles()
+    t = s.sim_time()
+    s.close()
+    prm = oracle.make_params(G, gravity=c["gravity"], softeningCells=c["soft"], sortInterval=40,
+                             keyMode=oracle.KEY_ROWMAJOR, fixedDt=c["dt"], allocN=n)
+    st = oracle.Stepper(prm, pos, vel)
+    st.step(steps)
+    assert np.array_equal(gp[:, 3], st.pos[:, 3])  # same permutation
+    dp = np.abs(gp[:, :3] - st.pos[:, :3]).max()
+    dv = np.abs(gv[:, :3] - st.vel[:, :3]).max() / np.abs(st.vel[:, :3]).max()
+    _say("  max |dx|", dp, "max rel dv", dv)
+    assert dp < 5e-6
+    assert dv < 2e-4
+    assert abs(t - st.state.simTime) < 1e-9 + 1e-6 * st.state.simTime
+
+
+def test_config0_product_path_vs_oracle(gpu, oracle):
+    c, pos, vel = _config0()
+    n, G, steps, dt = pos.shape[0], c["G"], c["steps"], c["dt"]
+    _say("config0 product path", n, G, steps)
+    cfg = sd.default_config(particleCount=n, gridSize=G, gravity=c["gravity"], softeningCells=c["soft"],
+                            sortInterval=40)
+    s = sd.Sim(cfg, sd.default_modes())  # fused, leapfrog, Morton keys
+    s.set_fixed_dt(dt)
+    s.set_particles(pos, vel)
+    s.run(steps)
+    gp, gv = s.get_particles()
+    s.close()
+    # leapfrog from v0 = kick-drift from v0 - a(x0) dt / 2 (positions); a(x0) from the oracle
+    prm = oracle.make_params(G, gravity=c["gravity"], softeningCells=c["soft"], sortInterval=40,
+                             keyMode=oracle.KEY_MORTON, fixedDt=dt, allocN=n)
+    a0 = oracle.particle_accel(prm, pos)
+    v0 = vel.copy()
+    v0[:, :3] -= 0.5 * dt * a0[:, :3]
+    st = oracle.Stepper(prm, pos, v0)
+    st.step(steps)
+    og = np.argsort(gp[:, 3])
+    oo = np.argsort(st.pos[:, 3])
+    dp = np.abs(gp[og, :3] - st.pos[oo, :3]).max()
+    _say("  max |dx|", dp)
+    assert np.array_equal(gp[og, 3], st.pos[oo, 3])
+    assert dp < 5e-6
+
+
+def test_config2_small_periodic_contact_vs_oracle(gpu, oracle):
+    pos, vel = ic.zeldovich(32, seed=1, rms_cells=0.3, G_mesh=64)
+    n, G, steps = pos.shape[0], 64, 4
+    grav = ic.gravity_periodic(G, 1.0, 1.0) * 1e-9  # weak gravity: contact and mesh both matter
+    _say("config2 small", n, G, steps)
+    cfg = sd.default_config(particleCount=n, gridSize=G, boundary=sd.BOUNDARY_PERIODIC, gravity=grav,
+                            softeningCells=3.0, sortInterval=1, contactEnabled=1, contactRadiusCells=0.2,
+                            contactStiffness=1e3)
+    vel = vel.copy()
+    vel[:, :3] = np.random.default_rng(2).normal(0, 0.002, (n, 3)).astype(np.float32)
+    s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
+                                     keyMode=sd.KEY_ROWMAJOR))
+    s.set_particles(pos, vel)
+    s.run(steps)
+    gp, gv = s.get_particles()
+    s.close()
+    prm = oracle.make_params(G, periodic=True, gravity=grav, softeningCells=3.0, sortInterval=1,
+                             keyMode=oracle.KEY_ROWMAJOR, contact=1, contactRadiusCells=0.2,
+                             contactStiffness=1e3, allocN=n)
+    st = oracle.Stepper(prm, pos, vel)
+    st.step(steps)
+    dp = np.abs(gp[:, :3] - st.pos[:, :3]).max()
+    dv = np.abs(gv[:, :3] - st.vel[:, :3]).max() / np.abs(st.vel[:, :3]).max()
+    _say("  max |dx|", dp, "max rel dv", dv)
+    assert dp < 5e-6
+    assert dv < 2e-4
