@@ -125,7 +125,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="integrate", help="kernel timed for the roofline")
     ap.add_argument("--workload", choices=("two_disk", "cosmic_web"), default="two_disk",

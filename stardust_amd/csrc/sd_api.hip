@@ -316,6 +316,23 @@ static int key_bits(const sd_sim* s) {
     return bits;
 }
 
+#ifndef SD_FUSE_REORDER
+#define SD_FUSE_REORDER 1
+#endif
+
+// Keys and the stable radix sort only: the permutation is left in s->sb.idx
+// for an integrate launch that reads through it (the fused reorder).
+static void sort_keys(sd_sim* s) {
+    const int n = s->allocN;
+    const int kmode = s->modes.keyMode == SD_KEY_MORTON ? 2 : 0;
+    TIMED(s, "sort", {
+        launch_cell_keys(s->pos, n, s->G, kmode, s->sb.keys, s->sb.idx, s->st);
+        radix_sort_pairs(s->sb, n, key_bits(s), s->st);
+    });
+    s->keysSorted = true;
+    s->corrValid = false;
+}
+
 // Impl::sortParticles, Sim.cu:3656-3671. forContact selects the contact
 // cell key (Morton of the cell) instead of the stencil-node key.
 static void sort_particles(sd_sim* s, bool forContact = false) {
@@ -499,7 +516,20 @@ static void enqueue_step(sd_sim* s) {
     HK(hipEventRecord(s->evA, s->st));
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
                          !s->cfg.contactEnabled;
-    if (sortDue) sort_particles(s);
+    // With nothing between the sort and the integrate that reads particles in
+    // sorted order (learned correction, contact), the integrate launch does the
+    // reorder itself: it reads through the permutation and writes the sorted
+    // arrays, which saves the separate gather pass.
+    // (Only when the density of the current positions is already deposited: a
+    // first deposit from unsorted positions would contend on the same cells.)
+    const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
+                             s->modes.depositMode == SD_DEPOSIT_FIXED;
+    if (sortDue) {
+        if (fuseReorder)
+            sort_keys(s);
+        else
+            sort_particles(s);
+    }
     bool corr = false;
     if (s->mlp) {
         if (!s->corrValid || s->corrAge >= s->mlpInterval) {
@@ -539,7 +569,18 @@ static void enqueue_step(sd_sim* s) {
         const bool fusedDeposit = s->modes.depositMode == SD_DEPOSIT_FIXED;
         a.depositTo = fusedDeposit ? s->rhoFix : nullptr;
         a.partials = s->partials;
+        if (fuseReorder) {
+            a.order = s->sb.idx;
+            a.srcPos = s->pos;
+            a.srcVel = s->vel;
+            a.pos = s->posTmp;
+            a.vel = s->velTmp;
+        }
         TIMED(s, "integrate", launch_integrate(a, s->st));
+        if (fuseReorder) {
+            std::swap(s->pos, s->posTmp);
+            std::swap(s->vel, s->velTmp);
+        }
         const int mode = (s->modes.momentumZero ? 1 : 0) | 2 | 8 | 4;
         TIMED(s, "finalize", launch_finalize(s->partials, s->npartsInt, mode, pol, s->sc, s->st));
         s->primed = true;

@@ -139,9 +139,29 @@ struct Line {
     static constexpr int LP = N + N / 16 + 1;  // padded line pitch (float2)
 };
 
+// Ordering of LDS accesses between the threads of one line. A line's T threads
+// are threadIdx / T == line, so when T divides 64 they are lanes of one
+// wavefront, whose LDS operations complete in issue order: a wavefront-scope
+// fence (a compiler barrier, no wait) is enough. Otherwise the block barrier.
+#ifndef SD_FFT_LINE_SYNC
+#define SD_FFT_LINE_SYNC 1
+#endif
+template <int T>
+__device__ __forceinline__ void line_sync() {
+    if constexpr (SD_FFT_LINE_SYNC && 64 % T == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // ---- one Stockham stage over an LDS line ---------------------------------
 // tw[m] = exp(-2 pi i m / N) (FP64-computed table); INV uses the conjugate.
-template <int N, int R, int NS, int T, bool INV>
+// The last stage of a transform ends with a block barrier: what follows reads
+// the lines with another thread mapping.
+template <int N, int R, int NS, int T, bool INV, bool LAST = false>
 __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* __restrict__ tw,
                                       int t) {
     constexpr int NBF = N / R;
@@ -155,7 +175,7 @@ __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* _
             for (int r = 0; r < R; ++r) v[q][r] = line[PADI(j + r * NBF)];
         }
     }
-    __syncthreads();
+    line_sync<T>();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int j = t + q * T;
@@ -175,7 +195,10 @@ __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* _
             for (int r = 0; r < R; ++r) line[PADI(base + r * NS)] = v[q][r];
         }
     }
-    __syncthreads();
+    if (LAST)
+        __syncthreads();
+    else
+        line_sync<T>();
 }
 
 // Radix plans: T threads per line; every stage is a compile-time (R, Ns).
@@ -187,7 +210,7 @@ struct Plan<16> {
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<16, 4, 1, T, INV>(l, tw, t);
-        stage<16, 4, 4, T, INV>(l, tw, t);
+        stage<16, 4, 4, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -196,7 +219,7 @@ struct Plan<32> {
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<32, 4, 1, T, INV>(l, tw, t);
-        stage<32, 8, 4, T, INV>(l, tw, t);
+        stage<32, 8, 4, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -205,7 +228,7 @@ struct Plan<64> {
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<64, 8, 1, T, INV>(l, tw, t);
-        stage<64, 8, 8, T, INV>(l, tw, t);
+        stage<64, 8, 8, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -214,7 +237,7 @@ struct Plan<128> {
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<128, 8, 1, T, INV>(l, tw, t);
-        stage<128, 16, 8, T, INV>(l, tw, t);
+        stage<128, 16, 8, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -224,7 +247,7 @@ struct Plan<192> {
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<192, 3, 1, T, INV>(l, tw, t);
         stage<192, 8, 3, T, INV>(l, tw, t);
-        stage<192, 8, 24, T, INV>(l, tw, t);
+        stage<192, 8, 24, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -233,7 +256,7 @@ struct Plan<256> {
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<256, 16, 1, T, INV>(l, tw, t);
-        stage<256, 16, 16, T, INV>(l, tw, t);
+        stage<256, 16, 16, T, INV, true>(l, tw, t);
     }
 };
 template <>
@@ -243,7 +266,7 @@ struct Plan<512> {
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<512, 8, 1, T, INV>(l, tw, t);
         stage<512, 8, 8, T, INV>(l, tw, t);
-        stage<512, 8, 64, T, INV>(l, tw, t);
+        stage<512, 8, 64, T, INV, true>(l, tw, t);
     }
 };
 
@@ -575,6 +598,136 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float
     cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y);
 }
 
+// Persistent form of the forward / inverse / fused-z column passes (same
+// arithmetic as cols_body, tile by tile): a block walks tiles
+// (bx, slab) = tile % colBlocks, tile / colBlocks with stride gridDim.x, and
+// issues the next tile's global loads right after staging the current one, so
+// they are in flight during the transforms and stores of the current tile.
+// The fused-z multipliers of a tile are read before that prefetch (vector
+// memory counters retire in order).
+template <int N, int MODE>
+__global__ void __launch_bounds__(TB * Plan<N>::T) k_cols_stream(ColArgs a, const float2* __restrict__ tw,
+                                                                 int colBlocks, int ntiles) {
+    static_assert(MODE == COL_FWD || MODE == COL_INV || MODE == COL_Z_ISO || MODE == COL_Z_PER, "mode");
+    constexpr int LP = Line<N>::LP;
+    constexpr int THREADS = TB * Plan<N>::T;
+    constexpr int PAIRS = TB / 2;
+    constexpr int RPI = THREADS / PAIRS;
+    constexpr int ITERS = (N + RPI - 1) / RPI;
+    __shared__ float2 s_line[TB * LP];
+    __shared__ float2 s_tw[N];
+    load_twiddles<N>(s_tw, tw);
+    const int pc = threadIdx.x % PAIRS, r0 = threadIdx.x / PAIRS;
+    const int col = 2 * pc;
+    const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
+    float2* line = s_line + c * LP;
+    float4 buf[ITERS];
+    auto fetch = [&](int tile) {
+        const int bx = tile % colBlocks, slab = tile / colBlocks;
+        const int b0 = bx * TB;
+        const bool ok0 = b0 + col < a.nb, ok1 = b0 + col + 1 < a.nb;
+        const float2* in = a.in + (size_t)slab * a.inA + b0 + col;
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int n = r0 + it * RPI;
+            buf[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (n < a.nin && ok0) {
+                if (ok1) {
+                    buf[it] = *reinterpret_cast<const float4*>(in + (size_t)n * a.inN);
+                } else {
+                    const float2 x = in[(size_t)n * a.inN];
+                    buf[it] = make_float4(x.x, x.y, 0.f, 0.f);
+                }
+            }
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int bx = tile % colBlocks, slab = tile / colBlocks;
+        const int b0 = bx * TB;
+        const bool ok0 = b0 + col < a.nb, ok1 = b0 + col + 1 < a.nb;
+        __syncthreads();  // the previous tile's lines have been read out
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int n = r0 + it * RPI;
+            if (n < N) {
+                s_line[col * LP + PADI(n)] = make_float2(buf[it].x, buf[it].y);
+                s_line[(col + 1) * LP + PADI(n)] = make_float2(buf[it].z, buf[it].w);
+            }
+        }
+        __syncthreads();
+        // fused-z multipliers of this tile (before the prefetch)
+        float m0[ITERS], m1[ITERS];
+        if (MODE == COL_Z_ISO || MODE == COL_Z_PER) {
+            const int h = N / 2;
+            const int ky = slab;
+            const int kyf = ky <= h ? ky : N - ky;
+            const int kx0 = b0 + col;
+#pragma unroll
+            for (int it = 0; it < ITERS; ++it) {
+                const int kz = r0 + it * RPI;
+                m0[it] = m1[it] = 0.f;
+                if (kz >= N) continue;
+                if (MODE == COL_Z_ISO) {
+                    const int kzf = kz <= h ? kz : N - kz;
+                    const float2 g = *reinterpret_cast<const float2*>(a.green + ((size_t)kzf * (h + 1) + kyf) * a.P + kx0);
+                    m0[it] = g.x;
+                    m1[it] = g.y;
+                } else {
+                    // kPoissonPeriodic, Sim.cu:528-557
+                    const int sky = ky <= h ? ky : ky - N;
+                    const int skz = kz <= h ? kz : kz - N;
+                    float mm[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int kx = kx0 + e;
+                        const float k2 = (float)(kx * kx + sky * sky + skz * skz);
+                        if (k2 < 1e-6f) {
+                            mm[e] = 0.f;
+                        } else {
+                            mm[e] = a.perScale / k2;
+                            if (a.perSoft > 0.f) mm[e] *= expf(-0.5f * k2 * a.perSoft);
+                        }
+                    }
+                    m0[it] = mm[0];
+                    m1[it] = mm[1];
+                }
+            }
+        }
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (MODE == COL_INV) {
+            Plan<N>::template run<true>(line, s_tw, t);
+        } else {
+            Plan<N>::template run<false>(line, s_tw, t);
+        }
+        if (MODE == COL_Z_ISO || MODE == COL_Z_PER) {
+#pragma unroll
+            for (int it = 0; it < ITERS; ++it) {
+                const int kz = r0 + it * RPI;
+                if (kz >= N) continue;
+                const float2 u = s_line[col * LP + PADI(kz)], v = s_line[(col + 1) * LP + PADI(kz)];
+                s_line[col * LP + PADI(kz)] = make_float2(u.x * m0[it], u.y * m0[it]);
+                s_line[(col + 1) * LP + PADI(kz)] = make_float2(v.x * m1[it], v.y * m1[it]);
+            }
+            __syncthreads();
+            Plan<N>::template run<true>(line, s_tw, t);
+        }
+        float2* out = a.out + (size_t)slab * a.outA + b0 + col;
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int k = r0 + it * RPI;
+            if (k < a.nout && ok0) {
+                const float2 u = s_line[col * LP + PADI(k)], v = s_line[(col + 1) * LP + PADI(k)];
+                if (ok1)
+                    *reinterpret_cast<float4*>(out + (size_t)k * a.outN) = make_float4(u.x, u.y, v.x, v.y);
+                else
+                    out[(size_t)k * a.outN] = u;
+            }
+        }
+    }
+}
+
 // Column pass of many meshes of the same FFT side and mesh size: blockIdx.z
 // selects the mesh.
 template <int N, int MODE>
@@ -653,9 +806,46 @@ static void launch_rows_inv(const RowsInvArgs& a, const float2* tw, hipStream_t 
     dim3 grid((a.rows + TB - 1) / TB), block(TB * Plan<N>::T);
     hipLaunchKernelGGL((k_rows_inv<N>), grid, block, 0, st, a, tw);
 }
+// Resident blocks of a kernel on the whole device.
+template <typename K>
+static int resident_blocks(K kernel, int threads) {
+    int per = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    return per * cus;
+}
+
+// Transform size from which the persistent, prefetching column kernels are
+// used: measured faster at N = 512 (z pass -13%), slower at N = 256, where
+// the prefetch registers cost more occupancy than the overlap gains.
+#ifndef SD_COLS_STREAM_MIN_N
+#define SD_COLS_STREAM_MIN_N 512
+#endif
+template <int N, int MODE>
+static void launch_cols_stream(const ColArgs& a, int slabs, const float2* tw, hipStream_t st) {
+    const int threads = TB * Plan<N>::T;
+    const int colBlocks = (a.nb + TB - 1) / TB;
+    const int ntiles = colBlocks * slabs;
+    static const int resident = resident_blocks(k_cols_stream<N, MODE>, threads);  // per instantiation
+    int grid = resident;
+    if (grid > ntiles) grid = ntiles;
+    hipLaunchKernelGGL((k_cols_stream<N, MODE>), dim3(grid), dim3(threads), 0, st, a, tw, colBlocks, ntiles);
+}
+
 template <int N>
 static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw, hipStream_t st) {
     dim3 grid((a.nb + TB - 1) / TB, slabs), block(TB * Plan<N>::T);
+    if constexpr (N >= SD_COLS_STREAM_MIN_N) {
+        switch (mode) {
+            case COL_FWD: launch_cols_stream<N, COL_FWD>(a, slabs, tw, st); return;
+            case COL_INV: launch_cols_stream<N, COL_INV>(a, slabs, tw, st); return;
+            case COL_Z_ISO: launch_cols_stream<N, COL_Z_ISO>(a, slabs, tw, st); return;
+            case COL_Z_PER: launch_cols_stream<N, COL_Z_PER>(a, slabs, tw, st); return;
+            default: break;  // the Green build keeps the one-tile kernel
+        }
+    }
     switch (mode) {
         case COL_FWD: hipLaunchKernelGGL((k_cols<N, COL_FWD>), grid, block, 0, st, a, tw); break;
         case COL_INV: hipLaunchKernelGGL((k_cols<N, COL_INV>), grid, block, 0, st, a, tw); break;

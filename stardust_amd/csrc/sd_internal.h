@@ -105,6 +105,12 @@ struct IntegrateArgs {
     int applyMeanSub;     // subtract sc->meanSub on load
     long long* depositTo; // fused deposit of the new positions (nullptr = off)
     ReducePartial* partials;
+    // Fused reorder: when order != nullptr, particle i is read from
+    // srcPos/srcVel[order[i]] (a sort's permutation) and written to pos/vel[i],
+    // which must be other arrays; otherwise it is read from pos/vel[i].
+    const int32_t* order;
+    const float4* srcPos;
+    const float4* srcVel;
 };
 int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
 int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)

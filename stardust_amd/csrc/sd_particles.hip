@@ -24,7 +24,10 @@
 
 namespace sd {
 
-constexpr int PT = 256;  // threads per particle block
+#ifndef SD_PT
+#define SD_PT 256
+#endif
+constexpr int PT = SD_PT;  // threads per particle block
 
 // ---- fills -------------------------------------------------------------------
 __global__ void k_fill_i64(long long* p, long long n) {
@@ -105,10 +108,10 @@ __device__ __forceinline__ void fixed_weights(const Cic& c, uint32_t w[8]) {
 // Tuning (compile-time): LDS tile size, particles per thread, and whether the
 // streamed particle arrays bypass the caches (they are touched once per step).
 #ifndef SD_DEP_CAP
-#define SD_DEP_CAP 2048
+#define SD_DEP_CAP 1024
 #endif
 #ifndef SD_KP
-#define SD_KP 4
+#define SD_KP 2
 #endif
 #ifndef SD_STREAM_PV
 #define SD_STREAM_PV 0
@@ -389,8 +392,14 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
     for (int k = 0; k < KP; ++k) {
         const int i = blk * BP + k * PT + threadIdx.x;
         act[k] = i < a.n;
-        P[k] = act[k] ? load_pv(a.pos + i) : make_float4(-1.f, -1.f, -1.f, 0.f);
-        V[k] = act[k] ? load_pv(a.vel + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (a.order) {
+            const int j = act[k] ? a.order[i] : 0;
+            P[k] = act[k] ? load_pv(a.srcPos + j) : make_float4(-1.f, -1.f, -1.f, 0.f);
+            V[k] = act[k] ? load_pv(a.srcVel + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            P[k] = act[k] ? load_pv(a.pos + i) : make_float4(-1.f, -1.f, -1.f, 0.f);
+            V[k] = act[k] ? load_pv(a.vel + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
     // phase 2: the CIC gathers (8 x KP loads, no stores in between)
     const float fG = (float)a.G;
@@ -517,6 +526,11 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
                 cnt += 1.0;
                 vm = fmaxf(vm, speed3(v.x, v.y, v.z));
             }
+        } else if (a.order) {
+            // an inactive slot is left untouched in place, but a fused reorder
+            // writes another buffer: carry it over
+            store_pv(a.pos + i, p);
+            store_pv(a.vel + i, V[k]);
         }
     }
     if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
