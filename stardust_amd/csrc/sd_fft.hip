@@ -240,9 +240,12 @@ struct Plan<128> {
         stage<128, 16, 8, T, INV, true>(l, tw, t);
     }
 };
+#ifndef SD_PLAN192_T
+#define SD_PLAN192_T 24
+#endif
 template <>
 struct Plan<192> {
-    static constexpr int T = 24;
+    static constexpr int T = SD_PLAN192_T;
     template <bool INV>
     __device__ static void run(float2* l, const float2* tw, int t) {
         stage<192, 3, 1, T, INV>(l, tw, t);
@@ -270,7 +273,10 @@ struct Plan<512> {
     }
 };
 
-constexpr int TB = 16;  // lines per block (columns of a tile / rows of a row batch)
+#ifndef SD_TB
+#define SD_TB 16
+#endif
+constexpr int TB = SD_TB;  // lines per block (columns of a tile / rows of a row batch)
 
 template <int N>
 __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __restrict__ tw) {

@@ -297,26 +297,21 @@ constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (bf16); +8 spreads rows o
 constexpr int WH_LD = MLP_H + 8;
 constexpr int INF_WAVES = 12;
 
-// One layer, Y^T = W X^T: the accumulators start from the layer's biases
-// (bq order: accumulator element j of row tile t is neuron
-// 32 t + 8 (j / 4) + 4 g + (j % 4), stored at bq[32 t + 16 (j / 8) + 8 g + (j % 8)]),
-// so no add is needed afterwards; bias == nullptr starts from zero.
-template <int K, int LDW, int NT>
+// One layer, Y^T = W X^T: the accumulators start from the layer's biases, so
+// no add is needed afterwards (BIAS false starts from zero). The biases
+// are staged per (row tile t, lane half g) as 16 contiguous floats in
+// accumulator order: element j of acc[t] is neuron 32 t + 8 (j / 4) + 4 g + (j % 4),
+// i.e. bq[32 t + 16 (j / 8) + 8 g + (j % 8)], so one aligned 64-byte read fills
+// the accumulator registers.
+template <int K, int LDW, int NT, bool BIAS>
 __device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const bf16x8* B, f32x16* acc, int l,
                                           const float* __restrict__ bias) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        if (bias) {
-            const float* bt = bias + 32 * t + 8 * (l >> 5);
-            const float4 p0 = *reinterpret_cast<const float4*>(bt);
-            const float4 p1 = *reinterpret_cast<const float4*>(bt + 4);
-            const float4 p2 = *reinterpret_cast<const float4*>(bt + 16);
-            const float4 p3 = *reinterpret_cast<const float4*>(bt + 20);
-            acc[t] = f32x16{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w,
-                            p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
-        } else {
+        if constexpr (BIAS)
+            acc[t] = *reinterpret_cast<const f32x16*>(bias + (2 * t + (l >> 5)) * 16);
+        else
             acc[t] = f32x16{};
-        }
     }
 #pragma unroll
     for (int s = 0; s < K / 16; ++s) {
@@ -346,9 +341,12 @@ __device__ __forceinline__ void mlp_activate(const f32x16* acc, bf16x8* B) {
 // values as particle_features: neighbour node q = 9 dz + 3 dy + dx around the
 // particle's cell, log(1 + count), count = rho / 2^24.
 __device__ __forceinline__ float log1p_count(const long long* __restrict__ rho, int idx) {
+    // count = rho / 2^24 from the two 32-bit halves (a generic int64 -> float
+    // conversion costs several more instructions); within an ulp of (float)rho / 2^24
     const long long r = rho[idx];
-    const float c = (float)r * (1.0f / (float)SD_FIX_ONE);
-    return __logf(1.0f + c);
+    const float c = fmaf((float)(int)(r >> 32), 256.0f, (float)(unsigned int)r * (1.0f / (float)SD_FIX_ONE));
+    // log2 by the hardware instruction (its argument is >= 1: no denormal path), times ln 2
+    return __builtin_amdgcn_logf(1.0f + c) * 0.69314718055994531f;
 }
 
 __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
@@ -410,8 +408,15 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
         *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W4q)[i];
     }
-    for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x)
-        sB[i] = i < 3 * MLP_H ? a.bq[i] : (i - 3 * MLP_H < MLP_OUT ? a.b[i] : 0.f);
+    for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x) {
+        if (i < 3 * MLP_H) {
+            // accumulator order of layer L: [t][g][j] <- bq[L H + 32 t + 16 (j / 8) + 8 g + (j % 8)]
+            const int L = i / MLP_H, r = i % MLP_H, t = r / 32, g = (r / 16) & 1, j = r & 15;
+            sB[i] = a.bq[L * MLP_H + 32 * t + 16 * (j >> 3) + 8 * g + (j & 7)];
+        } else {
+            sB[i] = i - 3 * MLP_H < MLP_OUT ? a.b[i] : 0.f;
+        }
+    }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5;
     const int ntiles = (a.n + 31) / 32;
@@ -425,13 +430,13 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         bf16x8 B[8];
         f32x16 acc[4];
         infer_feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
-        mlp_layer<MLP_INP, W1_LD, 4>(sW1, B, acc, l, sB);
+        mlp_layer<MLP_INP, W1_LD, 4, true>(sW1, B, acc, l, sB);
         mlp_activate(acc, B);
-        mlp_layer<MLP_H, WH_LD, 4>(sW2, B, acc, l, sB + MLP_H);
+        mlp_layer<MLP_H, WH_LD, 4, true>(sW2, B, acc, l, sB + MLP_H);
         mlp_activate(acc, B);
-        mlp_layer<MLP_H, WH_LD, 4>(sW3, B, acc, l, sB + 2 * MLP_H);
+        mlp_layer<MLP_H, WH_LD, 4, true>(sW3, B, acc, l, sB + 2 * MLP_H);
         mlp_activate(acc, B);
-        mlp_layer<MLP_H, WH_LD, 1>(sW4, B, acc, l, nullptr);
+        mlp_layer<MLP_H, WH_LD, 1, false>(sW4, B, acc, l, nullptr);
         // outputs 0..2 are accumulator rows 0..2 (elements 0..2 on lanes 0..31)
         if (g == 0 && i < a.n) {
             const float* b4 = sB + 3 * MLP_H;
