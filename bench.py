@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--workload", choices=("two_disk", "cosmic_web"), default="two_disk",
                     help="two_disk: the headline 4M-particle G=128 step; cosmic_web: 16.7M particles, "
                          "periodic G=512 with contact")
+    ap.add_argument("--timer-every", type=int, default=7,
+                    help="bracket one launch in N of the roofline kernel with HIP events")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -177,7 +179,9 @@ def main():
     log(f"N={n} G={G} warmup={args.warmup}")
     sim.run(args.warmup)
     sim.synchronize()
-    sim.kernel_timer(args.kernel)
+    # one launch in 7 bracketed by events (each bracket adds two queue markers; 7 is
+    # coprime with the sort interval, so sort steps are sampled at their share)
+    sim.kernel_timer(args.kernel, every=args.timer_every)
     log("timed region")
     sim.synchronize()
     t0 = time.perf_counter()

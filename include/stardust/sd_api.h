@@ -260,6 +260,9 @@ int sd_batch_get_snapshot(sd_batch* b, int snap, int scene, float* pos4, int n);
  * Brackets every launch of one named kernel with HIP events on the handle's
  * stream; sd_kernel_time_ms returns total milliseconds and launch count. */
 int sd_kernel_timer_enable(sd_sim* s, const char* kernelName);
+/* Bracket only one launch in `every` (default 1): each bracket is a pair of
+ * queue markers, so sampling keeps the timed run close to an untimed one. */
+int sd_kernel_timer_every(sd_sim* s, int every);
 int sd_kernel_time_ms(sd_sim* s, double* totalMs, long long* launches);
 
 #ifdef __cplusplus

@@ -196,6 +196,7 @@ def _declare(L):
         "sd_batch_snapshot_count": (C.c_int, [P]),
         "sd_batch_get_snapshot": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int]),
         "sd_kernel_timer_enable": (C.c_int, [P, C.c_char_p]),
+        "sd_kernel_timer_every": (C.c_int, [P, C.c_int]),
         "sd_kernel_time_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
@@ -428,8 +429,10 @@ class Sim:
         _check(lib().sd_mlp_correction(self._h, _ptr(out), n), "sd_mlp_correction")
         return out
 
-    def kernel_timer(self, name: str | None):
+    def kernel_timer(self, name: str | None, every: int = 1):
+        """Time launches of one named kernel with HIP events (one launch in `every`)."""
         _check(lib().sd_kernel_timer_enable(self._h, (name or "").encode()), "kernel_timer")
+        _check(lib().sd_kernel_timer_every(self._h, int(every)), "kernel_timer_every")
 
     def kernel_time_ms(self):
         ms, n = C.c_double(), C.c_longlong()

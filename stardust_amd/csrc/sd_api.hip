@@ -85,7 +85,10 @@ static bool dalloc(T*& p, size_t count) {
 
 static void timer_begin(sd_sim* s, const char* name) {
     KernelTimer& t = s->timer;
+    t.armed = false;
     if (t.name.empty() || t.name != name) return;
+    if ((t.seen++ % t.every) != 0) return;
+    t.armed = true;
     if (t.used + 2 > (int)t.pool.size()) {
         // harvest what is pending to recycle events
         HK(hipStreamSynchronize(s->st));
@@ -101,7 +104,8 @@ static void timer_begin(sd_sim* s, const char* name) {
 }
 static void timer_end(sd_sim* s, const char* name) {
     KernelTimer& t = s->timer;
-    if (t.name.empty() || t.name != name) return;
+    if (!t.armed || t.name.empty() || t.name != name) return;
+    t.armed = false;
     HK(hipEventRecord(t.pool[t.used + 1], s->st));
     t.used += 2;
 }
@@ -511,9 +515,11 @@ static void prime_scalars(sd_sim* s) {
 }
 
 // One step; no host synchronisation inside.
-static void enqueue_step(sd_sim* s) {
+// timed: record the three phase events of sd_timings (each is a queue marker
+// that drains the pipeline, so runs of steps record them on their last step).
+static void enqueue_step(sd_sim* s, bool timed) {
     const bool fused = s->modes.pipeline == SD_PIPELINE_FUSED;
-    HK(hipEventRecord(s->evA, s->st));
+    if (timed) HK(hipEventRecord(s->evA, s->st));
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
                          !s->cfg.contactEnabled;
     // With nothing between the sort and the integrate that reads particles in
@@ -547,7 +553,7 @@ static void enqueue_step(sd_sim* s) {
         solve_mesh(s);
     }
     const bool contact = contact_pass(s);
-    HK(hipEventRecord(s->ev1, s->st));
+    if (timed) HK(hipEventRecord(s->ev1, s->st));
 
     IntegrateArgs a{};
     a.accG = s->accG;
@@ -607,7 +613,7 @@ static void enqueue_step(sd_sim* s) {
     s->keysSorted = false;
     s->corrAge++;
     s->stepCount++;
-    HK(hipEventRecord(s->evB, s->st));
+    if (timed) HK(hipEventRecord(s->evB, s->st));
 }
 
 static void collect_timings(sd_sim* s) {
@@ -910,7 +916,7 @@ int sd_step(sd_sim* s) {
     int rc = check_ready(s);
     if (rc) return rc;
     if (s->allocN <= 0) return SD_OK;
-    enqueue_step(s);
+    enqueue_step(s, true);
     HK(hipGetLastError());
     collect_timings(s);
     timer_harvest(s);
@@ -922,7 +928,7 @@ int sd_run(sd_sim* s, int steps) {
     if (rc) return rc;
     if (steps < 0) return SD_ERR_ARG;
     if (s->allocN <= 0) return SD_OK;
-    for (int k = 0; k < steps; ++k) enqueue_step(s);
+    for (int k = 0; k < steps; ++k) enqueue_step(s, k == steps - 1);
     HK(hipGetLastError());
     if (steps > 0) {
         HK(hipStreamSynchronize(s->st));
@@ -1373,11 +1379,20 @@ int sd_kernel_timer_enable(sd_sim* s, const char* name) {
     t.used = 0;
     t.totalMs = 0.0;
     t.launches = 0;
+    t.seen = 0;
+    t.armed = false;
     if (!t.name.empty() && t.pool.empty()) {
         t.pool.resize(1024);
         for (auto& e : t.pool) HK(hipEventCreate(&e));
     }
     return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_kernel_timer_every(sd_sim* s, int every) {
+    if (!s || every < 1) return SD_ERR_ARG;
+    s->timer.every = every;
+    s->timer.seen = 0;
+    return SD_OK;
 }
 
 int sd_kernel_time_ms(sd_sim* s, double* totalMs, long long* launches) {

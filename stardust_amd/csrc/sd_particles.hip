@@ -601,7 +601,6 @@ __device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ 
     __shared__ float s_f[FIN];
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
-#pragma unroll 8
     for (int k = threadIdx.x; k < np; k += FIN) {
         const ReducePartial r = parts[k];
         sx += r.sx;

@@ -32,6 +32,9 @@ struct KernelTimer {
     int used = 0;
     double totalMs = 0.0;
     long long launches = 0;
+    int every = 1;        // time one launch in `every` (each bracket is two queue markers)
+    long long seen = 0;   // launches of the named kernel so far
+    bool armed = false;   // the current launch is bracketed
 };
 
 struct sd_sim {
