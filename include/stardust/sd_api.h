@@ -154,7 +154,7 @@ int sd_synchronize(sd_sim* s);     /* no reference counterpart */
 
 /* ---- reads: Sim::config/timings/simTime/gridSize/particleCount/activeCount */
 int sd_get_config(const sd_sim* s, sd_config* cfg);
-int sd_timings_get(sd_sim* s, sd_timings* t);
+int sd_timings_get(sd_sim* s, sd_timings* t);  /* phases of the last step (of the last sd_run) */
 double sd_sim_time(sd_sim* s);
 long long sd_step_count(sd_sim* s);  /* no reference counterpart */
 int sd_grid_size(const sd_sim* s);

@@ -404,7 +404,12 @@ static void solve_mesh(sd_sim* s) {
     TIMED(s, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, s->st));
     TIMED(s, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, s->st));
     const float potScale = s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1);
-    TIMED(s, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, s->st));
+    if (s->finDeferred) {
+        TIMED(s, "grad", launch_grid_accel_fin(s->pot, s->accG, s->G, potScale, s->periodic, s->finJob, s->st));
+        s->finDeferred = false;
+    } else {
+        TIMED(s, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, s->st));
+    }
     s->rhoValid = false;
     s->meshValid = true;
 }
@@ -517,7 +522,16 @@ static void prime_scalars(sd_sim* s) {
 // One step; no host synchronisation inside.
 // timed: record the three phase events of sd_timings (each is a queue marker
 // that drains the pipeline, so runs of steps record them on their last step).
-static void enqueue_step(sd_sim* s, bool timed) {
+// Enqueue a finalize that was left for the next gradient launch.
+static void flush_finalize(sd_sim* s) {
+    if (!s->finDeferred) return;
+    const FinalizeJob& j = s->finJob;
+    TIMED(s, "finalize", launch_finalize(j.partials, j.nparts, j.mode, j.pol, j.sc, s->st));
+    s->finDeferred = false;
+}
+
+// last: the final step of a run, whose finalize is launched at once.
+static void enqueue_step(sd_sim* s, bool timed, bool last) {
     const bool fused = s->modes.pipeline == SD_PIPELINE_FUSED;
     if (timed) HK(hipEventRecord(s->evA, s->st));
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
@@ -554,6 +568,7 @@ static void enqueue_step(sd_sim* s, bool timed) {
     }
     const bool contact = contact_pass(s);
     if (timed) HK(hipEventRecord(s->ev1, s->st));
+    flush_finalize(s);  // the scalars must be final before this integrate reads them
 
     IntegrateArgs a{};
     a.accG = s->accG;
@@ -588,7 +603,12 @@ static void enqueue_step(sd_sim* s, bool timed) {
             std::swap(s->vel, s->velTmp);
         }
         const int mode = (s->modes.momentumZero ? 1 : 0) | 2 | 8 | 4;
-        TIMED(s, "finalize", launch_finalize(s->partials, s->npartsInt, mode, pol, s->sc, s->st));
+        if (last) {
+            TIMED(s, "finalize", launch_finalize(s->partials, s->npartsInt, mode, pol, s->sc, s->st));
+        } else {
+            s->finJob = FinalizeJob{s->partials, s->npartsInt, mode, pol, s->sc};
+            s->finDeferred = true;
+        }
         s->primed = true;
         s->rhoValid = fusedDeposit;
     } else {
@@ -916,7 +936,7 @@ int sd_step(sd_sim* s) {
     int rc = check_ready(s);
     if (rc) return rc;
     if (s->allocN <= 0) return SD_OK;
-    enqueue_step(s, true);
+    enqueue_step(s, true, true);
     HK(hipGetLastError());
     collect_timings(s);
     timer_harvest(s);
@@ -928,7 +948,7 @@ int sd_run(sd_sim* s, int steps) {
     if (rc) return rc;
     if (steps < 0) return SD_ERR_ARG;
     if (s->allocN <= 0) return SD_OK;
-    for (int k = 0; k < steps; ++k) enqueue_step(s, k == steps - 1);
+    for (int k = 0; k < steps; ++k) enqueue_step(s, k == steps - 1, k == steps - 1);
     HK(hipGetLastError());
     if (steps > 0) {
         HK(hipStreamSynchronize(s->st));

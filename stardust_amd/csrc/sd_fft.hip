@@ -752,31 +752,7 @@ struct GradJob {
     int pad;
 };
 
-// ---- gradient: kGridAccel, Sim.cu:589-611 ----------------------------------
-// a = -(pot[+1] - pot[-1]) * (0.5 * G * potScale); isolated clamps the stencil,
-// periodic wraps. pot and acc are G^3 [z][y][x].
-__device__ __forceinline__ void grid_accel_body(const float* __restrict__ pot, float4* __restrict__ acc, int G,
-                                                float h, int periodic, long long i) {
-    const long long cells = (long long)G * G * G;
-    if (i >= cells) return;
-    const int x = (int)(i % G), y = (int)((i / G) % G), z = (int)(i / ((long long)G * G));
-    int xm, xp, ym, yp, zm, zp;
-    if (periodic) {
-        xm = x == 0 ? G - 1 : x - 1; xp = x == G - 1 ? 0 : x + 1;
-        ym = y == 0 ? G - 1 : y - 1; yp = y == G - 1 ? 0 : y + 1;
-        zm = z == 0 ? G - 1 : z - 1; zp = z == G - 1 ? 0 : z + 1;
-    } else {
-        xm = max(x - 1, 0); xp = min(x + 1, G - 1);
-        ym = max(y - 1, 0); yp = min(y + 1, G - 1);
-        zm = max(z - 1, 0); zp = min(z + 1, G - 1);
-    }
-    const size_t row = ((size_t)z * G + y) * G;
-    const float ax = -(pot[row + xp] - pot[row + xm]) * h;
-    const float ay = -(pot[((size_t)z * G + yp) * G + x] - pot[((size_t)z * G + ym) * G + x]) * h;
-    const float az = -(pot[((size_t)zp * G + y) * G + x] - pot[((size_t)zm * G + y) * G + x]) * h;
-    acc[i] = make_float4(ax, ay, az, 0.f);
-}
-
+// ---- gradient: kGridAccel, Sim.cu:589-611 (grid_accel_body in sd_device.h) ---
 __global__ void __launch_bounds__(256) k_grid_accel(const float* __restrict__ pot, float4* __restrict__ acc,
                                                     int G, float h, int periodic) {
     grid_accel_body(pot, acc, G, h, periodic, (long long)blockIdx.x * blockDim.x + threadIdx.x);

@@ -147,6 +147,9 @@ struct FinalizeJob {
     StepScalars* sc;
 };
 void launch_finalize_batch(const FinalizeJob* jobs, int njobs, hipStream_t st);
+// mesh_grid_accel with a finalize (of the previous integrate) as one more block
+void launch_grid_accel_fin(const float* pot, float4* acc, int G, float potScale, int periodic,
+                           const FinalizeJob& job, hipStream_t st);
 void launch_copy_f4(const float4* src, float4* dst, long long n, hipStream_t st);
 void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G,
                           int periodic, const StepScalars* sc, float4* velOut, hipStream_t st);

@@ -594,20 +594,36 @@ void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial*
 }
 
 // Deterministic reduction (fixed summation order) + dt policy.
-constexpr int FIN = 1024;
+// Threads of every finalize reduction (its summation order depends on it, so all
+// callers -- the finalize launch, the per-scene batch launch and the finalize
+// block riding on the next gradient launch -- use the same count).
+constexpr int FIN = 256;
 __device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ parts, int np, int mode,
                                               const DtPolicy& pol, StepScalars* sc) {
     __shared__ double s_d[FIN][4];
     __shared__ float s_f[FIN];
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
-    for (int k = threadIdx.x; k < np; k += FIN) {
-        const ReducePartial r = parts[k];
-        sx += r.sx;
-        sy += r.sy;
-        sz += r.sz;
-        cnt += r.cnt;
-        vm = fmaxf(vm, r.vmax);
+    // partials k = thread + i FIN in ascending i, loaded 8 at a time so the
+    // loads overlap (the summation order is that of the plain loop)
+    constexpr int UNR = 8;
+    for (int base = threadIdx.x; base < np; base += UNR * FIN) {
+        ReducePartial r[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int k = base + u * FIN;
+            if (k < np) r[u] = parts[k];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (base + u * FIN < np) {
+                sx += r[u].sx;
+                sy += r[u].sy;
+                sz += r[u].sz;
+                cnt += r[u].cnt;
+                vm = fmaxf(vm, r[u].vmax);
+            }
+        }
     }
     s_d[threadIdx.x][0] = sx;
     s_d[threadIdx.x][1] = sy;
@@ -667,6 +683,26 @@ __global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restric
 __global__ void __launch_bounds__(FIN) k_finalize_batch(const FinalizeJob* __restrict__ jobs) {
     const FinalizeJob j = jobs[blockIdx.x];
     finalize_body(j.partials, j.nparts, j.mode, j.pol, j.sc);
+}
+
+// Gradient of the next step with the previous step's finalize as block 0
+// (blocks 1.. are k_grid_accel's): the reduction only has to be done before
+// the next integrate reads the scalars, and it then overlaps the gradient.
+__global__ void __launch_bounds__(FIN) k_grid_accel_fin(const float* __restrict__ pot, float4* __restrict__ acc,
+                                                        int G, float h, int periodic, FinalizeJob job) {
+    if (blockIdx.x == 0) {
+        finalize_body(job.partials, job.nparts, job.mode, job.pol, job.sc);
+        return;
+    }
+    grid_accel_body(pot, acc, G, h, periodic, (long long)(blockIdx.x - 1) * FIN + threadIdx.x);
+}
+
+void launch_grid_accel_fin(const float* pot, float4* acc, int G, float potScale, int periodic,
+                           const FinalizeJob& job, hipStream_t st) {
+    const long long cells = (long long)G * G * G;
+    const float h = 0.5f * (float)G * potScale;
+    hipLaunchKernelGGL(k_grid_accel_fin, dim3((unsigned)(1 + (cells + FIN - 1) / FIN)), dim3(FIN), 0, st, pot, acc,
+                       G, h, periodic, job);
 }
 
 void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol, StepScalars* sc,

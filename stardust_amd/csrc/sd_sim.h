@@ -75,6 +75,10 @@ struct sd_sim {
     int nparts = 0;     // one-particle-per-thread reductions
     int npartsInt = 0;  // k_integrate partials
     sd::StepScalars* sc = nullptr;
+    // a finalize whose launch rides on the next gradient launch (only between
+    // steps of one sd_run; outside a run every finalize has been enqueued)
+    bool finDeferred = false;
+    sd::FinalizeJob finJob{};
     sd::StepScalars* scHost = nullptr;  // pinned mirror for reads
 
     // state
