@@ -597,16 +597,19 @@ void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial*
 // Threads of every finalize reduction (its summation order depends on it, so all
 // callers -- the finalize launch, the per-scene batch launch and the finalize
 // block riding on the next gradient launch -- use the same count).
-constexpr int FIN = 256;
+constexpr int FIN = 1024;
 __device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ parts, int np, int mode,
                                               const DtPolicy& pol, StepScalars* sc) {
-    __shared__ double s_d[FIN][4];
-    __shared__ float s_f[FIN];
-    double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+    // fixed order: per thread the partials k = thread + i FIN in ascending i
+    // (loaded 4 at a time so the loads overlap), a butterfly inside each
+    // wavefront, then the wavefronts in index order
+    __shared__ double s_w[FIN / 64][4];
+    __shared__ float s_v[FIN / 64];
+    __shared__ double s_d[1][4];
+    __shared__ float s_f[1];
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
     float vm = 0.f;
-    // partials k = thread + i FIN in ascending i, loaded 8 at a time so the
-    // loads overlap (the summation order is that of the plain loop)
-    constexpr int UNR = 8;
+    constexpr int UNR = 4;
     for (int base = threadIdx.x; base < np; base += UNR * FIN) {
         ReducePartial r[UNR];
 #pragma unroll
@@ -617,27 +620,32 @@ __device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ 
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             if (base + u * FIN < np) {
-                sx += r[u].sx;
-                sy += r[u].sy;
-                sz += r[u].sz;
-                cnt += r[u].cnt;
+                d[0] += r[u].sx;
+                d[1] += r[u].sy;
+                d[2] += r[u].sz;
+                d[3] += r[u].cnt;
                 vm = fmaxf(vm, r[u].vmax);
             }
         }
     }
-    s_d[threadIdx.x][0] = sx;
-    s_d[threadIdx.x][1] = sy;
-    s_d[threadIdx.x][2] = sz;
-    s_d[threadIdx.x][3] = cnt;
-    s_f[threadIdx.x] = vm;
-    __syncthreads();
-    for (int s = FIN / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) s_d[threadIdx.x][q] += s_d[threadIdx.x + s][q];
-            s_f[threadIdx.x] = fmaxf(s_f[threadIdx.x], s_f[threadIdx.x + s]);
+    for (int q = 0; q < 4; ++q) d[q] = wave_sum(d[q]);
+    vm = wave_max(vm);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_w[threadIdx.x >> 6][q] = d[q];
+        s_v[threadIdx.x >> 6] = vm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        float m = 0.f;
+        for (int w = 0; w < FIN / 64; ++w) {
+            for (int q = 0; q < 4; ++q) t[q] += s_w[w][q];
+            m = fmaxf(m, s_v[w]);
         }
-        __syncthreads();
+        for (int q = 0; q < 4; ++q) s_d[0][q] = t[q];
+        s_f[0] = m;
     }
     if (threadIdx.x != 0) return;
     if (mode & 1) {
