@@ -137,7 +137,7 @@ static void free_buffers(sd_sim* s) {
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
-    dfree(s->cellStart); dfree(s->cellEnd); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
+    dfree(s->cellRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
     s->cellTableCells = 0;
     if (s->scHost) {
         (void)hipHostFree(s->scHost);
@@ -289,14 +289,10 @@ static long long cell_table_size(const sd_sim* s) {
 
 static bool ensure_contact_buffers(sd_sim* s) {
     const size_t cells = (size_t)cell_table_size(s);
-    if (s->cellTableCells < (long long)cells) {
-        dfree(s->cellStart);
-        dfree(s->cellEnd);
-    }
+    if (s->cellTableCells < (long long)cells) dfree(s->cellRange);
     s->cellTableCells = (long long)cells;
     if (!s->accContact && !dalloc(s->accContact, (size_t)s->allocN)) return false;
-    if (!s->cellStart && !dalloc(s->cellStart, cells)) return false;
-    if (!s->cellEnd && !dalloc(s->cellEnd, cells)) return false;
+    if (!s->cellRange && !dalloc(s->cellRange, cells)) return false;
     return true;
 }
 
@@ -432,10 +428,9 @@ static bool contact_pass(sd_sim* s) {
     if ((double)s->allocN > 0.764 * (double)G * G * G) return false;
     if (!ensure_contact_buffers(s)) return false;
     sort_particles(s, true);
-    const long long cells = cell_table_size(s);
     TIMED(s, "contact", {
-        launch_cell_range(s->sb.keys, s->allocN, (int)cells, s->cellStart, s->cellEnd, s->st);
-        launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellStart, s->cellEnd,
+        launch_cell_range(s->sb.keys, s->allocN, s->cellRange, s->st);
+        launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellRange, s->sb.keys,
                        s->cfg.contactRadiusCells / (float)G, s->cfg.contactStiffness, s->cfg.contactDamping,
                        s->accContact, s->st);
     });

@@ -77,8 +77,12 @@ int sort_chunks_for(int n);
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st);
 void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
                     float4* dstV, int n, hipStream_t st);
-void launch_cell_range(const uint32_t* keys, int n, int ncells, int32_t* start, int32_t* end,
-                       hipStream_t st);
+// Cell table of sorted keys (key = cell + 1, 0 = inactive): range[c] = (first,
+// one past last) sorted index of cell c. Only non-empty cells are written; an
+// entry is current exactly when its first index f satisfies 0 <= f < n and
+// keys[f] == c + 1 (entries of cells emptied since an earlier call keep stale
+// values, so the table is never cleared).
+void launch_cell_range(const uint32_t* keys, int n, int2* range, hipStream_t st);
 
 // ---- deposit / integrate ----------------------------------------------------
 void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long long* rho,
@@ -156,7 +160,7 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
 
 // ---- contact ------------------------------------------------------------------
 void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int32_t* cellStart, const int32_t* cellEnd, float radius,
+                    const int2* cellRange, const uint32_t* sortedKeys, float radius,
                     float stiffness, float damping, float4* accOut, hipStream_t st);
 
 // ---- direct sum -------------------------------------------------------------

@@ -1,8 +1,10 @@
 // sd_mlp.hip -- learned sub-cell force correction (no reference counterpart).
 //
-// MLP 30 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form), BF16 weights and
-// activations, FP32 accumulation on the gfx950 matrix cores
-// (v_mfma_f32_32x32x16_bf16). Inputs per particle: the 27 CIC-node densities
+// MLP 30 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form), BF16 weights, FP32
+// accumulation on the gfx950 matrix cores. Training runs BF16 activations
+// (v_mfma_f32_32x32x16_bf16); the fused inference kernel holds the same BF16
+// weight values as FP16 (exact) with FP16 activations
+// (v_mfma_f32_32x32x16_f16). Inputs per particle: the 27 CIC-node densities
 // around it (log1p of particles per cell) and its offset from the nearest node
 // (x2); output: the direct-sum minus PM acceleration in units of
 // a_unit * outScale, a_unit = gravity * G / N (one particle at one cell).
@@ -51,6 +53,64 @@ __device__ __forceinline__ float gelu_d(float u) {
 
 __device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---- inference element type --------------------------------------------------
+// The fused inference kernel runs its matrix products on FP16 operands: the
+// BF16 weights convert to FP16 exactly (8 significant bits fit in 11; every
+// |w| in [2^-17, 65504] keeps its value), the activations gain 3 mantissa bits
+// over BF16, and GELU evaluates on packed FP16 pairs (2 values per VALU lane
+// op), which is what bounds this kernel. SD_MLP_F16=0 keeps BF16 activations
+// and the FP32 GELU.
+#ifndef SD_MLP_F16
+#define SD_MLP_F16 1
+#endif
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+#if SD_MLP_F16
+typedef _Float16 inf_t;
+typedef f16x8 infx8;
+__device__ __forceinline__ f32x16 mfma_inf(f16x8 a, f16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+#else
+typedef __bf16 inf_t;
+typedef bf16x8 infx8;
+__device__ __forceinline__ f32x16 mfma_inf(bf16x8 a, bf16x8 b, f32x16 c) { return mfma32(a, b, c); }
+#endif
+
+// GELU on FP16 values, 8 at a time: gelu(u) = u phi(u), phi = 0.5 + uc P(x),
+// uc = clamp(u, -4, 4), x = uc^2 / 8 - 1 in [-1, 1], P degree 6 (a weighted
+// least-squares fit of tanh(sqrt(2/pi) (u + 0.044715 u^3)) / (2u) with
+// P(1) = 1/8, so phi(+-4) is exactly 1 and 0 in FP16 arithmetic). The centred
+// variable keeps the coefficients below 0.2, so Horner's rule in FP16 loses
+// little: over every FP16 input the result is within 2.4e-3 of the tanh-form
+// GELU (mean 6e-5 for |u| <= 4). Each vector operation is four independent
+// packed instructions, which keeps dependent packed operations apart (gfx950
+// needs a wait state between a packed FP16 result and its next use).
+__device__ __forceinline__ f16x8 gelu_h8(f16x8 u) {
+    const f16x8 uc = __builtin_elementwise_min(__builtin_elementwise_max(u, (f16x8)(-4.0f16)), (f16x8)(4.0f16));
+    const f16x8 x = __builtin_elementwise_fma(uc * uc, (f16x8)(0.125f16), (f16x8)(-1.0f16));
+    f16x8 r = (f16x8)(0.00838470458984375f16);
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.0185089111328125f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.0215911865234375f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.0335693359375f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.055816650390625f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.08477783203125f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.176025390625f16));
+    return u * __builtin_elementwise_fma(uc, r, (f16x8)(0.5f16));
+}
+
+// Eight FP32 accumulators -> FP16 (round toward zero, 2 per instruction).
+__device__ __forceinline__ f16x8 to_f16x8(const float* v) {
+    f16x8 h;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        const f16x2 p = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(v[e], v[e + 1]));
+        h[e] = p[0];
+        h[e + 1] = p[1];
+    }
+    return h;
 }
 
 // ---- generic NT GEMM: C[M x N] = A[M x K] * B^T, A row-major (lda), Bt = B^T
@@ -284,8 +344,8 @@ __global__ void k_features(const float4* __restrict__ pos, const int32_t* __rest
 }
 
 // ---- fused inference ------------------------------------------------------------
-// Persistent blocks of 8 waves; the BF16 weights (W1p and the permuted W2q,
-// W3q, W4q; padded rows) and biases live in LDS for the block's lifetime.
+// Persistent blocks (one per CU); the weights (W1p and the permuted W2q, W3q,
+// W4q; padded rows, as FP16) and biases live in LDS for the block's lifetime.
 // Each wave advances tiles of 32 particles: Y^T = W X^T with the
 // weights as the A operand (from LDS) and
 // the activations as the B operand, which never leave registers: accumulator
@@ -293,9 +353,14 @@ __global__ void k_features(const float4* __restrict__ pos, const int32_t* __rest
 // fragment of k-step 2t + h of the next layer because that layer's weight
 // columns are stored in the same (permuted) order. Output: the correction as
 // a float4 per particle (inactive particles get 0).
-constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (bf16); +8 spreads rows over banks
+constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (2-byte elements); +8 spreads rows over banks
 constexpr int WH_LD = MLP_H + 8;
-constexpr int INF_WAVES = 12;
+#ifndef SD_INF_WAVES
+#define SD_INF_WAVES 16
+#endif
+// Waves per block. The weights take 90 KB of LDS, so one block runs per CU;
+// 16 waves are the 4 per SIMD that 118 registers allow (12: 5% slower).
+constexpr int INF_WAVES = SD_INF_WAVES;
 
 // One layer, Y^T = W X^T: the accumulators start from the layer's biases, so
 // no add is needed afterwards (BIAS false starts from zero). The biases
@@ -304,7 +369,7 @@ constexpr int INF_WAVES = 12;
 // i.e. bq[32 t + 16 (j / 8) + 8 g + (j % 8)], so one aligned 64-byte read fills
 // the accumulator registers.
 template <int K, int LDW, int NT, bool BIAS>
-__device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const bf16x8* B, f32x16* acc, int l,
+__device__ __forceinline__ void mlp_layer(const inf_t* __restrict__ Wl, const infx8* B, f32x16* acc, int l,
                                           const float* __restrict__ bias) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -317,20 +382,27 @@ __device__ __forceinline__ void mlp_layer(const __bf16* __restrict__ Wl, const b
     for (int s = 0; s < K / 16; ++s) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(Wl + (t * 32 + (l & 31)) * LDW + 16 * s + 8 * (l >> 5));
-            acc[t] = mfma32(a, B[s], acc[t]);
+            const infx8 a = *reinterpret_cast<const infx8*>(Wl + (t * 32 + (l & 31)) * LDW + 16 * s + 8 * (l >> 5));
+            acc[t] = mfma_inf(a, B[s], acc[t]);
         }
     }
 }
 
 // GELU of a layer's accumulators (biases already in them) into the next
 // layer's B fragments: activation (s, e) is accumulator (s / 2, 8 (s % 2) + e).
-__device__ __forceinline__ void mlp_activate(const f32x16* acc, bf16x8* B) {
+__device__ __forceinline__ void mlp_activate(const f32x16* acc, infx8* B) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        bf16x8 v;
+        infx8 v;
+#if SD_MLP_F16
+        float a8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a8[e] = acc[s >> 1][8 * (s & 1) + e];
+        v = gelu_h8(to_f16x8(a8));
+#else
 #pragma unroll
         for (int e = 0; e < 8; ++e) v[e] = (__bf16)gelu_f(acc[s >> 1][8 * (s & 1) + e]);
+#endif
         B[s] = v;
     }
 }
@@ -350,7 +422,7 @@ __device__ __forceinline__ float log1p_count(const long long* __restrict__ rho, 
 }
 
 __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
-                                                    int periodic, int g, bf16x8* B) {
+                                                    int periodic, int g, infx8* B) {
     const float fG = (float)G;
     const int cx = min(max((int)floorf(p.x * fG), 0), G - 1);
     const int cy = min(max((int)floorf(p.y * fG), 0), G - 1);
@@ -367,7 +439,7 @@ __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const 
     const float off2 = 2.0f * (p.z * fG - 0.5f - (float)cz);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        bf16x8 v;
+        infx8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int q0 = 16 * s + e, q1 = q0 + 8;  // feature of half 0 / half 1 (compile time)
@@ -382,31 +454,46 @@ __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const 
                 if (alive && !g) d0 = log1p_count(rho, zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3]);
                 val = g ? (alive ? o : 0.f) : d0;
             }
-            v[e] = (__bf16)val;
+            v[e] = (inf_t)val;
         }
         B[s] = v;
     }
 }
 
+// Eight BF16 weights (as stored) -> eight inference elements (FP16: exact).
+__device__ __forceinline__ uint4 to_inf(uint4 v) {
+#if SD_MLP_F16
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const f16x2 h = {(_Float16)__uint_as_float(w[k] << 16), (_Float16)__uint_as_float(w[k] & 0xffff0000u)};
+        w[k] = __builtin_bit_cast(uint32_t, h);
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+#else
+    return v;
+#endif
+}
+
 __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     extern __shared__ unsigned char smem[];
-    __bf16* sW1 = reinterpret_cast<__bf16*>(smem);
-    __bf16* sW2 = sW1 + MLP_H * W1_LD;
-    __bf16* sW3 = sW2 + MLP_H * WH_LD;
-    __bf16* sW4 = sW3 + MLP_H * WH_LD;
+    inf_t* sW1 = reinterpret_cast<inf_t*>(smem);
+    inf_t* sW2 = sW1 + MLP_H * W1_LD;
+    inf_t* sW3 = sW2 + MLP_H * WH_LD;
+    inf_t* sW4 = sW3 + MLP_H * WH_LD;
     float* sB = reinterpret_cast<float*>(sW4 + MLP_OUTP * WH_LD);  // b1q | b2q | b3q | b4 (padded to 32)
     for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
         const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
-        *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W1p)[i];
+        *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i]);
     }
     for (int i = threadIdx.x; i < MLP_H * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W2q)[i];
-        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W3q)[i];
+        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W2q)[i]);
+        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W3q)[i]);
     }
     for (int i = threadIdx.x; i < MLP_OUTP * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = reinterpret_cast<const uint4*>(a.W4q)[i];
+        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W4q)[i]);
     }
     for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x) {
         if (i < 3 * MLP_H) {
@@ -427,7 +514,7 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         const int i = tile * 32 + (l & 31);
         float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
         if (i < a.n) p = a.pos[i];
-        bf16x8 B[8];
+        infx8 B[8];
         f32x16 acc[4];
         infer_feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
         mlp_layer<MLP_INP, W1_LD, 4, true>(sW1, B, acc, l, sB);
@@ -447,7 +534,7 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
 }
 
 size_t mlp_infer_smem() {
-    return sizeof(__bf16) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD) +
+    return sizeof(inf_t) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD) +
            sizeof(float) * (3 * MLP_H + MLP_OUTP);
 }
 

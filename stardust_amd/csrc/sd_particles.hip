@@ -801,9 +801,31 @@ __device__ __forceinline__ int cell_slot(int x, int y, int z, int G, int periodi
     return keyMode ? (int)morton3((uint32_t)x, (uint32_t)y, (uint32_t)z) : (z * G + y) * G + x;
 }
 
+// Squared distance bound (in cells) along one axis from a particle at offset o
+// (cell units, [0, 1)) inside cell c to the cells c - 1, c, c + 1: every
+// particle keyed to cell c - 1 lies at least o away, to c + 1 at least 1 - o.
+// A neighbour that folds onto another cell (the clamped edge of an isolated
+// box) gets a large negative value, so that no sum including it passes the
+// pruning test.
+__device__ __forceinline__ void axis_gaps(float o, int c, int G, int periodic, float g[3]) {
+    const float lo = fmaxf(o, 0.f), hi = fmaxf(1.f - o, 0.f);
+    g[0] = (periodic || c > 0) ? lo * lo : -1e30f;
+    g[1] = 0.f;
+    g[2] = (periodic || c < G - 1) ? hi * hi : -1e30f;
+}
+
+// The reference visits the 27 cells around the particle's cell in z, y, x
+// order and stops after 96 candidates (self excluded; every candidate counts,
+// in range or not). Here the nine range entries of a z-plane load together,
+// and a cell whose nearest point lies beyond the contact diameter plus 1e-3
+// cell is not read: its particles would all fail the distance test, so it only
+// adds its count to the candidate tally (capped at 96 as the reference's loop
+// caps it). The 1e-3 cell margin is far above the float rounding of keys,
+// offsets and distances (about 3e-5 cell at G = 512), so forces and the
+// candidate cap are exactly those of the full visit.
 __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
-                                               int G, int periodic, int keyMode, const int32_t* __restrict__ cellStart,
-                                               const int32_t* __restrict__ cellEnd, float radius, float stiffness,
+                                               int G, int periodic, int keyMode, const int2* __restrict__ range,
+                                               const uint32_t* __restrict__ keys, float radius, float stiffness,
                                                float damping, float4* __restrict__ accOut) {
     const int i = blockIdx.x * PT + threadIdx.x;
     if (i >= n) return;
@@ -820,51 +842,71 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
     const int cx = min(max((int)(p.x * fG), 0), G - 1);
     const int cy = min(max((int)(p.y * fG), 0), G - 1);
     const int cz = min(max((int)(p.z * fG), 0), G - 1);
+    float gx[3], gy[3], gz[3];
+    axis_gaps(p.x * fG - (float)cx, cx, G, periodic, gx);
+    axis_gaps(p.y * fG - (float)cy, cy, G, periodic, gy);
+    axis_gaps(p.z * fG - (float)cz, cz, G, periodic, gz);
+    const float reach = fmaf(d0, fG, 1e-3f);  // contact diameter in cells, plus the margin
+    const float reach2 = reach * reach;
     const int kMaxPeers = 96;
     int seen = 0;
-    for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz)
-        for (int dy = -1; dy <= 1 && seen < kMaxPeers; ++dy)
-            for (int dx = -1; dx <= 1 && seen < kMaxPeers; ++dx) {
-                const int c = cell_slot(cx + dx, cy + dy, cz + dz, G, periodic, keyMode);
-                const int s0 = cellStart[c], e0 = cellEnd[c];
-                if (s0 < 0 || e0 <= s0) continue;
-                for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
-                    if (j == i) continue;
-                    ++seen;
-                    const float4 q = pos[j];
-                    if (q.x < 0.f) continue;
-                    float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
-                    if (periodic) {
-                        if (ex > 0.5f) ex -= 1.f; else if (ex < -0.5f) ex += 1.f;
-                        if (ey > 0.5f) ey -= 1.f; else if (ey < -0.5f) ey += 1.f;
-                        if (ez > 0.5f) ez -= 1.f; else if (ez < -0.5f) ez += 1.f;
-                    }
-                    const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                    if (dd >= d02 || dd < 1e-16f) continue;
-                    const float d = sqrtf(dd);
-                    const float nx = ex / d, ny = ey / d, nz = ez / d;
-                    const float4 vj = vel[j];
-                    const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
-                    float f = 0.f;
-                    if (d < d0) {
-                        float fc = fmaf(stiffness, d0 - d, -(damping * vn));
-                        if (fc < 0.f) fc = 0.f;
-                        f += fc;
-                    }
-                    ax = fmaf(f, nx, ax);
-                    ay = fmaf(f, ny, ay);
-                    az = fmaf(f, nz, az);
-                }
+    for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz) {
+        int cs[9];
+        int2 r[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            cs[q] = cell_slot(cx + q % 3 - 1, cy + q / 3 - 1, cz + dz, G, periodic, keyMode);
+            r[q] = range[cs[q]];
+        }
+        uint32_t kf[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) kf[q] = (r[q].x >= 0 && r[q].x < n) ? keys[r[q].x] : 0u;
+        for (int q = 0; q < 9 && seen < kMaxPeers; ++q) {
+            if (kf[q] != (uint32_t)cs[q] + 1u) continue;  // empty cell (stale or unwritten entry)
+            const int s0 = r[q].x, e0 = r[q].y;
+            if (e0 <= s0) continue;
+            if (gx[q % 3] + gy[q / 3] + gz[dz + 1] > reach2) {
+                seen = min(seen + (e0 - s0), kMaxPeers);  // out of reach: the particle is not in this cell
+                continue;
             }
+            for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
+                if (j == i) continue;
+                ++seen;
+                const float4 q4 = pos[j];
+                if (q4.x < 0.f) continue;
+                float ex = p.x - q4.x, ey = p.y - q4.y, ez = p.z - q4.z;
+                if (periodic) {
+                    if (ex > 0.5f) ex -= 1.f; else if (ex < -0.5f) ex += 1.f;
+                    if (ey > 0.5f) ey -= 1.f; else if (ey < -0.5f) ey += 1.f;
+                    if (ez > 0.5f) ez -= 1.f; else if (ez < -0.5f) ez += 1.f;
+                }
+                const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                if (dd >= d02 || dd < 1e-16f) continue;
+                const float d = sqrtf(dd);
+                const float nx = ex / d, ny = ey / d, nz = ez / d;
+                const float4 vj = vel[j];
+                const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
+                float f = 0.f;
+                if (d < d0) {
+                    float fc = fmaf(stiffness, d0 - d, -(damping * vn));
+                    if (fc < 0.f) fc = 0.f;
+                    f += fc;
+                }
+                ax = fmaf(f, nx, ax);
+                ay = fmaf(f, ny, ay);
+                az = fmaf(f, nz, az);
+            }
+        }
+    }
     accOut[i] = make_float4(ax, ay, az, 0.f);
 }
 
 void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int32_t* cellStart, const int32_t* cellEnd, float radius, float stiffness, float damping,
+                    const int2* cellRange, const uint32_t* sortedKeys, float radius, float stiffness, float damping,
                     float4* accOut, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_contact, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
-                       cellStart, cellEnd, radius, stiffness, damping, accOut);
+                       cellRange, sortedKeys, radius, stiffness, damping, accOut);
 }
 
 // ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------

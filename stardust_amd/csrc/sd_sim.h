@@ -66,7 +66,7 @@ struct sd_sim {
     float4* accG = nullptr;
     float2* tw = nullptr;
     float4* accContact = nullptr;
-    int32_t *cellStart = nullptr, *cellEnd = nullptr;
+    int2* cellRange = nullptr;  // contact cell table: (first, one past last) sorted index per cell
     long long cellTableCells = 0;
     sd::MeshPlan plan{};
 

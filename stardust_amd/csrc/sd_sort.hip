@@ -270,14 +270,19 @@ __global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP
 }
 
 __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
-                                                    int32_t* __restrict__ start, int32_t* __restrict__ end) {
+                                                    int2* __restrict__ range) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i];
     if (k == 0u) return;
     const int c = (int)(k - 1u);
-    if (i == 0 || keys[i - 1] != k) start[c] = i;
-    if (i == n - 1 || keys[i + 1] != k) end[c] = i + 1;
+    const bool first = i == 0 || keys[i - 1] != k, last = i == n - 1 || keys[i + 1] != k;
+    if (first && last)
+        range[c] = make_int2(i, i + 1);  // a cell of one particle: one 8-byte store
+    else if (first)
+        range[c].x = i;
+    else if (last)
+        range[c].y = i + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -327,12 +332,9 @@ void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order
     hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, st, srcP, srcV, order, dstP, dstV, n);
 }
 
-void launch_cell_range(const uint32_t* keys, int n, int ncells, int32_t* start, int32_t* end,
-                       hipStream_t st) {
-    launch_fill_i32(start, ncells, -1, st);
-    launch_fill_i32(end, ncells, -1, st);
+void launch_cell_range(const uint32_t* keys, int n, int2* range, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, start, end);
+    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range);
 }
 
 }  // namespace sd

@@ -254,6 +254,42 @@ def test_contact_vs_oracle(gpu, oracle):
         assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
 
 
+def test_contact_table_reuse_vs_oracle(gpu, oracle):
+    """The contact cell table is never cleared: a second particle set in other
+    cells (every first-set cell now empty) must see none of the first set's
+    entries. Periodic box, sparse and dense regions, cap of 96 reached."""
+    _say("contact table reuse")
+    n, G = 6000, 16
+    rng = np.random.default_rng(8)
+    first = np.zeros((n, 4), np.float32)
+    first[:, :3] = rng.uniform(0.05, 0.45, (n, 3))
+    second = np.zeros((n, 4), np.float32)
+    second[: n // 2, :3] = rng.uniform(0.55, 0.75, (n // 2, 3))  # dense: cap reached
+    second[n // 2:, :3] = rng.uniform(0.5, 1.0, (n - n // 2, 3))
+    second[n // 2:, 0] = np.where(second[n // 2:, 0] > 0.99, 0.995, second[n // 2:, 0])  # near the wrap
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0, 0.01, (n, 3))
+    cfg = sd.default_config(particleCount=n, gridSize=G, boundary=sd.BOUNDARY_PERIODIC, softeningCells=1.0,
+                            gravity=0.05, sortInterval=1, contactEnabled=1, contactRadiusCells=0.3,
+                            contactStiffness=1e3)
+    s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
+                                     keyMode=sd.KEY_MORTON))
+    s.set_fixed_dt(1e-3)
+    s.set_particles(first, vel)
+    s.run(2)
+    s.set_particles(second, vel)
+    s.run(2)
+    gp, gv = s.get_particles()
+    s.close()
+    prm = oracle.make_params(G, periodic=True, gravity=0.05, softeningCells=1.0, sortInterval=1,
+                             keyMode=oracle.KEY_MORTON, contact=1, contactRadiusCells=0.3, contactStiffness=1e3,
+                             fixedDt=1e-3, allocN=n)
+    st = oracle.Stepper(prm, second, vel)
+    st.step(2)
+    assert np.abs(gp[:, :3] - st.pos[:, :3]).max() < 2e-6
+    assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
+
+
 def test_direct_sum_vs_fp64(gpu, oracle):
     n = 50000
     pos, vel = random_particles(n, seed=21)
