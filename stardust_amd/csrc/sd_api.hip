@@ -137,8 +137,9 @@ static void free_buffers(sd_sim* s) {
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
-    dfree(s->cellRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
+    dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
     s->cellTableCells = 0;
+    s->blockCells = 0;
     if (s->scHost) {
         (void)hipHostFree(s->scHost);
         s->scHost = nullptr;
@@ -293,6 +294,13 @@ static bool ensure_contact_buffers(sd_sim* s) {
     s->cellTableCells = (long long)cells;
     if (!s->accContact && !dalloc(s->accContact, (size_t)s->allocN)) return false;
     if (!s->cellRange && !dalloc(s->cellRange, cells)) return false;
+    if (s->modes.keyMode == SD_KEY_MORTON) {
+        // Morton cell codes run over (2^k)^3 >= G^3; a block is 64 consecutive codes
+        const long long blocks = (long long)(cells + 63) / 64;
+        if (s->blockCells < blocks) dfree(s->blockRange);
+        s->blockCells = blocks;
+        if (!s->blockRange && !dalloc(s->blockRange, (size_t)blocks)) return false;
+    }
     return true;
 }
 
@@ -429,8 +437,10 @@ static bool contact_pass(sd_sim* s) {
     if (!ensure_contact_buffers(s)) return false;
     sort_particles(s, true);
     TIMED(s, "contact", {
-        launch_cell_range(s->sb.keys, s->allocN, s->cellRange, s->st);
+        int2* blocks = s->modes.keyMode == SD_KEY_MORTON ? s->blockRange : nullptr;
+        launch_cell_range(s->sb.keys, s->allocN, s->cellRange, blocks, s->blockCells, s->st);
         launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellRange, s->sb.keys,
+                       blocks,
                        s->cfg.contactRadiusCells / (float)G, s->cfg.contactStiffness, s->cfg.contactDamping,
                        s->accContact, s->st);
     });
@@ -770,7 +780,7 @@ size_t sd_estimate_bytes(int particleCount, int gridSize, int boundary) {
     b += 8 * cells;                     // fixed-point density
     b += 8 * G * G * P + 8 * G * S * P; // half-spectrum work arrays
     b += 4 * cells + 16 * cells;        // pot, accG
-    b += 8 * cells;                     // cell ranges (contact)
+    b += 8 * cells + cells / 8;         // cell ranges and 4^3-block ranges (contact)
     if (boundary == SD_BOUNDARY_ISOLATED) {
         b += 4 * (S / 2 + 1) * (S / 2 + 1) * P;  // Green octant
         b += 2 * 8 * S * S * P;                  // transient Green build arrays

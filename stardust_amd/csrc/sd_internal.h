@@ -82,7 +82,12 @@ void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order
 // entry is current exactly when its first index f satisfies 0 <= f < n and
 // keys[f] == c + 1 (entries of cells emptied since an earlier call keep stale
 // values, so the table is never cleared).
-void launch_cell_range(const uint32_t* keys, int n, int2* range, hipStream_t st);
+// With blockRange (Morton keys: cell c = Morton(x, y, z), block c >> 6 = Morton
+// of the 4^3-cell block), blockRange[b] = (first, one past last) sorted index of
+// block b's particles, (0, 0) for an empty block: the table (blockEntries int2)
+// is zeroed here first.
+void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRange, long long blockEntries,
+                       hipStream_t st);
 
 // ---- deposit / integrate ----------------------------------------------------
 void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long long* rho,
@@ -160,7 +165,7 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
 
 // ---- contact ------------------------------------------------------------------
 void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int2* cellRange, const uint32_t* sortedKeys, float radius,
+                    const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
                     float stiffness, float damping, float4* accOut, hipStream_t st);
 
 // ---- direct sum -------------------------------------------------------------

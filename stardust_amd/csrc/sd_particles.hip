@@ -814,19 +814,73 @@ __device__ __forceinline__ void axis_gaps(float o, int c, int G, int periodic, f
     g[2] = (periodic || c < G - 1) ? hi * hi : -1e30f;
 }
 
+// Distinct blocks (at most 3) holding the cells c - 1, c, c + 1 along one axis.
+__device__ __forceinline__ int axis_blocks(int c, int G, int periodic, int b[3]) {
+    const int b0 = wrap_or_clamp(c - 1, G, periodic) >> 2, b1 = c >> 2, b2 = wrap_or_clamp(c + 1, G, periodic) >> 2;
+    int m = 0;
+    b[m++] = b0;
+    if (b1 != b0) b[m++] = b1;
+    if (b2 != b0 && b2 != b1) b[m++] = b2;
+    return m;
+}
+
+// One cell's candidates [s0, e0) in the reference's order (kContact, Sim.cu:1063-1100):
+// each counts toward the cap of 96 (self excluded), in range or not.
+__device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, const float4* __restrict__ vel, int i,
+                                             float4 p, float4 v, int s0, int e0, int periodic, float d0, float d02,
+                                             float stiffness, float damping, int& seen, float& ax, float& ay,
+                                             float& az) {
+    constexpr int kMaxPeers = 96;
+    for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
+        if (j == i) continue;
+        ++seen;
+        const float4 q = pos[j];
+        if (q.x < 0.f) continue;
+        float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
+        if (periodic) {
+            if (ex > 0.5f) ex -= 1.f; else if (ex < -0.5f) ex += 1.f;
+            if (ey > 0.5f) ey -= 1.f; else if (ey < -0.5f) ey += 1.f;
+            if (ez > 0.5f) ez -= 1.f; else if (ez < -0.5f) ez += 1.f;
+        }
+        const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+        if (dd >= d02 || dd < 1e-16f) continue;
+        const float d = sqrtf(dd);
+        const float nx = ex / d, ny = ey / d, nz = ez / d;
+        const float4 vj = vel[j];
+        const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
+        float f = 0.f;
+        if (d < d0) {
+            float fc = fmaf(stiffness, d0 - d, -(damping * vn));
+            if (fc < 0.f) fc = 0.f;
+            f += fc;
+        }
+        ax = fmaf(f, nx, ax);
+        ay = fmaf(f, ny, ay);
+        az = fmaf(f, nz, az);
+    }
+}
+
 // The reference visits the 27 cells around the particle's cell in z, y, x
 // order and stops after 96 candidates (self excluded; every candidate counts,
-// in range or not). Here the nine range entries of a z-plane load together,
-// and a cell whose nearest point lies beyond the contact diameter plus 1e-3
-// cell is not read: its particles would all fail the distance test, so it only
-// adds its count to the candidate tally (capped at 96 as the reference's loop
-// caps it). The 1e-3 cell margin is far above the float rounding of keys,
-// offsets and distances (about 3e-5 cell at G = 512), so forces and the
-// candidate cap are exactly those of the full visit.
+// in range or not). A cell whose nearest point lies beyond the contact
+// diameter plus 1e-3 cell holds only particles that fail the distance test;
+// the margin is far above the float rounding of keys, offsets and distances
+// (about 3e-5 cell at G = 512). Two visits give exactly the reference's forces:
+// * when the 4^3-cell blocks covering the 27 cells hold at most 97 particles
+//   (the particle itself and at most 96 candidates), the cap cannot be reached
+//   and out-of-reach cells are skipped outright (Morton keys only: a block is
+//   then one run of the sorted keys, blockRange[Morton(block)]);
+// * otherwise the nine range entries of a z-plane load together and an
+//   out-of-reach cell adds only its count to the tally (capped at 96 as the
+//   reference's loop caps it).
+// Isolated boxes take the second visit at their edge cells, where the clamped
+// neighbour folds onto the particle's own cell (periodic meshes below 3 cells
+// likewise).
 __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
                                                int G, int periodic, int keyMode, const int2* __restrict__ range,
-                                               const uint32_t* __restrict__ keys, float radius, float stiffness,
-                                               float damping, float4* __restrict__ accOut) {
+                                               const uint32_t* __restrict__ keys, const int2* __restrict__ blockRange,
+                                               float radius, float stiffness, float damping,
+                                               float4* __restrict__ accOut) {
     const int i = blockIdx.x * PT + threadIdx.x;
     if (i >= n) return;
     const float4 p = pos[i];
@@ -850,51 +904,53 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
     const float reach2 = reach * reach;
     const int kMaxPeers = 96;
     int seen = 0;
-    for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz) {
-        int cs[9];
-        int2 r[9];
+
+    bool sparse = blockRange != nullptr &&
+                  (periodic ? G >= 3 : (cx > 0 && cx < G - 1 && cy > 0 && cy < G - 1 && cz > 0 && cz < G - 1));
+    if (sparse) {
+        int bx[3], by[3], bz[3];
+        const int mx = axis_blocks(cx, G, periodic, bx), my = axis_blocks(cy, G, periodic, by),
+                  mz = axis_blocks(cz, G, periodic, bz);
+        int total = 0;
+        for (int c = 0; c < mz; ++c)
+            for (int b = 0; b < my; ++b)
+                for (int a = 0; a < mx; ++a) {
+                    const int2 r = blockRange[morton3((uint32_t)bx[a], (uint32_t)by[b], (uint32_t)bz[c])];
+                    total += r.y - r.x;
+                }
+        sparse = total <= kMaxPeers + 1;
+    }
+    if (sparse) {
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
+                    const int c = cell_slot(cx + dx, cy + dy, cz + dz, G, periodic, keyMode);
+                    const int2 r = range[c];
+                    if (r.x < 0 || r.x >= n || keys[r.x] != (uint32_t)c + 1u || r.y <= r.x) continue;
+                    contact_cell(pos, vel, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
+                }
+    } else {
+        for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz) {
+            int cs[9];
+            int2 r[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            cs[q] = cell_slot(cx + q % 3 - 1, cy + q / 3 - 1, cz + dz, G, periodic, keyMode);
-            r[q] = range[cs[q]];
-        }
-        uint32_t kf[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) kf[q] = (r[q].x >= 0 && r[q].x < n) ? keys[r[q].x] : 0u;
-        for (int q = 0; q < 9 && seen < kMaxPeers; ++q) {
-            if (kf[q] != (uint32_t)cs[q] + 1u) continue;  // empty cell (stale or unwritten entry)
-            const int s0 = r[q].x, e0 = r[q].y;
-            if (e0 <= s0) continue;
-            if (gx[q % 3] + gy[q / 3] + gz[dz + 1] > reach2) {
-                seen = min(seen + (e0 - s0), kMaxPeers);  // out of reach: the particle is not in this cell
-                continue;
+            for (int q = 0; q < 9; ++q) {
+                cs[q] = cell_slot(cx + q % 3 - 1, cy + q / 3 - 1, cz + dz, G, periodic, keyMode);
+                r[q] = range[cs[q]];
             }
-            for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
-                if (j == i) continue;
-                ++seen;
-                const float4 q4 = pos[j];
-                if (q4.x < 0.f) continue;
-                float ex = p.x - q4.x, ey = p.y - q4.y, ez = p.z - q4.z;
-                if (periodic) {
-                    if (ex > 0.5f) ex -= 1.f; else if (ex < -0.5f) ex += 1.f;
-                    if (ey > 0.5f) ey -= 1.f; else if (ey < -0.5f) ey += 1.f;
-                    if (ez > 0.5f) ez -= 1.f; else if (ez < -0.5f) ez += 1.f;
+            uint32_t kf[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) kf[q] = (r[q].x >= 0 && r[q].x < n) ? keys[r[q].x] : 0u;
+            for (int q = 0; q < 9 && seen < kMaxPeers; ++q) {
+                if (kf[q] != (uint32_t)cs[q] + 1u) continue;  // empty cell (stale or unwritten entry)
+                const int s0 = r[q].x, e0 = r[q].y;
+                if (e0 <= s0) continue;
+                if (gx[q % 3] + gy[q / 3] + gz[dz + 1] > reach2) {
+                    seen = min(seen + (e0 - s0), kMaxPeers);  // out of reach: the particle is not in this cell
+                    continue;
                 }
-                const float dd = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                if (dd >= d02 || dd < 1e-16f) continue;
-                const float d = sqrtf(dd);
-                const float nx = ex / d, ny = ey / d, nz = ez / d;
-                const float4 vj = vel[j];
-                const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
-                float f = 0.f;
-                if (d < d0) {
-                    float fc = fmaf(stiffness, d0 - d, -(damping * vn));
-                    if (fc < 0.f) fc = 0.f;
-                    f += fc;
-                }
-                ax = fmaf(f, nx, ax);
-                ay = fmaf(f, ny, ay);
-                az = fmaf(f, nz, az);
+                contact_cell(pos, vel, i, p, v, s0, e0, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
             }
         }
     }
@@ -902,11 +958,11 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
 }
 
 void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int2* cellRange, const uint32_t* sortedKeys, float radius, float stiffness, float damping,
-                    float4* accOut, hipStream_t st) {
+                    const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
+                    float stiffness, float damping, float4* accOut, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_contact, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
-                       cellRange, sortedKeys, radius, stiffness, damping, accOut);
+                       cellRange, sortedKeys, blockRange, radius, stiffness, damping, accOut);
 }
 
 // ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------

@@ -67,6 +67,8 @@ struct sd_sim {
     float2* tw = nullptr;
     float4* accContact = nullptr;
     int2* cellRange = nullptr;  // contact cell table: (first, one past last) sorted index per cell
+    int2* blockRange = nullptr;  // contact (Morton keys): sorted index range per 4^3-cell block
+    long long blockCells = 0;
     long long cellTableCells = 0;
     sd::MeshPlan plan{};
 

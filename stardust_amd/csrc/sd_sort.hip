@@ -17,13 +17,18 @@
 //   scan    : exclusive scan of the digit-major table (reduce / top / down),
 //   scatter : stable in-tile ranks (same ballot match, running per-digit
 //             counters across rounds and waves) -> global positions.
+#include <cmath>
+
 #include "sd_device.h"
 #include "sd_internal.h"
 
 namespace sd {
 
 constexpr int SORT_THREADS = 256;
-constexpr int SORT_ROUNDS = 16;  // keys per thread per tile
+#ifndef SD_SORT_ROUNDS
+#define SD_SORT_ROUNDS 16
+#endif
+constexpr int SORT_ROUNDS = SD_SORT_ROUNDS;  // keys per thread per tile
 constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;
 constexpr int SCAN_CHUNK = 1024;  // hist entries per scan block
 constexpr int WAVES = SORT_THREADS / 64;
@@ -203,6 +208,18 @@ __global__ void __launch_bounds__(256) k_scan_down(uint32_t* __restrict__ h, int
         if (base + k < total) h[base + k] = v4[k] + c;
 }
 
+#ifndef SD_SORT_LOCAL
+#define SD_SORT_LOCAL 1
+#endif
+
+// Scatter of one tile. Every key's rank within its digit in the tile comes
+// from the ballot match and running per-digit counters. LOCAL first orders the
+// tile by digit in LDS (the tile's digit counts are the differences of
+// consecutive entries of the digit-major exclusive scan), so that each digit's
+// keys leave as one contiguous run instead of one store per key per round:
+// that pays when a round's 256 keys spread over many digits (sparse keys), and
+// costs LDS traffic and occupancy when equal digits already come in long runs.
+template <bool LOCAL>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* __restrict__ keys,
                                                                const int32_t* __restrict__ vals, int n, int shift,
                                                                const uint32_t* __restrict__ offs, int ntiles,
@@ -213,10 +230,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
     __shared__ uint32_t s_wcnt[WAVES][256];
     __shared__ uint32_t s_wbase[WAVES][256];
     const int t = threadIdx.x;
-    s_off[t] = offs[(size_t)t * ntiles + blockIdx.x];
+    const size_t flat = (size_t)t * ntiles + blockIdx.x;
+    s_off[t] = offs[flat];
     s_run[t] = 0;
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) s_wcnt[w][t] = 0;
+    __shared__ uint32_t s_start[LOCAL ? 256 : 1];  // first tile slot of each digit
+    __shared__ uint32_t s_lk[LOCAL ? SORT_TILE : 1];
+    __shared__ int32_t s_lv[LOCAL ? SORT_TILE : 1];
+    if constexpr (LOCAL) {
+        __shared__ uint32_t s_wave[WAVES];
+        const uint32_t next = flat + 1 < (size_t)256 * ntiles ? offs[flat + 1] : (uint32_t)n;
+        uint32_t v4[4] = {next - s_off[t], 0u, 0u, 0u};
+        block_scan_excl(v4, s_wave);
+        s_start[t] = v4[0];
+    }
     __syncthreads();
     const int base = blockIdx.x * SORT_TILE;
     const int lane = t & 63, wv = t >> 6;
@@ -250,12 +278,29 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
         }
         __syncthreads();
         if (valid) {
-            const uint32_t dst = s_off[d] + s_wbase[wv][d] + rank;
-            keysOut[dst] = key;
-            valsOut[dst] = val;
+            if constexpr (LOCAL) {
+                const uint32_t slot = s_start[d] + s_wbase[wv][d] + rank;
+                s_lk[slot] = key;
+                s_lv[slot] = val;
+            } else {
+                const uint32_t dst = s_off[d] + s_wbase[wv][d] + rank;
+                keysOut[dst] = key;
+                valsOut[dst] = val;
+            }
         }
         // s_wbase is rewritten next round only after the barrier that follows
         // the next round's counting, so no extra barrier is needed here.
+    }
+    if constexpr (LOCAL) {
+        __syncthreads();
+        const int inTile = min(SORT_TILE, n - base);
+        for (int e = t; e < inTile; e += SORT_THREADS) {
+            const uint32_t key = s_lk[e];
+            const uint32_t d = (key >> shift) & 255u;
+            const uint32_t dst = s_off[d] + (uint32_t)e - s_start[d];
+            keysOut[dst] = key;
+            valsOut[dst] = s_lv[e];
+        }
     }
 }
 
@@ -270,7 +315,7 @@ __global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP
 }
 
 __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
-                                                    int2* __restrict__ range) {
+                                                    int2* __restrict__ range, int2* __restrict__ blockRange) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i];
@@ -283,6 +328,18 @@ __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__
         range[c].x = i;
     else if (last)
         range[c].y = i + 1;
+    if (blockRange) {
+        // keys ascend, so keys[i + 1] >= k > 0; an inactive predecessor (key 0) starts a run
+        const uint32_t b = (uint32_t)c >> 6;
+        const bool bfirst = i == 0 || keys[i - 1] == 0u || ((keys[i - 1] - 1u) >> 6) != b;
+        const bool blast = i == n - 1 || ((keys[i + 1] - 1u) >> 6) != b;
+        if (bfirst && blast)
+            blockRange[b] = make_int2(i, i + 1);
+        else if (bfirst)
+            blockRange[b].x = i;
+        else if (blast)
+            blockRange[b].y = i + 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -309,14 +366,25 @@ void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
     const int ntiles = sort_tiles_for(n);
     const int total = 256 * ntiles;
     const int nchunks = (total + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    // Keys sparse in their code space (under one key per two codes) spread each
+    // round over many digits: order the tile in LDS first, except in the last
+    // pass, whose few digit values come in long runs. (G=512 cosmic web, 16.7 M
+    // keys of 27 bits: 214/342/365/86 us per pass direct, 136/133/128/104
+    // ordered; two-disk G=128, 4.2 M keys of 21 bits: 43/24/23 direct,
+    // 38/34/33 ordered.)
+    const bool sparse = SD_SORT_LOCAL && 2.0 * (double)n < std::ldexp(1.0, bits);
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
                            buf.hist, ntiles);
         hipLaunchKernelGGL(k_scan_reduce, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, buf.chunk, nchunks);
         hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
-                           shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
+        if (sparse && shift + 8 < bits)
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
+                               shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
+        else
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
+                               shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
         uint32_t* tk = buf.keys;
         buf.keys = buf.keysAlt;
         buf.keysAlt = tk;
@@ -332,9 +400,11 @@ void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order
     hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, st, srcP, srcV, order, dstP, dstV, n);
 }
 
-void launch_cell_range(const uint32_t* keys, int n, int2* range, hipStream_t st) {
+void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRange, long long blockEntries,
+                       hipStream_t st) {
+    if (blockRange) launch_fill_i32(reinterpret_cast<int32_t*>(blockRange), 2 * blockEntries, 0, st);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range);
+    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range, blockRange);
 }
 
 }  // namespace sd
