@@ -83,7 +83,7 @@ static bool dalloc(T*& p, size_t count) {
     return true;
 }
 
-static void timer_begin(sd_sim* s, const char* name) {
+static void timer_begin(sd_sim* s, const char* name, hipStream_t st) {
     KernelTimer& t = s->timer;
     t.armed = false;
     if (t.name.empty() || t.name != name) return;
@@ -100,13 +100,13 @@ static void timer_begin(sd_sim* s, const char* name) {
         }
         t.used = 0;
     }
-    HK(hipEventRecord(t.pool[t.used], s->st));
+    HK(hipEventRecord(t.pool[t.used], st));
 }
-static void timer_end(sd_sim* s, const char* name) {
+static void timer_end(sd_sim* s, const char* name, hipStream_t st) {
     KernelTimer& t = s->timer;
     if (!t.armed || t.name.empty() || t.name != name) return;
     t.armed = false;
-    HK(hipEventRecord(t.pool[t.used + 1], s->st));
+    HK(hipEventRecord(t.pool[t.used + 1], st));
     t.used += 2;
 }
 static void timer_harvest(sd_sim* s) {
@@ -122,12 +122,15 @@ static void timer_harvest(sd_sim* s) {
     t.used = 0;
 }
 
-#define TIMED(s, name, stmt)     \
-    do {                         \
-        timer_begin(s, name);    \
-        stmt;                    \
-        timer_end(s, name);      \
+// Work of one timed phase on stream st (the library's stream, or the mesh
+// stream of an overlapped step, which always joins the library's stream).
+#define TIMED_ON(s, st, name, stmt)  \
+    do {                             \
+        timer_begin(s, name, st);    \
+        stmt;                        \
+        timer_end(s, name, st);      \
     } while (0)
+#define TIMED(s, name, stmt) TIMED_ON(s, (s)->st, name, stmt)
 
 static void free_buffers(sd_sim* s) {
     if (!s->borrowed) {
@@ -399,20 +402,22 @@ __global__ void k_float_to_fix(const float* __restrict__ f, long long* __restric
 }
 
 // solvePoisson + kGridAccel (Sim.cu:3586-3611): consumes (and clears) rhoFix.
-static void solve_mesh(sd_sim* s) {
+// Density -> potential -> acceleration grid (and a deferred finalize) on st:
+// the library's stream, or the mesh stream of an overlapped step.
+static void solve_mesh(sd_sim* s, hipStream_t st) {
     const long long cells = (long long)s->G * s->G * s->G;
     if (s->modes.depositMode == SD_DEPOSIT_FLOAT) {
-        hipLaunchKernelGGL(k_float_to_fix, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->st, s->rhoF,
+        hipLaunchKernelGGL(k_float_to_fix, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, s->rhoF,
                            s->rhoFix, cells);
     }
-    TIMED(s, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, s->st));
-    TIMED(s, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, s->st));
+    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, st));
+    TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, st));
     const float potScale = s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1);
     if (s->finDeferred) {
-        TIMED(s, "grad", launch_grid_accel_fin(s->pot, s->accG, s->G, potScale, s->periodic, s->finJob, s->st));
+        TIMED_ON(s, st, "grad", launch_grid_accel_fin(s->pot, s->accG, s->G, potScale, s->periodic, s->finJob, st));
         s->finDeferred = false;
     } else {
-        TIMED(s, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, s->st));
+        TIMED_ON(s, st, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, st));
     }
     s->rhoValid = false;
     s->meshValid = true;
@@ -426,7 +431,7 @@ static void ensure_rho(sd_sim* s) {
 static void ensure_mesh(sd_sim* s) {
     if (s->meshValid) return;
     ensure_rho(s);
-    solve_mesh(s);
+    solve_mesh(s, s->st);
 }
 
 // doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685)
@@ -535,6 +540,10 @@ static void flush_finalize(sd_sim* s) {
     s->finDeferred = false;
 }
 
+#ifndef SD_OVERLAP_MESH
+#define SD_OVERLAP_MESH 1
+#endif
+
 // last: the final step of a run, whose finalize is launched at once.
 static void enqueue_step(sd_sim* s, bool timed, bool last) {
     const bool fused = s->modes.pipeline == SD_PIPELINE_FUSED;
@@ -549,6 +558,20 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     // first deposit from unsorted positions would contend on the same cells.)
     const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
                              s->modes.depositMode == SD_DEPOSIT_FIXED;
+    // The mesh reads only the deposited density and the sort / contact chain
+    // only the particles, so on such steps the mesh runs on a second stream
+    // beside them and joins before the integrate. (Not with a learned
+    // correction, which reads the density that the mesh's first pass clears,
+    // nor while the density still has to be deposited from the particles.)
+    const bool contactDue = s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)s->G * s->G * s->G;
+    const bool overlap = SD_OVERLAP_MESH && fused && (sortDue || contactDue) && !s->mlp && !s->borrowed &&
+                         s->rhoValid && !s->meshValid && s->modes.depositMode == SD_DEPOSIT_FIXED && s->st2;
+    if (overlap) {
+        HK(hipEventRecord(s->evFork, s->st));
+        HK(hipStreamWaitEvent(s->st2, s->evFork, 0));
+        solve_mesh(s, s->st2);
+        HK(hipEventRecord(s->evJoin, s->st2));
+    }
     if (sortDue) {
         if (fuseReorder)
             sort_keys(s);
@@ -569,9 +592,10 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         ensure_mesh(s);
     } else {
         ensure_rho(s);
-        solve_mesh(s);
+        solve_mesh(s, s->st);
     }
     const bool contact = contact_pass(s);
+    if (overlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
     if (timed) HK(hipEventRecord(s->ev1, s->st));
     flush_finalize(s);  // the scalars must be final before this integrate reads them
 
@@ -841,6 +865,9 @@ void sd_destroy(sd_sim* s) {
     if (s->evA) (void)hipEventDestroy(s->evA);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->evB) (void)hipEventDestroy(s->evB);
+    if (s->evFork) (void)hipEventDestroy(s->evFork);
+    if (s->evJoin) (void)hipEventDestroy(s->evJoin);
+    if (s->st2) (void)hipStreamDestroy(s->st2);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -891,6 +918,10 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
         HK(hipEventCreate(&s->evA));
         HK(hipEventCreate(&s->ev1));
         HK(hipEventCreate(&s->evB));
+        // the mesh stream of overlapped steps (default priority, like the library's stream)
+        HK(hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking));
+        HK(hipEventCreateWithFlags(&s->evFork, hipEventDisableTiming));
+        HK(hipEventCreateWithFlags(&s->evJoin, hipEventDisableTiming));
         if (g_failed) return SD_ERR_DEVICE;
     }
     const bool structural = !s->ready || G != s->G || c->boundary != (s->periodic ? 1 : 0) ||

@@ -46,6 +46,8 @@ struct sd_sim {
     int active = 0;  // slots in use (prefix), for state files
     float fixedDt = 0.f;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;  // mesh stream of overlapped steps (joins st within the step)
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
 
     // particles (borrowed: the four arrays are slices of a batch's storage,
     // which owns them; their addresses never change)
