@@ -144,6 +144,47 @@ __device__ __forceinline__ uint32_t dpp_src(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, true);
 }
 
+// Wavefront reductions on DPP (VALU, no LDS permute): quad_perm xor 1 and
+// xor 2, row_half_mirror and row_mirror leave every lane of a 16-lane row with
+// the row's result; row_bcast 15 (into rows 1, 3) and row_bcast 31 (into rows
+// 2, 3) then leave the wavefront's result in lane 63 only. Lanes of rows a
+// broadcast does not write take the identity.
+#ifndef SD_DPP_REDUCE
+#define SD_DPP_REDUCE 1
+#endif
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_or(int identity, int v) {
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xF, false);
+}
+#define SD_WAVE63(STEP) STEP(0xB1, 0xF) STEP(0x4E, 0xF) STEP(0x141, 0xF) STEP(0x140, 0xF) STEP(0x142, 0xA) STEP(0x143, 0xC)
+__device__ __forceinline__ int wave_min63(int v) {
+#define SD_STEP(C, M) v = min(v, dpp_or<C, M>(INT_MAX, v));
+    SD_WAVE63(SD_STEP)
+#undef SD_STEP
+    return v;
+}
+__device__ __forceinline__ int wave_max63(int v) {
+#define SD_STEP(C, M) v = max(v, dpp_or<C, M>(INT_MIN, v));
+    SD_WAVE63(SD_STEP)
+#undef SD_STEP
+    return v;
+}
+__device__ __forceinline__ double wave_sum63(double v) {
+#define SD_STEP(C, M)                                                                              \
+    v += __hiloint2double(dpp_or<C, M>(0, __double2hiint(v)), dpp_or<C, M>(0, __double2loint(v)));
+    SD_WAVE63(SD_STEP)
+#undef SD_STEP
+    return v;
+}
+__device__ __forceinline__ float wave_max63(float v) {  // v >= 0
+#define SD_STEP(C, M) v = fmaxf(v, __int_as_float(dpp_or<C, M>(0, __float_as_int(v))));
+    SD_WAVE63(SD_STEP)
+#undef SD_STEP
+    return v;
+}
+// The previous lane's value (DPP wave_shr 1; lane 0 gets `first`).
+__device__ __forceinline__ int lane_prev(int first, int v) { return dpp_or<0x138, 0xF>(first, v); }
+
 // Segmented inclusive scan over the wavefront; returns true on the lane that
 // holds a run's total in v[8]. A run lies inside one wavefront, so its sums
 // are <= 64 * 2^24 = 2^30 and fit 32 bits. The scan is Hillis-Steele on DPP:
@@ -159,7 +200,11 @@ __device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, uint32_t v[8
         for (int k = 0; k < 8; ++k) v[k] = 0u;
     }
     const int ix = alive ? c.ix : INT_MIN, iy = alive ? c.iy : INT_MIN, iz = alive ? c.iz : INT_MIN;
+#if SD_DPP_REDUCE
+    const int pix = lane_prev(ix, ix), piy = lane_prev(iy, iy), piz = lane_prev(iz, iz);
+#else
     const int pix = __shfl_up(ix, 1, 64), piy = __shfl_up(iy, 1, 64), piz = __shfl_up(iz, 1, 64);
+#endif
     const bool head = lane == 0 || !alive || pix != ix || piy != iy || piz != iz;
     const uint64_t headMask = __ballot(head);
     const uint64_t le = lane == 63 ? headMask : (headMask & ((2ull << lane) - 1ull));
@@ -217,6 +262,14 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
             lo[2] = min(lo[2], c[k].iz); hi[2] = max(hi[2], c[k].iz);
         }
     }
+#if SD_DPP_REDUCE
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_min63(lo[d]);
+        hi[d] = wave_max63(hi[d]);
+    }
+    constexpr int LEAD = 63;
+#else
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -225,8 +278,10 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
             hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
         }
     }
+    constexpr int LEAD = 0;
+#endif
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == LEAD) {
         for (int d = 0; d < 3; ++d) {
             s_box[w][d] = lo[d];
             s_box[w][3 + d] = hi[d];
@@ -343,13 +398,23 @@ __device__ __forceinline__ void block_partial(double sx, double sy, double sz, d
                                               ReducePartial* out, int blk) {
     __shared__ double s_d[PT / 64][4];
     __shared__ float s_f[PT / 64];
+#if SD_DPP_REDUCE
+    sx = wave_sum63(sx);
+    sy = wave_sum63(sy);
+    sz = wave_sum63(sz);
+    cnt = wave_sum63(cnt);
+    vmax = wave_max63(vmax);
+    constexpr int LEAD = 63;
+#else
     sx = wave_sum(sx);
     sy = wave_sum(sy);
     sz = wave_sum(sz);
     cnt = wave_sum(cnt);
     vmax = wave_max(vmax);
+    constexpr int LEAD = 0;
+#endif
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
+    if (lane == LEAD) {
         s_d[w][0] = sx;
         s_d[w][1] = sy;
         s_d[w][2] = sz;
