@@ -304,49 +304,88 @@ struct RowsFwdArgs {
     float soft;    // Green: softening (cells)
 };
 
+// Rows per LDS line in the x passes. With SD_FFT_ROWS2 two real rows share one
+// complex transform (row 2c in the real part, row 2c + 1 in the imaginary
+// part); the forward pass splits the two half spectra by Hermitian symmetry,
+// A = (Z[k] + conj Z[N-k]) / 2, B = -i (Z[k] - conj Z[N-k]) / 2, and the
+// inverse pass transforms Z = A + i B (bins 0 and N/2 taken real) and keeps
+// the real part for one row and the imaginary part for the other. A block
+// Lines per block and rows per line are set per direction (S=256, two-disk
+// step: forward 1 row x 16 lines 14.5 us, 2 x 16: 15.9, 2 x 8: 16.9 -- the
+// split reads Z[N-k] across the line; inverse 1 x 16: 11.6 us, 2 x 8: 10.2,
+// 2 x 16: 9.5).
+#ifndef SD_FFT_RPL_FWD
+#define SD_FFT_RPL_FWD 1
+#endif
+#ifndef SD_FFT_LINES_FWD
+#define SD_FFT_LINES_FWD 16
+#endif
+#ifndef SD_FFT_RPL_INV
+#define SD_FFT_RPL_INV 2
+#endif
+#ifndef SD_FFT_LINES_INV
+#define SD_FFT_LINES_INV 16
+#endif
+template <int RPL_, int LINES_>
+struct XPass {
+    static constexpr int RPL = RPL_;      // rows per LDS line
+    static constexpr int LINES = LINES_;  // lines per block
+    static constexpr int ROWS = RPL * LINES;
+};
+using XFwd = XPass<SD_FFT_RPL_FWD, SD_FFT_LINES_FWD>;
+using XInv = XPass<SD_FFT_RPL_INV, SD_FFT_LINES_INV>;
+
 template <int N, int SRC>
 __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2* __restrict__ tw, int bx) {
     constexpr int LP = Line<N>::LP;
-    constexpr int THREADS = TB * Plan<N>::T;
-    __shared__ float2 s_line[TB * LP];
+    constexpr int RPL = XFwd::RPL, TBR = XFwd::LINES, TB = XFwd::ROWS;  // rows per line, lines, rows
+    constexpr int THREADS = TBR * Plan<N>::T;
+    __shared__ float2 s_line[TBR * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
+    float* s_f = reinterpret_cast<float*>(s_line);  // component h of slot e: s_f[2 e + h]
     const int row0 = bx * TB;
     // zero the padded half of every line
-    for (int i = threadIdx.x; i < TB * N; i += THREADS) {
+    for (int i = threadIdx.x; i < TBR * N; i += THREADS) {
         const int c = i / N, x = i - c * N;
         if (x >= a.nin) s_line[c * LP + PADI(x)] = make_float2(0.f, 0.f);
     }
     if (SRC == 0) {
-        constexpr int ITERS = (TB * (N / 2) + THREADS - 1) / THREADS;  // int64 pairs
+        constexpr int ITERS = (TB * (N / 2) + THREADS - 1) / THREADS;  // int64 pairs of the TB rows
         const int pairs = a.nin / 2;
         longlong2 buf[ITERS];
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int q = threadIdx.x + it * THREADS;
-            const int c = q / pairs, j = q - c * pairs;
+            const int r = q / pairs, j = q - r * pairs;
             buf[it] = make_longlong2(0, 0);
-            if (c < TB && row0 + c < a.rows) {
-                longlong2* p = reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + c) * a.nin) + j;
+            if (r < TB && row0 + r < a.rows) {
+                longlong2* p = reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + r) * a.nin) + j;
                 buf[it] = *p;
             }
         }
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int q = threadIdx.x + it * THREADS;
-            const int c = q / pairs, j = q - c * pairs;
-            if (c < TB) {
+            const int r = q / pairs, j = q - r * pairs;
+            if (r < TB) {
                 const float sc = 1.0f / (float)SD_FIX_ONE;
-                s_line[c * LP + PADI(2 * j)] = make_float2((float)buf[it].x * sc, 0.f);
-                s_line[c * LP + PADI(2 * j + 1)] = make_float2((float)buf[it].y * sc, 0.f);
-                if (a.clearSrc && row0 + c < a.rows)
-                    reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + c) * a.nin)[j] = make_longlong2(0, 0);
+                const int c = r / RPL, h = r % RPL;
+                if constexpr (RPL == 1) {
+                    s_line[c * LP + PADI(2 * j)] = make_float2((float)buf[it].x * sc, 0.f);
+                    s_line[c * LP + PADI(2 * j + 1)] = make_float2((float)buf[it].y * sc, 0.f);
+                } else {
+                    s_f[2 * (c * LP + PADI(2 * j)) + h] = (float)buf[it].x * sc;
+                    s_f[2 * (c * LP + PADI(2 * j + 1)) + h] = (float)buf[it].y * sc;
+                }
+                if (a.clearSrc && row0 + r < a.rows)
+                    reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + r) * a.nin)[j] = make_longlong2(0, 0);
             }
         }
     } else {
         for (int i = threadIdx.x; i < TB * a.nin; i += THREADS) {
-            const int c = i / a.nin, x = i - c * a.nin;
-            const int row = row0 + c;
+            const int r = i / a.nin, x = i - r * a.nin;
+            const int row = row0 + r;
             float val = 0.f;
             if (row < a.rows) {
                 const int y = row % a.S, z = row / a.S;
@@ -354,10 +393,14 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
                 const int dx = x <= h ? x : S - x;
                 const int dy = y <= h ? y : S - y;
                 const int dz = z <= h ? z : S - z;
-                const float r = sqrtf((float)(dx * dx + dy * dy + dz * dz)) + a.soft;
-                val = -1.0f / r;
+                const float rr = sqrtf((float)(dx * dx + dy * dy + dz * dz)) + a.soft;
+                val = -1.0f / rr;
             }
-            s_line[c * LP + PADI(x)] = make_float2(val, 0.f);
+            const int c = r / RPL;
+            if constexpr (RPL == 1)
+                s_line[c * LP + PADI(x)] = make_float2(val, 0.f);
+            else
+                s_f[2 * (c * LP + PADI(x)) + (r % RPL)] = val;
         }
     }
     __syncthreads();
@@ -366,27 +409,46 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
     // store k = 0 .. N/2 as complex pairs (the pair's second half at N/2 + 1
     // lands in the row padding)
     constexpr int KP2 = N / 4 + 1;
-    constexpr int OITERS = (TB * KP2 + THREADS - 1) / THREADS;
+    constexpr int OITERS = (TBR * KP2 + THREADS - 1) / THREADS;
 #pragma unroll
     for (int it = 0; it < OITERS; ++it) {
         const int q = threadIdx.x + it * THREADS;
         const int cc = q / KP2, j = q - cc * KP2;
-        if (cc < TB && row0 + cc < a.rows) {
-            const float2 u = s_line[cc * LP + PADI(2 * j)], v = s_line[cc * LP + PADI(2 * j + 1)];
-            reinterpret_cast<float4*>(a.out + (size_t)(row0 + cc) * a.P)[j] = make_float4(u.x, u.y, v.x, v.y);
+        if (cc >= TBR) continue;
+        const float2* l = s_line + cc * LP;
+        if constexpr (RPL == 1) {
+            if (row0 + cc < a.rows) {
+                const float2 u = l[PADI(2 * j)], v = l[PADI(2 * j + 1)];
+                reinterpret_cast<float4*>(a.out + (size_t)(row0 + cc) * a.P)[j] = make_float4(u.x, u.y, v.x, v.y);
+            }
+        } else {
+            float2 A[2], B[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int k = 2 * j + e;
+                const float2 z = l[PADI(k)], m = l[PADI(k == 0 ? 0 : N - k)];  // k <= N/2 + 1
+                A[e] = make_float2(0.5f * (z.x + m.x), 0.5f * (z.y - m.y));
+                B[e] = make_float2(0.5f * (z.y + m.y), 0.5f * (m.x - z.x));
+            }
+            const int ra = row0 + 2 * cc;
+            if (ra < a.rows)
+                reinterpret_cast<float4*>(a.out + (size_t)ra * a.P)[j] = make_float4(A[0].x, A[0].y, A[1].x, A[1].y);
+            if (ra + 1 < a.rows)
+                reinterpret_cast<float4*>(a.out + (size_t)(ra + 1) * a.P)[j] =
+                    make_float4(B[0].x, B[0].y, B[1].x, B[1].y);
         }
     }
 }
 
 template <int N, int SRC>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(XFwd::LINES * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
     rows_fwd_body<N, SRC>(a, tw, blockIdx.x);
 }
 
 // One launch over the X pass of many meshes of the same FFT side: blockIdx.y
 // selects the mesh (every job has the same row count).
 template <int N>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_fwd_batch(const RowsFwdArgs* __restrict__ jobs,
+__global__ void __launch_bounds__(XFwd::LINES * Plan<N>::T) k_rows_fwd_batch(const RowsFwdArgs* __restrict__ jobs,
                                                                     const float2* __restrict__ tw) {
     const RowsFwdArgs a = jobs[blockIdx.y];
     rows_fwd_body<N, 0>(a, tw, blockIdx.x);
@@ -404,34 +466,45 @@ struct RowsInvArgs {
 template <int N>
 __device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2* __restrict__ tw, int bx) {
     constexpr int LP = Line<N>::LP;
-    constexpr int THREADS = TB * Plan<N>::T;
-    __shared__ float2 s_line[TB * LP];
+    constexpr int RPL = XInv::RPL, TBR = XInv::LINES, TB = XInv::ROWS;  // rows per line, lines, rows
+    constexpr int THREADS = TBR * Plan<N>::T;
+    __shared__ float2 s_line[TBR * LP];
     __shared__ float2 s_tw[N];
     load_twiddles<N>(s_tw, tw);
     const int row0 = bx * TB;
     constexpr int KP2 = N / 4 + 1;
-    constexpr int ITERS = (TB * KP2 + THREADS - 1) / THREADS;
-    float4 buf[ITERS];
+    constexpr int ITERS = (TBR * KP2 + THREADS - 1) / THREADS;
+    float4 buf[ITERS][RPL];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int q = threadIdx.x + it * THREADS;
         const int c = q / KP2, j = q - c * KP2;
-        buf[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (c < TB && row0 + c < a.rows)
-            buf[it] = reinterpret_cast<const float4*>(a.in + (size_t)(row0 + c) * a.P)[j];
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            const int row = row0 + RPL * c + h;
+            buf[it][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (c < TBR && row < a.rows) buf[it][h] = reinterpret_cast<const float4*>(a.in + (size_t)row * a.P)[j];
+        }
     }
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int q = threadIdx.x + it * THREADS;
         const int c = q / KP2, j = q - c * KP2;
-        if (c < TB) {
+        if (c < TBR) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int k = 2 * j + h;
+            for (int e = 0; e < 2; ++e) {
+                const int k = 2 * j + e;
                 if (k > N / 2) continue;
-                const float2 v = h ? make_float2(buf[it].z, buf[it].w) : make_float2(buf[it].x, buf[it].y);
-                s_line[c * LP + PADI(k)] = v;
-                if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(v.x, -v.y);
+                float2 A = e ? make_float2(buf[it][0].z, buf[it][0].w) : make_float2(buf[it][0].x, buf[it][0].y);
+                if constexpr (RPL == 1) {
+                    s_line[c * LP + PADI(k)] = A;
+                    if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(A.x, -A.y);
+                } else {
+                    float2 B = e ? make_float2(buf[it][1].z, buf[it][1].w) : make_float2(buf[it][1].x, buf[it][1].y);
+                    if (k == 0 || k == N / 2) A.y = B.y = 0.f;  // the real bins of a real row
+                    s_line[c * LP + PADI(k)] = make_float2(A.x - B.y, A.y + B.x);
+                    if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(A.x + B.y, B.x - A.y);
+                }
             }
         }
     }
@@ -440,23 +513,23 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2
     Plan<N>::template run<true>(s_line + c * LP, s_tw, t);
     const int q4 = a.nout / 4;
     for (int i = threadIdx.x; i < TB * q4; i += THREADS) {
-        const int cc = i / q4, j = i - cc * q4;
-        const int row = row0 + cc;
+        const int r = i / q4, j = i - r * q4;
+        const int row = row0 + r;
         if (row < a.rows) {
-            const float2* l = s_line + cc * LP;
+            const float* l = reinterpret_cast<const float*>(s_line + (r / RPL) * LP) + (r % RPL);
             reinterpret_cast<float4*>(a.out + (size_t)row * a.nout)[j] =
-                make_float4(l[PADI(4 * j)].x, l[PADI(4 * j + 1)].x, l[PADI(4 * j + 2)].x, l[PADI(4 * j + 3)].x);
+                make_float4(l[2 * PADI(4 * j)], l[2 * PADI(4 * j + 1)], l[2 * PADI(4 * j + 2)], l[2 * PADI(4 * j + 3)]);
         }
     }
 }
 
 template <int N>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(XInv::LINES * Plan<N>::T) k_rows_inv(RowsInvArgs a, const float2* __restrict__ tw) {
     rows_inv_body<N>(a, tw, blockIdx.x);
 }
 
 template <int N>
-__global__ void __launch_bounds__(TB * Plan<N>::T) k_rows_inv_batch(const RowsInvArgs* __restrict__ jobs,
+__global__ void __launch_bounds__(XInv::LINES * Plan<N>::T) k_rows_inv_batch(const RowsInvArgs* __restrict__ jobs,
                                                                     const float2* __restrict__ tw) {
     const RowsInvArgs a = jobs[blockIdx.y];
     rows_inv_body<N>(a, tw, blockIdx.x);
@@ -777,7 +850,7 @@ namespace sd {
 
 template <int N>
 static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hipStream_t st) {
-    dim3 grid((a.rows + TB - 1) / TB), block(TB * Plan<N>::T);
+    dim3 grid((a.rows + XFwd::ROWS - 1) / XFwd::ROWS), block(XFwd::LINES * Plan<N>::T);
     if (src == 0)
         hipLaunchKernelGGL((k_rows_fwd<N, 0>), grid, block, 0, st, a, tw);
     else
@@ -785,7 +858,7 @@ static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hip
 }
 template <int N>
 static void launch_rows_inv(const RowsInvArgs& a, const float2* tw, hipStream_t st) {
-    dim3 grid((a.rows + TB - 1) / TB), block(TB * Plan<N>::T);
+    dim3 grid((a.rows + XInv::ROWS - 1) / XInv::ROWS), block(XInv::LINES * Plan<N>::T);
     hipLaunchKernelGGL((k_rows_inv<N>), grid, block, 0, st, a, tw);
 }
 // Resident blocks of a kernel on the whole device.
@@ -1054,16 +1127,17 @@ void mesh_group_destroy(MeshGroup* g) {
 
 template <int N>
 static void group_solve(const MeshGroup& g, hipStream_t st) {
-    const dim3 blk(TB * Plan<N>::T);
-    const int NK = g.S / 2 + 1, rowBlocks = (g.G * g.G + TB - 1) / TB, colBlocks = (NK + TB - 1) / TB;
-    hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(rowBlocks, g.n), blk, 0, st, g.xf, g.tw);
+    const dim3 blk(TB * Plan<N>::T), fblk(XFwd::LINES * Plan<N>::T), iblk(XInv::LINES * Plan<N>::T);
+    const int NK = g.S / 2 + 1, colBlocks = (NK + TB - 1) / TB;
+    const int fBlocks = (g.G * g.G + XFwd::ROWS - 1) / XFwd::ROWS, iBlocks = (g.G * g.G + XInv::ROWS - 1) / XInv::ROWS;
+    hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(fBlocks, g.n), fblk, 0, st, g.xf, g.tw);
     hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
     if (g.periodic)
         hipLaunchKernelGGL((k_cols_batch<N, COL_Z_PER>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
     else
         hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
     hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
-    hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(rowBlocks, g.n), blk, 0, st, g.xi, g.tw);
+    hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(iBlocks, g.n), iblk, 0, st, g.xi, g.tw);
     const long long cells = (long long)g.G * g.G * g.G;
     hipLaunchKernelGGL(k_grid_accel_batch, dim3((unsigned)((cells + 255) / 256), g.n), dim3(256), 0, st, g.grad);
 }
