@@ -353,8 +353,18 @@ __global__ void k_features(const float4* __restrict__ pos, const int32_t* __rest
 // fragment of k-step 2t + h of the next layer because that layer's weight
 // columns are stored in the same (permuted) order. Output: the correction as
 // a float4 per particle (inactive particles get 0).
+// Biases of layers 1-3 inside the matrix products (FP16 inference): each bias
+// is split into FP16 halves bh = fp16(b), bl = fp16(b - bh) that multiply inputs
+// fixed at 1 -- layer 1's two padding inputs (30, 31), and one more k-step of
+// 16 for layers 2 and 3 (columns 128, 129 of their rows) -- so the accumulators
+// start from zero and the biases cost no LDS reads (48 of the tile's 128 reads).
+#ifndef SD_MLP_BIAS_MFMA
+#define SD_MLP_BIAS_MFMA 1
+#endif
+constexpr bool BIAS_MM = SD_MLP_F16 && SD_MLP_BIAS_MFMA;
+constexpr int KH = MLP_H + (BIAS_MM ? 16 : 0);  // k extent of layers 2 and 3
 constexpr int W1_LD = MLP_INP + 8;  // LDS row pitches (2-byte elements); +8 spreads rows over banks
-constexpr int WH_LD = MLP_H + 8;
+constexpr int WH_LD = KH + 8;
 #ifndef SD_INF_WAVES
 #define SD_INF_WAVES 16
 #endif
@@ -449,7 +459,7 @@ __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const 
                                   : zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3];
                 val = alive ? log1p_count(rho, idx) : 0.f;
             } else {
-                const float o = q1 == 27 ? off0 : (q1 == 28 ? off1 : (q1 == 29 ? off2 : 0.f));
+                const float o = q1 == 27 ? off0 : (q1 == 28 ? off1 : (q1 == 29 ? off2 : (BIAS_MM ? 1.f : 0.f)));
                 float d0 = 0.f;
                 if (alive && !g) d0 = log1p_count(rho, zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3]);
                 val = g ? (alive ? o : 0.f) : d0;
@@ -475,6 +485,13 @@ __device__ __forceinline__ uint4 to_inf(uint4 v) {
 #endif
 }
 
+// FP32 bias -> its FP16 halves (bh, bl) as one packed word.
+__device__ __forceinline__ uint32_t bias_halves(float b) {
+    const _Float16 bh = (_Float16)b;
+    const f16x2 h = {bh, (_Float16)(b - (float)bh)};
+    return __builtin_bit_cast(uint32_t, h);
+}
+
 __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     extern __shared__ unsigned char smem[];
     inf_t* sW1 = reinterpret_cast<inf_t*>(smem);
@@ -484,12 +501,23 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     float* sB = reinterpret_cast<float*>(sW4 + MLP_OUTP * WH_LD);  // b1q | b2q | b3q | b4 (padded to 32)
     for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
         const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
-        *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i]);
+        uint4 v = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i]);
+        if (BIAS_MM && c == MLP_INP / 8 - 1) v.w = bias_halves(a.b[r]);  // columns 30, 31 (zero weights)
+        *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = v;
     }
-    for (int i = threadIdx.x; i < MLP_H * (MLP_H / 8); i += blockDim.x) {
-        const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W2q)[i]);
-        *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W3q)[i]);
+    for (int i = threadIdx.x; i < MLP_H * (KH / 8); i += blockDim.x) {
+        const int r = i / (KH / 8), c = i % (KH / 8);
+        if (c < MLP_H / 8) {
+            const int q = r * (MLP_H / 8) + c;
+            *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W2q)[q]);
+            *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W3q)[q]);
+        } else {  // the bias k-step: columns 128, 129 = (bh, bl), the rest zero
+            const bool first = c == MLP_H / 8;
+            *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) =
+                make_uint4(first ? bias_halves(a.b[MLP_H + r]) : 0u, 0u, 0u, 0u);
+            *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) =
+                make_uint4(first ? bias_halves(a.b[2 * MLP_H + r]) : 0u, 0u, 0u, 0u);
+        }
     }
     for (int i = threadIdx.x; i < MLP_OUTP * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
@@ -514,14 +542,20 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         const int i = tile * 32 + (l & 31);
         float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
         if (i < a.n) p = a.pos[i];
-        infx8 B[8];
+        infx8 B[KH / 16];
+        if constexpr (BIAS_MM) {
+            // inputs of the bias k-step: k = 128, 129 are 1 (lanes 0-31 hold k = 128..135)
+            infx8 one{};
+            if (g == 0) one[0] = one[1] = (inf_t)1.0f;
+            B[KH / 16 - 1] = one;
+        }
         f32x16 acc[4];
         infer_feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
-        mlp_layer<MLP_INP, W1_LD, 4, true>(sW1, B, acc, l, sB);
+        mlp_layer<MLP_INP, W1_LD, 4, !BIAS_MM>(sW1, B, acc, l, sB);
         mlp_activate(acc, B);
-        mlp_layer<MLP_H, WH_LD, 4, true>(sW2, B, acc, l, sB + MLP_H);
+        mlp_layer<KH, WH_LD, 4, !BIAS_MM>(sW2, B, acc, l, sB + MLP_H);
         mlp_activate(acc, B);
-        mlp_layer<MLP_H, WH_LD, 4, true>(sW3, B, acc, l, sB + 2 * MLP_H);
+        mlp_layer<KH, WH_LD, 4, !BIAS_MM>(sW3, B, acc, l, sB + 2 * MLP_H);
         mlp_activate(acc, B);
         mlp_layer<MLP_H, WH_LD, 1, false>(sW4, B, acc, l, nullptr);
         // outputs 0..2 are accumulator rows 0..2 (elements 0..2 on lanes 0..31)
