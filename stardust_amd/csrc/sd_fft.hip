@@ -13,7 +13,8 @@
 //   Y  columns  : A -> B[z][ky][kx]                                  (input pruned to G)
 //   Z  columns  : B -> B, forward * Green * inverse fused in LDS     (only z < G kept)
 //   Y' columns  : B -> A                                             (only y < G kept)
-//   X' rows     : A -> pot[z][y][x]                                  (C2R, only x < G kept)
+//   X' rows     : A -> pot[z][y][x]                                  (C2R, only x < G kept;
+//                                                                    two rows per complex line)
 //
 // Complex arrays keep kx (S/2+1 values) contiguous with a row pitch P that is
 // a multiple of 16 complex (128 B), so every column tile of 16 is one cache

@@ -9,7 +9,9 @@
 //   finalize  : deterministic reduction of those partials and the dt policy
 //               (Sim.cu:4254-4296) evaluated on the device, so a step needs no
 //               host round trip;
-//   contact   : kContact (Sim.cu:1030-1203), contact term.
+//   gradient  : kGridAccel carrying the previous step's finalize as block 0;
+//   contact   : kContact (Sim.cu:1030-1203), contact term, with an exact
+//               out-of-reach bound that skips cells without changing a bit.
 //
 // Arithmetic mirrors the reference as built by nvcc (fused multiply-adds
 // where nvcc contracts "a*b + c"), written as explicit fmaf with contraction
