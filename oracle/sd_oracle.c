@@ -832,9 +832,17 @@ int sdo_step(const sdo_params* prm, sdo_state* st, float* pos4, float* vel4, int
     float* accC = NULL;
     if (prm->contact && !((double)prm->allocN > 0.764 * (double)G * G * G)) {
         sort_particles(prm, pos4, vel4, n, scratchPos4, scratchVel4, keys, idx, 1);
-        int32_t* cs = (int32_t*)malloc(sizeof(int32_t) * cells);
-        int32_t* ce = (int32_t*)malloc(sizeof(int32_t) * cells);
-        sdo_cell_range(keys, n, (int)cells, cs, ce);
+        /* Morton cell codes of a G that is not a power of two run over the
+         * enclosing power-of-two cube */
+        size_t tableCells = cells;
+        if (prm->keyMode == SDO_KEY_MORTON) {
+            size_t g = 1;
+            while (g < (size_t)G) g <<= 1;
+            tableCells = g * g * g;
+        }
+        int32_t* cs = (int32_t*)malloc(sizeof(int32_t) * tableCells);
+        int32_t* ce = (int32_t*)malloc(sizeof(int32_t) * tableCells);
+        sdo_cell_range(keys, n, (int)tableCells, cs, ce);
         accC = (float*)malloc(sizeof(float) * 4 * (size_t)(n > 0 ? n : 1));
         sdo_contact(pos4, vel4, n, G, prm->periodic, prm->keyMode, cs, ce,
                     prm->contactRadiusCells / (float)G, prm->contactStiffness,

@@ -291,19 +291,22 @@ static long long cell_table_size(const sd_sim* s) {
     return g * g * g;
 }
 
+// Row-major keys: one table entry per cell. Morton keys: one per 4^3-cell
+// block (a cell's particles are found inside its block's run of sorted keys).
 static bool ensure_contact_buffers(sd_sim* s) {
     const size_t cells = (size_t)cell_table_size(s);
-    if (s->cellTableCells < (long long)cells) dfree(s->cellRange);
-    s->cellTableCells = (long long)cells;
     if (!s->accContact && !dalloc(s->accContact, (size_t)s->allocN)) return false;
-    if (!s->cellRange && !dalloc(s->cellRange, cells)) return false;
     if (s->modes.keyMode == SD_KEY_MORTON) {
         // Morton cell codes run over (2^k)^3 >= G^3; a block is 64 consecutive codes
         const long long blocks = (long long)(cells + 63) / 64;
         if (s->blockCells < blocks) dfree(s->blockRange);
         s->blockCells = blocks;
         if (!s->blockRange && !dalloc(s->blockRange, (size_t)blocks)) return false;
+        return true;
     }
+    if (s->cellTableCells < (long long)cells) dfree(s->cellRange);
+    s->cellTableCells = (long long)cells;
+    if (!s->cellRange && !dalloc(s->cellRange, cells)) return false;
     return true;
 }
 
@@ -442,8 +445,9 @@ static bool contact_pass(sd_sim* s) {
     if (!ensure_contact_buffers(s)) return false;
     sort_particles(s, true);
     TIMED(s, "contact", {
-        int2* blocks = s->modes.keyMode == SD_KEY_MORTON ? s->blockRange : nullptr;
-        launch_cell_range(s->sb.keys, s->allocN, s->cellRange, blocks, s->blockCells, s->st);
+        const bool morton = s->modes.keyMode == SD_KEY_MORTON;
+        int2* blocks = morton ? s->blockRange : nullptr;
+        launch_cell_range(s->sb.keys, s->allocN, morton ? nullptr : s->cellRange, blocks, s->blockCells, s->st);
         launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellRange, s->sb.keys,
                        blocks,
                        s->cfg.contactRadiusCells / (float)G, s->cfg.contactStiffness, s->cfg.contactDamping,
@@ -804,7 +808,7 @@ size_t sd_estimate_bytes(int particleCount, int gridSize, int boundary) {
     b += 8 * cells;                     // fixed-point density
     b += 8 * G * G * P + 8 * G * S * P; // half-spectrum work arrays
     b += 4 * cells + 16 * cells;        // pot, accG
-    b += 8 * cells + cells / 8;         // cell ranges and 4^3-block ranges (contact)
+    b += 8 * cells;                     // contact cell ranges (row-major keys; Morton keys: cells / 8)
     if (boundary == SD_BOUNDARY_ISOLATED) {
         b += 4 * (S / 2 + 1) * (S / 2 + 1) * P;  // Green octant
         b += 2 * 8 * S * S * P;                  // transient Green build arrays

@@ -77,15 +77,15 @@ int sort_chunks_for(int n);
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st);
 void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
                     float4* dstV, int n, hipStream_t st);
-// Cell table of sorted keys (key = cell + 1, 0 = inactive): range[c] = (first,
-// one past last) sorted index of cell c. Only non-empty cells are written; an
-// entry is current exactly when its first index f satisfies 0 <= f < n and
-// keys[f] == c + 1 (entries of cells emptied since an earlier call keep stale
-// values, so the table is never cleared).
+// Cell table of sorted keys (key = cell + 1, 0 = inactive), when range is not
+// null: range[c] = (first, one past last) sorted index of cell c. Only
+// non-empty cells are written; an entry is current exactly when its first
+// index f satisfies 0 <= f < n and keys[f] == c + 1 (entries of cells emptied
+// since an earlier call keep stale values, so the table is never cleared).
 // With blockRange (Morton keys: cell c = Morton(x, y, z), block c >> 6 = Morton
 // of the 4^3-cell block), blockRange[b] = (first, one past last) sorted index of
 // block b's particles, (0, 0) for an empty block: the table (blockEntries int2)
-// is zeroed here first.
+// is zeroed here first. Contact with Morton keys needs only the block table.
 void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRange, long long blockEntries,
                        hipStream_t st);
 
@@ -164,6 +164,8 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
                           int periodic, const StepScalars* sc, float4* velOut, hipStream_t st);
 
 // ---- contact ------------------------------------------------------------------
+// keyMode 0 (row-major keys) reads the cell table, keyMode 1 (Morton keys) the
+// 4^3-block table and the sorted keys only (cellRange unused).
 void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
                     const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
                     float stiffness, float damping, float4* accOut, hipStream_t st);

@@ -891,6 +891,37 @@ __device__ __forceinline__ int axis_blocks(int c, int G, int periodic, int b[3])
     return m;
 }
 
+// Sorted indices [first, end) of the particles with key k (cell k - 1) inside
+// the run br of that cell's 4^3-cell block (Morton keys ascend, so the cell's
+// particles are one sub-run of the block's run): a binary search for the
+// first key >= k, then, if that key is k, one for the first key > k. Every
+// probe lies in [br.x, br.y).
+__device__ __forceinline__ int2 run_of_key(const uint32_t* __restrict__ keys, int2 br, uint32_t k) {
+    int a = br.x, len = br.y - br.x;
+    while (len > 0) {
+        const int h = len >> 1;
+        if (keys[a + h] < k) {
+            a += h + 1;
+            len -= h + 1;
+        } else {
+            len = h;
+        }
+    }
+    if (a >= br.y || keys[a] != k) return make_int2(a, a);
+    int e = a + 1;
+    len = br.y - e;
+    while (len > 0) {
+        const int h = len >> 1;
+        if (keys[e + h] == k) {
+            e += h + 1;
+            len -= h + 1;
+        } else {
+            len = h;
+        }
+    }
+    return make_int2(a, e);
+}
+
 // One cell's candidates [s0, e0) in the reference's order (kContact, Sim.cu:1063-1100):
 // each counts toward the cap of 96 (self excluded), in range or not.
 __device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, const float4* __restrict__ vel, int i,
@@ -932,19 +963,23 @@ __device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, con
 // in range or not). A cell whose nearest point lies beyond the contact
 // diameter plus 1e-3 cell holds only particles that fail the distance test;
 // the margin is far above the float rounding of keys, offsets and distances
-// (about 3e-5 cell at G = 512). Two visits give exactly the reference's forces:
-// * when the 4^3-cell blocks covering the 27 cells hold at most 97 particles
-//   (the particle itself and at most 96 candidates), the cap cannot be reached
-//   and out-of-reach cells are skipped outright (Morton keys only: a block is
-//   then one run of the sorted keys, blockRange[Morton(block)]);
-// * otherwise the nine range entries of a z-plane load together and an
-//   out-of-reach cell adds only its count to the tally (capped at 96 as the
-//   reference's loop caps it).
+// (about 3e-5 cell at G = 512).
+// A cell's particles come from the cell table (row-major keys, range[c]) or,
+// with Morton keys (BLOCKS), from a search inside its 4^3-cell block's run of
+// the sorted keys (blockRange[c >> 6]); there is then no per-cell table.
+// Two visits give exactly the reference's forces:
+// * when the blocks covering the 27 cells hold at most 97 particles (the
+//   particle itself and at most 96 candidates), the cap cannot be reached and
+//   out-of-reach cells are skipped outright (Morton keys only);
+// * otherwise the nine entries of a z-plane load together and an out-of-reach
+//   cell adds only its count to the tally (capped at 96 as the reference's
+//   loop caps it).
 // Isolated boxes take the second visit at their edge cells, where the clamped
 // neighbour folds onto the particle's own cell (periodic meshes below 3 cells
 // likewise).
+template <bool BLOCKS>
 __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
-                                               int G, int periodic, int keyMode, const int2* __restrict__ range,
+                                               int G, int periodic, const int2* __restrict__ range,
                                                const uint32_t* __restrict__ keys, const int2* __restrict__ blockRange,
                                                float radius, float stiffness, float damping,
                                                float4* __restrict__ accOut) {
@@ -970,10 +1005,10 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
     const float reach = fmaf(d0, fG, 1e-3f);  // contact diameter in cells, plus the margin
     const float reach2 = reach * reach;
     const int kMaxPeers = 96;
+    const int keyMode = BLOCKS ? 1 : 0;
     int seen = 0;
 
-    bool sparse = blockRange != nullptr &&
-                  (periodic ? G >= 3 : (cx > 0 && cx < G - 1 && cy > 0 && cy < G - 1 && cz > 0 && cz < G - 1));
+    bool sparse = BLOCKS && (periodic ? G >= 3 : (cx > 0 && cx < G - 1 && cy > 0 && cy < G - 1 && cz > 0 && cz < G - 1));
     if (sparse) {
         int bx[3], by[3], bz[3];
         const int mx = axis_blocks(cx, G, periodic, bx), my = axis_blocks(cy, G, periodic, by),
@@ -993,8 +1028,10 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
                     const int c = cell_slot(cx + dx, cy + dy, cz + dz, G, periodic, keyMode);
-                    const int2 r = range[c];
-                    if (r.x < 0 || r.x >= n || keys[r.x] != (uint32_t)c + 1u || r.y <= r.x) continue;
+                    const int2 br = blockRange[c >> 6];
+                    if (br.y <= br.x) continue;
+                    const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
+                    if (r.y <= r.x) continue;
                     contact_cell(pos, vel, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
                 }
     } else {
@@ -1004,14 +1041,21 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 cs[q] = cell_slot(cx + q % 3 - 1, cy + q / 3 - 1, cz + dz, G, periodic, keyMode);
-                r[q] = range[cs[q]];
+                r[q] = BLOCKS ? blockRange[cs[q] >> 6] : range[cs[q]];
             }
-            uint32_t kf[9];
+            if (!BLOCKS) {
+                // a table entry counts only when its first index holds the cell's key
+                uint32_t kf[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) kf[q] = (r[q].x >= 0 && r[q].x < n) ? keys[r[q].x] : 0u;
+                for (int q = 0; q < 9; ++q) kf[q] = (r[q].x >= 0 && r[q].x < n) ? keys[r[q].x] : 0u;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    if (kf[q] != (uint32_t)cs[q] + 1u) r[q] = make_int2(0, 0);  // empty cell (stale or unwritten)
+            }
             for (int q = 0; q < 9 && seen < kMaxPeers; ++q) {
-                if (kf[q] != (uint32_t)cs[q] + 1u) continue;  // empty cell (stale or unwritten entry)
-                const int s0 = r[q].x, e0 = r[q].y;
+                if (r[q].y <= r[q].x) continue;
+                const int2 cr = BLOCKS ? run_of_key(keys, r[q], (uint32_t)cs[q] + 1u) : r[q];
+                const int s0 = cr.x, e0 = cr.y;
                 if (e0 <= s0) continue;
                 if (gx[q % 3] + gy[q / 3] + gz[dz + 1] > reach2) {
                     seen = min(seen + (e0 - s0), kMaxPeers);  // out of reach: the particle is not in this cell
@@ -1028,8 +1072,12 @@ void launch_contact(const float4* pos, const float4* vel, int n, int G, int peri
                     const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
                     float stiffness, float damping, float4* accOut, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_contact, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic, keyMode,
-                       cellRange, sortedKeys, blockRange, radius, stiffness, damping, accOut);
+    if (keyMode)
+        hipLaunchKernelGGL(k_contact<true>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic,
+                           nullptr, sortedKeys, blockRange, radius, stiffness, damping, accOut);
+    else
+        hipLaunchKernelGGL(k_contact<false>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic,
+                           cellRange, sortedKeys, nullptr, radius, stiffness, damping, accOut);
 }
 
 // ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------

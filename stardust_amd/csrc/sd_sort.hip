@@ -321,13 +321,15 @@ __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__
     const uint32_t k = keys[i];
     if (k == 0u) return;
     const int c = (int)(k - 1u);
-    const bool first = i == 0 || keys[i - 1] != k, last = i == n - 1 || keys[i + 1] != k;
-    if (first && last)
-        range[c] = make_int2(i, i + 1);  // a cell of one particle: one 8-byte store
-    else if (first)
-        range[c].x = i;
-    else if (last)
-        range[c].y = i + 1;
+    if (range) {
+        const bool first = i == 0 || keys[i - 1] != k, last = i == n - 1 || keys[i + 1] != k;
+        if (first && last)
+            range[c] = make_int2(i, i + 1);  // a cell of one particle: one 8-byte store
+        else if (first)
+            range[c].x = i;
+        else if (last)
+            range[c].y = i + 1;
+    }
     if (blockRange) {
         // keys ascend, so keys[i + 1] >= k > 0; an inactive predecessor (key 0) starts a run
         const uint32_t b = (uint32_t)c >> 6;

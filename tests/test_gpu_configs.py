@@ -89,24 +89,28 @@ def test_config0_product_path_vs_oracle(gpu, oracle):
     assert dp < 5e-6
 
 
-def test_config2_small_periodic_contact_vs_oracle(gpu, oracle):
+@pytest.mark.parametrize("key", ["rowmajor", "morton"])
+def test_config2_small_periodic_contact_vs_oracle(gpu, oracle, key):
+    """Row-major keys read the per-cell table; Morton keys find each cell's
+    particles inside its 4^3-cell block's run of the sorted keys."""
     pos, vel = ic.zeldovich(32, seed=1, rms_cells=0.3, G_mesh=64)
     n, G, steps = pos.shape[0], 64, 4
     grav = ic.gravity_periodic(G, 1.0, 1.0) * 1e-9  # weak gravity: contact and mesh both matter
-    _say("config2 small", n, G, steps)
+    skey, okey = (sd.KEY_ROWMAJOR, oracle.KEY_ROWMAJOR) if key == "rowmajor" else (sd.KEY_MORTON, oracle.KEY_MORTON)
+    _say("config2 small", key, n, G, steps)
     cfg = sd.default_config(particleCount=n, gridSize=G, boundary=sd.BOUNDARY_PERIODIC, gravity=grav,
                             softeningCells=3.0, sortInterval=1, contactEnabled=1, contactRadiusCells=0.2,
                             contactStiffness=1e3)
     vel = vel.copy()
     vel[:, :3] = np.random.default_rng(2).normal(0, 0.002, (n, 3)).astype(np.float32)
     s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
-                                     keyMode=sd.KEY_ROWMAJOR))
+                                     keyMode=skey))
     s.set_particles(pos, vel)
     s.run(steps)
     gp, gv = s.get_particles()
     s.close()
     prm = oracle.make_params(G, periodic=True, gravity=grav, softeningCells=3.0, sortInterval=1,
-                             keyMode=oracle.KEY_ROWMAJOR, contact=1, contactRadiusCells=0.2,
+                             keyMode=okey, contact=1, contactRadiusCells=0.2,
                              contactStiffness=1e3, allocN=n)
     st = oracle.Stepper(prm, pos, vel)
     st.step(steps)

@@ -254,11 +254,15 @@ def test_contact_vs_oracle(gpu, oracle):
         assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
 
 
-def test_contact_table_reuse_vs_oracle(gpu, oracle):
-    """The contact cell table is never cleared: a second particle set in other
-    cells (every first-set cell now empty) must see none of the first set's
-    entries. Periodic box, sparse and dense regions, cap of 96 reached."""
-    _say("contact table reuse")
+@pytest.mark.parametrize("key", ["rowmajor", "morton"])
+def test_contact_table_reuse_vs_oracle(gpu, oracle, key):
+    """Contact tables carried from one particle set to the next: a second set
+    in other cells (every first-set cell now empty) must see none of the first
+    set's entries. Row-major keys: the per-cell table is never cleared; Morton
+    keys: the 4^3-block table is rebuilt per call and cells are searched inside
+    block runs. Periodic box, sparse and dense regions, cap of 96 reached."""
+    _say("contact table reuse", key)
+    skey, okey = (sd.KEY_ROWMAJOR, oracle.KEY_ROWMAJOR) if key == "rowmajor" else (sd.KEY_MORTON, oracle.KEY_MORTON)
     n, G = 6000, 16
     rng = np.random.default_rng(8)
     first = np.zeros((n, 4), np.float32)
@@ -273,7 +277,7 @@ def test_contact_table_reuse_vs_oracle(gpu, oracle):
                             gravity=0.05, sortInterval=1, contactEnabled=1, contactRadiusCells=0.3,
                             contactStiffness=1e3)
     s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD,
-                                     keyMode=sd.KEY_MORTON))
+                                     keyMode=skey))
     s.set_fixed_dt(1e-3)
     s.set_particles(first, vel)
     s.run(2)
@@ -282,10 +286,45 @@ def test_contact_table_reuse_vs_oracle(gpu, oracle):
     gp, gv = s.get_particles()
     s.close()
     prm = oracle.make_params(G, periodic=True, gravity=0.05, softeningCells=1.0, sortInterval=1,
-                             keyMode=oracle.KEY_MORTON, contact=1, contactRadiusCells=0.3, contactStiffness=1e3,
+                             keyMode=okey, contact=1, contactRadiusCells=0.3, contactStiffness=1e3,
                              fixedDt=1e-3, allocN=n)
     st = oracle.Stepper(prm, second, vel)
     st.step(2)
+    assert np.abs(gp[:, :3] - st.pos[:, :3]).max() < 2e-6
+    assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
+
+
+@pytest.mark.parametrize("key", ["rowmajor", "morton"])
+def test_contact_isolated_edges_vs_oracle(gpu, oracle, key):
+    """Isolated box of a side that is not a power of two (Morton codes then run
+    over 128^3 codes for G = 96, taken exactly rather than rounded down to a
+    power of two): particles in the edge and corner cells, whose clamped
+    neighbours fold onto their own cells, and a dense corner that reaches the
+    cap of 96."""
+    _say("contact isolated edges", key)
+    skey, okey = (sd.KEY_ROWMAJOR, oracle.KEY_ROWMAJOR) if key == "rowmajor" else (sd.KEY_MORTON, oracle.KEY_MORTON)
+    n, G = 5000, 96
+    rng = np.random.default_rng(17)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.002, 0.998, (n, 3))
+    pos[: n // 5, :3] = rng.uniform(0.975, 0.998, (n // 5, 3))  # dense corner cells
+    pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)  # tags the particle order
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0, 0.005, (n, 3))
+    cfg = sd.default_config(particleCount=n, gridSize=G, softeningCells=1.0, gravity=0.02, sortInterval=1,
+                            contactEnabled=1, contactRadiusCells=0.4, contactStiffness=1e3)
+    s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD, keyMode=skey,
+                                     exactGrid=1))
+    s.set_fixed_dt(5e-4)
+    s.set_particles(pos, vel)
+    s.run(2)
+    gp, gv = s.get_particles()
+    s.close()
+    prm = oracle.make_params(G, gravity=0.02, softeningCells=1.0, sortInterval=1, keyMode=okey, contact=1,
+                             contactRadiusCells=0.4, contactStiffness=1e3, fixedDt=5e-4, allocN=n)
+    st = oracle.Stepper(prm, pos, vel)
+    st.step(2)
+    assert np.array_equal(gp[:, 3], st.pos[:, 3])
     assert np.abs(gp[:, :3] - st.pos[:, :3]).max() < 2e-6
     assert np.abs(gv[:, :3] - st.vel[:, :3]).max() < 1e-4 * np.abs(st.vel[:, :3]).max()
 
