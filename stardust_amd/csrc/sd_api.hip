@@ -407,21 +407,29 @@ __global__ void k_float_to_fix(const float* __restrict__ f, long long* __restric
 // solvePoisson + kGridAccel (Sim.cu:3586-3611): consumes (and clears) rhoFix.
 // Density -> potential -> acceleration grid (and a deferred finalize) on st:
 // the library's stream, or the mesh stream of an overlapped step.
-static void solve_mesh(sd_sim* s, hipStream_t st) {
+static float pot_scale(const sd_sim* s) { return s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1); }
+
+// kGridAccel: the acceleration grid from the potential
+static void grad_now(sd_sim* s, hipStream_t st) {
+    TIMED_ON(s, st, "grad", mesh_grid_accel(s->pot, s->accG, s->G, pot_scale(s), s->periodic, st));
+    s->accGValid = true;
+}
+
+// The fused step's integrate differentiates the potential in its gather, so
+// it solves without the gradient (grad = false); the acceleration grid is
+// then made only when something reads it (ensure_mesh).
+static void solve_mesh(sd_sim* s, hipStream_t st, bool grad) {
     const long long cells = (long long)s->G * s->G * s->G;
     if (s->modes.depositMode == SD_DEPOSIT_FLOAT) {
         hipLaunchKernelGGL(k_float_to_fix, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, s->rhoF,
                            s->rhoFix, cells);
     }
-    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, st));
-    TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, st));
-    const float potScale = s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1);
-    if (s->finDeferred) {
-        TIMED_ON(s, st, "grad", launch_grid_accel_fin(s->pot, s->accG, s->G, potScale, s->periodic, s->finJob, st));
-        s->finDeferred = false;
-    } else {
-        TIMED_ON(s, st, "grad", mesh_grid_accel(s->pot, s->accG, s->G, potScale, s->periodic, st));
-    }
+    const FinalizeJob* fin = s->finDeferred ? &s->finJob : nullptr;
+    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, fin, st));
+    TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, fin, st));
+    s->finDeferred = false;
+    if (grad) grad_now(s, st);
+    s->accGValid = grad;
     s->rhoValid = false;
     s->meshValid = true;
 }
@@ -431,10 +439,13 @@ static void ensure_rho(sd_sim* s) {
     if (s->modes.depositMode == SD_DEPOSIT_FLOAT || !s->rhoValid) deposit_now(s);
 }
 
-static void ensure_mesh(sd_sim* s) {
-    if (s->meshValid) return;
+static void ensure_mesh(sd_sim* s, bool grad = true) {
+    if (s->meshValid) {
+        if (grad && !s->accGValid) grad_now(s, s->st);
+        return;
+    }
     ensure_rho(s);
-    solve_mesh(s, s->st);
+    solve_mesh(s, s->st, grad);
 }
 
 // doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685)
@@ -536,7 +547,7 @@ static void prime_scalars(sd_sim* s) {
 // One step; no host synchronisation inside.
 // timed: record the three phase events of sd_timings (each is a queue marker
 // that drains the pipeline, so runs of steps record them on their last step).
-// Enqueue a finalize that was left for the next gradient launch.
+// Enqueue a finalize that no mesh solve has carried.
 static void flush_finalize(sd_sim* s) {
     if (!s->finDeferred) return;
     const FinalizeJob& j = s->finJob;
@@ -573,7 +584,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     if (overlap) {
         HK(hipEventRecord(s->evFork, s->st));
         HK(hipStreamWaitEvent(s->st2, s->evFork, 0));
-        solve_mesh(s, s->st2);
+        solve_mesh(s, s->st2, !fused);
         HK(hipEventRecord(s->evJoin, s->st2));
     }
     if (sortDue) {
@@ -593,10 +604,10 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         }
     }
     if (fused) {
-        ensure_mesh(s);
+        ensure_mesh(s, false);
     } else {
         ensure_rho(s);
-        solve_mesh(s, s->st);
+        solve_mesh(s, s->st, true);
     }
     const bool contact = contact_pass(s);
     if (overlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
@@ -605,6 +616,11 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
 
     IntegrateArgs a{};
     a.accG = s->accG;
+    if (fused) {
+        // kGridAccel's differences taken in the gather (bit-identical accelerations)
+        a.pot = s->pot;
+        a.gradH = 0.5f * (float)s->G * pot_scale(s);
+    }
     a.pos = s->pos;
     a.vel = s->vel;
     a.n = s->allocN;
@@ -639,6 +655,8 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         if (last) {
             TIMED(s, "finalize", launch_finalize(s->partials, s->npartsInt, mode, pol, s->sc, s->st));
         } else {
+            // Within a run no mesh pass (nor the sort) reads the scalars: the
+            // next solve's x pass carries the reduction as one more block.
             s->finJob = FinalizeJob{s->partials, s->npartsInt, mode, pol, s->sc};
             s->finDeferred = true;
         }
@@ -1376,7 +1394,7 @@ int sd_get_grid(sd_sim* s, int which, void* out, size_t bytes) {
     if (which == SD_GRID_POTENTIAL || which == SD_GRID_ACCEL) {
         const size_t need = which == SD_GRID_POTENTIAL ? cells * 4 : cells * 16;
         if (bytes < need) return arg_error("sd_get_grid: buffer too small");
-        ensure_mesh(s);
+        ensure_mesh(s, which == SD_GRID_ACCEL);
         HK(hipMemcpyAsync(out, which == SD_GRID_POTENTIAL ? (const void*)s->pot : (const void*)s->accG, need,
                           hipMemcpyDeviceToHost, s->st));
         HK(hipStreamSynchronize(s->st));

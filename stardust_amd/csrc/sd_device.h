@@ -103,4 +103,126 @@ struct ReducePartial {  // one per integrate block
     float pad[3];
 };
 
+struct DtPolicy {
+    float fixedDt;      // > 0: fixed step
+    float baseDt;       // 0.0016 * dtScaleFor(timeScale)
+    float cell;         // 1/G
+    float dtStiff;      // contact cap (0 = none)
+    int leapfrog;       // kick = (dtPrev + dt)/2
+};
+
+// One reduction of integrate partials into the step scalars.
+struct FinalizeJob {
+    const ReducePartial* partials;  // nullptr: none
+    int nparts;
+    int mode;  // bit0 momentum, bit1 vmax, bit2 next dt/kick, bit3 advance time
+    DtPolicy pol;
+    StepScalars* sc;
+};
+
+// Partials reduction and dt policy (Sim.cu:4254-4296) in one fixed summation
+// order: FIN virtual threads, virtual thread v adding partials v + i FIN in
+// ascending i, a butterfly inside each virtual wavefront of 64, then the
+// FIN / 64 wavefronts in index order. A block of TH threads (TH divides FIN)
+// plays the virtual threads j TH + t for j = 0 .. FIN / TH - 1 in turn, so
+// every block size gives the same bits.
+constexpr int FIN = 1024;
+
+// Virtual thread vb + threadIdx.x's sums, then its wavefront's (every lane).
+__device__ __forceinline__ void finalize_waves(const ReducePartial* __restrict__ parts, int np, int vb, double d[4],
+                                               float& vm) {
+    constexpr int UNR = 4;  // loads in flight
+    d[0] = d[1] = d[2] = d[3] = 0.0;
+    vm = 0.f;
+#pragma unroll 1
+    for (int base = vb + (int)threadIdx.x; base < np; base += UNR * FIN) {
+        ReducePartial r[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int k = base + u * FIN;
+            if (k < np) r[u] = parts[k];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (base + u * FIN < np) {
+                d[0] += r[u].sx;
+                d[1] += r[u].sy;
+                d[2] += r[u].sz;
+                d[3] += r[u].cnt;
+                vm = fmaxf(vm, r[u].vmax);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = wave_sum(d[q]);
+    vm = wave_max(vm);
+}
+
+// The step scalars from the totals (sums t, max speed m).
+__device__ __forceinline__ void finalize_apply(const double t[4], float m, int mode, const DtPolicy& pol,
+                                               StepScalars* sc) {
+    if (mode & 1) {
+        for (int q = 0; q < 4; ++q) sc->mom[q] = t[q];
+        const double c = t[3];
+        if (c >= 1.0) {
+            sc->meanSub[0] = (float)(t[0] / c);
+            sc->meanSub[1] = (float)(t[1] / c);
+            sc->meanSub[2] = (float)(t[2] / c);
+        } else {
+            sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
+        }
+    }
+    if (mode & 2) sc->vmax = m;
+    if (mode & 8) {  // advance time by the step just taken
+        sc->simTime += (double)sc->dt;
+        sc->dtPrev = sc->dt;
+        sc->kdkOpen = 0;
+        sc->stepCount += 1;
+    }
+    if (mode & 4) {  // dt (and kick) of the next integrate
+        float dt;
+        if (pol.fixedDt > 0.f) {
+            dt = pol.fixedDt;
+        } else {
+            const float vmax = sc->vmax;
+            dt = pol.baseDt;
+            const float dtMax = (vmax > 1e-6f) ? (0.8f * pol.cell / vmax) : dt;
+            if (dt > dtMax) dt = dtMax;
+            if (pol.dtStiff > 0.f && dt > pol.dtStiff) dt = pol.dtStiff;
+        }
+        sc->dt = dt;
+        sc->kick = pol.leapfrog ? 0.5f * ((sc->kdkOpen ? 0.f : sc->dtPrev) + dt) : dt;
+    }
+}
+
+// The whole reduction in one block of TH threads.
+template <int TH>
+__device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ parts, int np, int mode,
+                                              const DtPolicy& pol, StepScalars* sc) {
+    static_assert(FIN % TH == 0 && TH % 64 == 0, "block size must divide FIN");
+    __shared__ double s_w[FIN / 64][4];
+    __shared__ float s_v[FIN / 64];
+#pragma unroll 1
+    for (int j = 0; j < FIN / TH; ++j) {
+        double d[4];
+        float vm;
+        finalize_waves(parts, np, j * TH, d, vm);
+        if ((threadIdx.x & 63) == 0) {
+            const int w = (j * TH + (int)threadIdx.x) >> 6;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s_w[w][q] = d[q];
+            s_v[w] = vm;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    float m = 0.f;
+    for (int w = 0; w < FIN / 64; ++w) {
+        for (int q = 0; q < 4; ++q) t[q] += s_w[w][q];
+        m = fmaxf(m, s_v[w]);
+    }
+    finalize_apply(t, m, mode, pol, sc);
+}
+
 }  // namespace sd

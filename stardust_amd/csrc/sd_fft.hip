@@ -303,6 +303,7 @@ struct RowsFwdArgs {
     int clearSrc;
     int S;         // Green: grid side
     float soft;    // Green: softening (cells)
+    FinalizeJob fin;  // reduction carried by block 0 (fin.partials == nullptr: none)
 };
 
 // Rows per LDS line in the x passes. With SD_FFT_ROWS2 two real rows share one
@@ -441,8 +442,29 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
     }
 }
 
+// A pass can carry the previous integrate's partials reduction (finalize_body)
+// as one more block, which runs beside the transforms: the reduction must only
+// be done before the next integrate, and no mesh pass reads the step scalars.
+// The x pass carries it when its blocks have FIN threads; otherwise the fused
+// z pass does when its blocks have at least 256 (fewer threads would take
+// longer than the pass).
+template <int N>
+constexpr bool rows_fwd_carries_finalize() {
+    return XFwd::LINES * Plan<N>::T == FIN;
+}
+
 template <int N, int SRC>
 __global__ void __launch_bounds__(XFwd::LINES * Plan<N>::T) k_rows_fwd(RowsFwdArgs a, const float2* __restrict__ tw) {
+    if constexpr (rows_fwd_carries_finalize<N>()) {
+        if (a.fin.partials) {
+            if (blockIdx.x == 0) {
+                finalize_body<FIN>(a.fin.partials, a.fin.nparts, a.fin.mode, a.fin.pol, a.fin.sc);
+                return;
+            }
+            rows_fwd_body<N, SRC>(a, tw, blockIdx.x - 1);
+            return;
+        }
+    }
     rows_fwd_body<N, SRC>(a, tw, blockIdx.x);
 }
 
@@ -557,6 +579,7 @@ struct ColArgs {
     float perScale;      // periodic: -(1/G^3)/(4 pi^2)
     float perSoft;       // periodic: (2 pi soft / G)^2 (0 disables)
     float greenNorm;     // COL_GREEN: 1/S^3
+    FinalizeJob fin;     // fused z pass: reduction carried by block (0, 0) (fin.partials == nullptr: none)
 };
 
 template <int N, int MODE>
@@ -673,8 +696,25 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
     }
 }
 
+template <int N>
+constexpr bool cols_carry_finalize() {
+    constexpr int TH = TB * Plan<N>::T;
+    return FIN % TH == 0 && TH % 64 == 0 && TH >= 256;
+}
+
 template <int N, int MODE>
 __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols(ColArgs a, const float2* __restrict__ tw) {
+    if constexpr ((MODE == COL_Z_ISO || MODE == COL_Z_PER) && cols_carry_finalize<N>()) {
+        if (a.fin.partials) {  // one more row of blocks in front: block (0, 0) reduces, the rest exit
+            if (blockIdx.y == 0) {
+                if (blockIdx.x == 0)
+                    finalize_body<TB * Plan<N>::T>(a.fin.partials, a.fin.nparts, a.fin.mode, a.fin.pol, a.fin.sc);
+                return;
+            }
+            cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y - 1);
+            return;
+        }
+    }
     cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y);
 }
 
@@ -851,7 +891,8 @@ namespace sd {
 
 template <int N>
 static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hipStream_t st) {
-    dim3 grid((a.rows + XFwd::ROWS - 1) / XFwd::ROWS), block(XFwd::LINES * Plan<N>::T);
+    const int carry = a.fin.partials ? 1 : 0;  // only where rows_fwd_carries_finalize<N>()
+    dim3 grid((a.rows + XFwd::ROWS - 1) / XFwd::ROWS + carry), block(XFwd::LINES * Plan<N>::T);
     if (src == 0)
         hipLaunchKernelGGL((k_rows_fwd<N, 0>), grid, block, 0, st, a, tw);
     else
@@ -892,7 +933,8 @@ static void launch_cols_stream(const ColArgs& a, int slabs, const float2* tw, hi
 
 template <int N>
 static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw, hipStream_t st) {
-    dim3 grid((a.nb + TB - 1) / TB, slabs), block(TB * Plan<N>::T);
+    // a carried reduction (fused z pass, no persistent form) adds a row of blocks
+    dim3 grid((a.nb + TB - 1) / TB, slabs + (a.fin.partials ? 1 : 0)), block(TB * Plan<N>::T);
     if constexpr (N >= SD_COLS_STREAM_MIN_N) {
         switch (mode) {
             case COL_FWD: launch_cols_stream<N, COL_FWD>(a, slabs, tw, st); return;
@@ -929,8 +971,29 @@ bool fft_length_supported(int n) {
 
 int fft_pitch(int S) { return ((S / 2 + 1) + 15) / 16 * 16; }
 
-int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, hipStream_t st) {
+// The pass that carries a partials reduction at FFT side N: 1 the x pass, 2
+// the fused z pass (one-tile form), 0 none (it is launched before the passes).
+template <int N>
+static int carrier_of() {
+    if constexpr (rows_fwd_carries_finalize<N>()) return 1;
+    if constexpr (cols_carry_finalize<N>() && N < SD_COLS_STREAM_MIN_N) return 2;
+    return 0;
+}
+static int finalize_carrier(int S) {
+#define CALL(NN) return carrier_of<NN>()
+    SD_DISPATCH_N(S, CALL)
+#undef CALL
+    return 0;
+}
+
+int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, const FinalizeJob* fin,
+                  hipStream_t st) {
     RowsFwdArgs a{};
+    if (fin) {
+        const int c = finalize_carrier(m.S);
+        if (c == 1) a.fin = *fin;
+        if (c == 0) launch_finalize(fin->partials, fin->nparts, fin->mode, fin->pol, fin->sc, st);
+    }
     a.src = rhoFix;
     a.out = A;
     a.rows = m.G * m.G;
@@ -944,7 +1007,7 @@ int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc,
 }
 
 int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
-                    hipStream_t st) {
+                    const FinalizeJob* fin, hipStream_t st) {
     const int G = m.G, S = m.S, P = m.P, NK = S / 2 + 1;
     // Y forward: A[z][y][kx] (y < G) -> B[z][ky][kx]
     {
@@ -967,6 +1030,7 @@ int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green,
         a.green = green;
         a.perScale = m.perScale;
         a.perSoft = m.perSoft;
+        if (fin && finalize_carrier(S) == 2) a.fin = *fin;
         const int mode = m.periodic ? COL_Z_PER : COL_Z_ISO;
 #define CALL(NN) launch_cols<NN>(a, mode, S, m.tw, st)
         SD_DISPATCH_N(S, CALL)

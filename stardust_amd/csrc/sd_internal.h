@@ -21,9 +21,15 @@ struct MeshPlan {
 
 bool fft_length_supported(int n);
 int fft_pitch(int S);
-int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, hipStream_t st);
+// The solve: x pass of the density (mesh_rows_fwd), then the y, z, y' and x'
+// passes (mesh_solve_rest). fin (optional, the same job to both): the previous
+// integrate's partials reduction, carried as one more block by the x pass or
+// the fused z pass where their block size allows, else launched just before
+// the x pass.
+int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, const FinalizeJob* fin,
+                  hipStream_t st);
 int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
-                    hipStream_t st);
+                    const FinalizeJob* fin, hipStream_t st);
 int mesh_build_green(const MeshPlan& m, float softCells, float2* tmpA, float2* tmpB,
                      float* greenOut, hipStream_t st);
 void mesh_grid_accel(const float* pot, float4* acc, int G, float potScale, int periodic,
@@ -120,6 +126,11 @@ struct IntegrateArgs {
     const int32_t* order;
     const float4* srcPos;
     const float4* srcVel;
+    // With pot set, the gather takes kGridAccel's central differences of the
+    // potential itself (accG unused): node accelerations -(pot[+1] - pot[-1]) * gradH,
+    // gradH = 0.5 * G * potScale, bit-identical to gathering the gradient grid.
+    const float* pot;
+    float gradH;
 };
 int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
 int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)
@@ -130,13 +141,6 @@ void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* par
                  hipStream_t st);
 void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial* partials,
                      hipStream_t st);
-struct DtPolicy {
-    float fixedDt;      // > 0: fixed step
-    float baseDt;       // 0.0016 * dtScaleFor(timeScale)
-    float cell;         // 1/G
-    float dtStiff;      // contact cap (0 = none)
-    int leapfrog;       // kick = (dtPrev + dt)/2
-};
 // Reduce `nparts` partials; mode bit0 = momentum (store mean into sc->meanSub and
 // sc->mom), bit1 = vmax -> sc->vmax, bit2 = compute next dt/kick from vmax,
 // bit3 = advance simTime/stepCount by the current dt.
@@ -148,17 +152,7 @@ void launch_subtract_mean(const float4* pos, float4* vel, int n, StepScalars* sc
 // prefix[j] .. prefix[j + 1] - 1 (prefix[njobs] = totalBlocks).
 void launch_integrate_batch(const IntegrateArgs* jobs, const int* prefix, int njobs, int totalBlocks,
                             hipStream_t st);
-struct FinalizeJob {
-    const ReducePartial* partials;
-    int nparts;
-    int mode;
-    DtPolicy pol;
-    StepScalars* sc;
-};
 void launch_finalize_batch(const FinalizeJob* jobs, int njobs, hipStream_t st);
-// mesh_grid_accel with a finalize (of the previous integrate) as one more block
-void launch_grid_accel_fin(const float* pot, float4* acc, int G, float potScale, int periodic,
-                           const FinalizeJob& job, hipStream_t st);
 void launch_copy_f4(const float4* src, float4* dst, long long n, hipStream_t st);
 void launch_sync_velocity(const float4* accG, const float4* pos, const float4* vel, int n, int G,
                           int periodic, const StepScalars* sc, float4* velOut, hipStream_t st);

@@ -3,13 +3,14 @@
 //   deposit   : kScatter (src/sim/Sim.cu:503-521) as an int64 fixed-point CIC
 //               (exact, order-independent) or as the reference's float adds;
 //   integrate : kIntegrate + sampleAcc (Sim.cu:614-629, 685-906), plain-PM
-//               subset, with an optional fused deposit of the new positions and
+//               subset, gathering either the acceleration grid or kGridAccel's
+//               differences of the potential (Sim.cu:589-611, bit-identical),
+//               with an optional fused deposit of the new positions and
 //               fused block partials for the momentum sum (kMomentumAccum,
 //               2252-2268) and the CFL speed (kMaxSpeed, 2923-2930);
 //   finalize  : deterministic reduction of those partials and the dt policy
 //               (Sim.cu:4254-4296) evaluated on the device, so a step needs no
 //               host round trip;
-//   gradient  : kGridAccel carrying the previous step's finalize as block 0;
 //   contact   : kContact (Sim.cu:1030-1203), contact term, with an exact
 //               out-of-reach bound that skips cells without changing a bit.
 //
@@ -395,6 +396,100 @@ __device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, co
     return make_float3(ax, ay, az);
 }
 
+// sampleAcc of the node accelerations that kGridAccel (Sim.cu:589-611) would
+// write, taken from the potential itself: node (x, y, z) gets
+// -(pot[.., x+1] - pot[.., x-1]) * h per axis with the gradient kernel's edge
+// rule, in the same operations, so the result is bit-identical to sample_acc
+// over that kernel's grid. Interior stencils (ix - 1 .. ix + 2 inside the mesh
+// on every axis) load four x-rows of 4 values and eight of 2; the rest take
+// the 48 single loads of the general form.
+__device__ __forceinline__ float4 load4u(const float* p) {
+    float4 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    return v;
+}
+__device__ __forceinline__ float2 load2u(const float* p) {
+    float2 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    return v;
+}
+__device__ __forceinline__ float3 sample_acc_pot(const float* __restrict__ pot, const Cic& c, int G, int periodic,
+                                                 float h) {
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const bool inner = c.ix >= 1 && c.ix + 2 <= G - 1 && c.iy >= 1 && c.iy + 2 <= G - 1 && c.iz >= 1 &&
+                       c.iz + 2 <= G - 1;
+    if (inner) {
+        const size_t GG = (size_t)G * G;
+        const float* base = pot + ((size_t)c.iz * G + c.iy) * G + c.ix;
+        float4 r[2][2];      // [oz][oy]: x = ix - 1 .. ix + 2
+        float2 fy[2][2];     // [oz][0: y = iy - 1, 1: y = iy + 2]: x = ix, ix + 1
+        float2 fz[2][2];     // [0: z = iz - 1, 1: z = iz + 2][oy]
+#pragma unroll
+        for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) r[oz][oy] = load4u(base + oz * GG + (size_t)oy * G - 1);
+#pragma unroll
+        for (int oz = 0; oz < 2; ++oz) {
+            fy[oz][0] = load2u(base + oz * GG - G);
+            fy[oz][1] = load2u(base + oz * GG + 2 * (size_t)G);
+        }
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            fz[0][oy] = load2u(base - GG + (size_t)oy * G);
+            fz[1][oy] = load2u(base + 2 * GG + (size_t)oy * G);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+            const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
+            const float4 R = r[oz][oy];
+            const float xm = ox ? R.y : R.x, xp = ox ? R.w : R.z;
+            const float4 Ro = r[oz][oy ^ 1];
+            const float2 Fy = fy[oz][oy];
+            const float yo = ox ? Ro.z : Ro.y;  // the other core row at this x
+            const float yf = ox ? Fy.y : Fy.x;  // the face row at this x
+            const float ym = oy ? yo : yf, yp = oy ? yf : yo;
+            const float4 Zo = r[oz ^ 1][oy];
+            const float2 Fz = fz[oz][oy];
+            const float zo = ox ? Zo.z : Zo.y;
+            const float zf = ox ? Fz.y : Fz.x;
+            const float zm = oz ? zo : zf, zp = oz ? zf : zo;
+            const float gx = -(xp - xm) * h, gy = -(yp - ym) * h, gz = -(zp - zm) * h;
+            ax = fmaf(gx, w, ax);
+            ay = fmaf(gy, w, ay);
+            az = fmaf(gz, w, az);
+        }
+        return make_float3(ax, ay, az);
+    }
+    const int xs[2] = {wrap_or_clamp(c.ix, G, periodic), wrap_or_clamp(c.ix + 1, G, periodic)};
+    const int ys[2] = {wrap_or_clamp(c.iy, G, periodic), wrap_or_clamp(c.iy + 1, G, periodic)};
+    const int zs[2] = {wrap_or_clamp(c.iz, G, periodic), wrap_or_clamp(c.iz + 1, G, periodic)};
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
+        const int x = xs[ox], y = ys[oy], z = zs[oz];
+        int xm, xp, ym, yp, zm, zp;
+        if (periodic) {
+            xm = x == 0 ? G - 1 : x - 1; xp = x == G - 1 ? 0 : x + 1;
+            ym = y == 0 ? G - 1 : y - 1; yp = y == G - 1 ? 0 : y + 1;
+            zm = z == 0 ? G - 1 : z - 1; zp = z == G - 1 ? 0 : z + 1;
+        } else {
+            xm = max(x - 1, 0); xp = min(x + 1, G - 1);
+            ym = max(y - 1, 0); yp = min(y + 1, G - 1);
+            zm = max(z - 1, 0); zp = min(z + 1, G - 1);
+        }
+        const size_t row = ((size_t)z * G + y) * G;
+        const float gx = -(pot[row + xp] - pot[row + xm]) * h;
+        const float gy = -(pot[((size_t)z * G + yp) * G + x] - pot[((size_t)z * G + ym) * G + x]) * h;
+        const float gz = -(pot[((size_t)zp * G + y) * G + x] - pot[((size_t)zm * G + y) * G + x]) * h;
+        ax = fmaf(gx, w, ax);
+        ay = fmaf(gy, w, ay);
+        az = fmaf(gz, w, az);
+    }
+    return make_float3(ax, ay, az);
+}
+
 // Block reduction of (sum v, count, max |v|) into one partial per block.
 __device__ __forceinline__ void block_partial(double sx, double sy, double sz, double cnt, float vmax,
                                               ReducePartial* out, int blk) {
@@ -474,7 +569,9 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
         ACC[k] = make_float3(0.f, 0.f, 0.f);
-        if (P[k].x >= 0.f) ACC[k] = sample_acc(a.accG, cic_setup(P[k], fG), a.G, a.periodic);
+        if (P[k].x >= 0.f)
+            ACC[k] = a.pot ? sample_acc_pot(a.pot, cic_setup(P[k], fG), a.G, a.periodic, a.gradH)
+                           : sample_acc(a.accG, cic_setup(P[k], fG), a.G, a.periodic);
     }
     const float kick = a.sc->kick, drift = a.sc->dt;
     // phase 3: kick, drift, boundaries, stores
@@ -660,124 +757,15 @@ void launch_momentum(const float4* pos, const float4* vel, int n, ReducePartial*
     hipLaunchKernelGGL(k_momentum, dim3(particle_blocks(n > 0 ? n : 1)), dim3(PT), 0, st, pos, vel, n, partials);
 }
 
-// Deterministic reduction (fixed summation order) + dt policy.
-// Threads of every finalize reduction (its summation order depends on it, so all
-// callers -- the finalize launch, the per-scene batch launch and the finalize
-// block riding on the next gradient launch -- use the same count).
-constexpr int FIN = 1024;
-__device__ __forceinline__ void finalize_body(const ReducePartial* __restrict__ parts, int np, int mode,
-                                              const DtPolicy& pol, StepScalars* sc) {
-    // fixed order: per thread the partials k = thread + i FIN in ascending i
-    // (loaded 4 at a time so the loads overlap), a butterfly inside each
-    // wavefront, then the wavefronts in index order
-    __shared__ double s_w[FIN / 64][4];
-    __shared__ float s_v[FIN / 64];
-    __shared__ double s_d[1][4];
-    __shared__ float s_f[1];
-    double d[4] = {0.0, 0.0, 0.0, 0.0};
-    float vm = 0.f;
-    constexpr int UNR = 4;
-    for (int base = threadIdx.x; base < np; base += UNR * FIN) {
-        ReducePartial r[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int k = base + u * FIN;
-            if (k < np) r[u] = parts[k];
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            if (base + u * FIN < np) {
-                d[0] += r[u].sx;
-                d[1] += r[u].sy;
-                d[2] += r[u].sz;
-                d[3] += r[u].cnt;
-                vm = fmaxf(vm, r[u].vmax);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) d[q] = wave_sum(d[q]);
-    vm = wave_max(vm);
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s_w[threadIdx.x >> 6][q] = d[q];
-        s_v[threadIdx.x >> 6] = vm;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[4] = {0.0, 0.0, 0.0, 0.0};
-        float m = 0.f;
-        for (int w = 0; w < FIN / 64; ++w) {
-            for (int q = 0; q < 4; ++q) t[q] += s_w[w][q];
-            m = fmaxf(m, s_v[w]);
-        }
-        for (int q = 0; q < 4; ++q) s_d[0][q] = t[q];
-        s_f[0] = m;
-    }
-    if (threadIdx.x != 0) return;
-    if (mode & 1) {
-        for (int q = 0; q < 4; ++q) sc->mom[q] = s_d[0][q];
-        const double c = s_d[0][3];
-        if (c >= 1.0) {
-            sc->meanSub[0] = (float)(s_d[0][0] / c);
-            sc->meanSub[1] = (float)(s_d[0][1] / c);
-            sc->meanSub[2] = (float)(s_d[0][2] / c);
-        } else {
-            sc->meanSub[0] = sc->meanSub[1] = sc->meanSub[2] = 0.f;
-        }
-    }
-    if (mode & 2) sc->vmax = s_f[0];
-    if (mode & 8) {  // advance time by the step just taken
-        sc->simTime += (double)sc->dt;
-        sc->dtPrev = sc->dt;
-        sc->kdkOpen = 0;
-        sc->stepCount += 1;
-    }
-    if (mode & 4) {  // dt (and kick) of the next integrate
-        float dt;
-        if (pol.fixedDt > 0.f) {
-            dt = pol.fixedDt;
-        } else {
-            const float vmax = sc->vmax;
-            dt = pol.baseDt;
-            const float dtMax = (vmax > 1e-6f) ? (0.8f * pol.cell / vmax) : dt;
-            if (dt > dtMax) dt = dtMax;
-            if (pol.dtStiff > 0.f && dt > pol.dtStiff) dt = pol.dtStiff;
-        }
-        sc->dt = dt;
-        sc->kick = pol.leapfrog ? 0.5f * ((sc->kdkOpen ? 0.f : sc->dtPrev) + dt) : dt;
-    }
-}
-
 __global__ void __launch_bounds__(FIN) k_finalize(const ReducePartial* __restrict__ parts, int np, int mode,
                                                   DtPolicy pol, StepScalars* sc) {
-    finalize_body(parts, np, mode, pol, sc);
+    finalize_body<FIN>(parts, np, mode, pol, sc);
 }
 
 // One block per simulation.
 __global__ void __launch_bounds__(FIN) k_finalize_batch(const FinalizeJob* __restrict__ jobs) {
     const FinalizeJob j = jobs[blockIdx.x];
-    finalize_body(j.partials, j.nparts, j.mode, j.pol, j.sc);
-}
-
-// Gradient of the next step with the previous step's finalize as block 0
-// (blocks 1.. are k_grid_accel's): the reduction only has to be done before
-// the next integrate reads the scalars, and it then overlaps the gradient.
-__global__ void __launch_bounds__(FIN) k_grid_accel_fin(const float* __restrict__ pot, float4* __restrict__ acc,
-                                                        int G, float h, int periodic, FinalizeJob job) {
-    if (blockIdx.x == 0) {
-        finalize_body(job.partials, job.nparts, job.mode, job.pol, job.sc);
-        return;
-    }
-    grid_accel_body(pot, acc, G, h, periodic, (long long)(blockIdx.x - 1) * FIN + threadIdx.x);
-}
-
-void launch_grid_accel_fin(const float* pot, float4* acc, int G, float potScale, int periodic,
-                           const FinalizeJob& job, hipStream_t st) {
-    const long long cells = (long long)G * G * G;
-    const float h = 0.5f * (float)G * potScale;
-    hipLaunchKernelGGL(k_grid_accel_fin, dim3((unsigned)(1 + (cells + FIN - 1) / FIN)), dim3(FIN), 0, st, pot, acc,
-                       G, h, periodic, job);
+    finalize_body<FIN>(j.partials, j.nparts, j.mode, j.pol, j.sc);
 }
 
 void launch_finalize(const ReducePartial* partials, int nparts, int mode, DtPolicy pol, StepScalars* sc,

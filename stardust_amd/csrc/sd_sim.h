@@ -79,15 +79,16 @@ struct sd_sim {
     int nparts = 0;     // one-particle-per-thread reductions
     int npartsInt = 0;  // k_integrate partials
     sd::StepScalars* sc = nullptr;
-    // a finalize whose launch rides on the next gradient launch (only between
-    // steps of one sd_run; outside a run every finalize has been enqueued)
+    // a finalize whose launch rides on the next mesh solve's x pass (only
+    // between steps of one sd_run; outside a run every finalize has been enqueued)
     bool finDeferred = false;
     sd::FinalizeJob finJob{};
     sd::StepScalars* scHost = nullptr;  // pinned mirror for reads
 
     // state
     bool rhoValid = false;   // rhoFix holds the deposit of the current positions
-    bool meshValid = false;  // accG matches the current positions
+    bool meshValid = false;  // pot matches the current positions
+    bool accGValid = false;  // accG holds pot's gradient (with meshValid)
     bool primed = false;     // sc->dt / sc->kick are set for the next integrate
     long long stepCount = 0;
 

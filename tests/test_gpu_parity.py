@@ -204,6 +204,34 @@ def test_fused_pipeline_matches_reference_order(gpu, oracle, integrator):
     assert abs(t0 - t1) < 1e-9
 
 
+@pytest.mark.parametrize("G,n", [(128, 2_500_000), (64, 2_500_000), (32, 300_000), (128, 300_000)])
+def test_run_equals_single_steps(gpu, G, n):
+    """Within a run the partials reduction of each step is carried by the next
+    mesh solve's x pass (split over several blocks where the pass allows);
+    single steps reduce in a launch of their own. Both sum in one fixed order,
+    so a run and the same steps taken one at a time agree bit for bit, here
+    under the CFL dt with momentum removal (more than 4 partials per virtual
+    thread at 2.5 M particles)."""
+    _say("run vs steps", G, n)
+    pos, vel = random_particles(n, seed=31)
+    out = []
+    for single in (False, True):
+        s = make_sim(n, G, gravity=0.5, sort_interval=3)
+        s.set_particles(pos, vel)
+        if single:
+            for _ in range(7):
+                s.step()
+        else:
+            s.run(7)
+        p, v = s.get_particles()
+        out.append((p, v, s.sim_time()))
+        s.close()
+    (p0, v0, t0), (p1, v1, t1) = out
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(v0, v1)
+    assert t0 == t1
+
+
 def test_kdk_equals_shifted_kd(gpu):
     """Leapfrog from v0 equals the kick-drift scheme started from
     v0 - a(x0) dt / 2 (positions), a closed-form relation of the two."""
