@@ -30,10 +30,10 @@ def log(*a):
 
 def integrate_bytes(n_live, G, contact=False):
     """Algorithmic bytes of one k_integrate launch: pos + vel read and written
-    (64 B per particle) + the acceleration grid read once (16 B per cell) + the
-    fixed-point density written once (8 B per cell) (+ the contact acceleration,
-    16 B per particle, when contact is on)."""
-    return (80.0 if contact else 64.0) * n_live + 24.0 * G ** 3
+    (64 B per particle) + the potential read once (4 B per cell; the gather
+    takes its differences) + the fixed-point density written once (8 B per
+    cell) (+ the contact acceleration, 16 B per particle, when contact is on)."""
+    return (80.0 if contact else 64.0) * n_live + 12.0 * G ** 3
 
 
 def cpu_baseline(cfg, n_steps, threads):

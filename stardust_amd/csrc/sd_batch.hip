@@ -11,9 +11,11 @@
 //              sort keeps every scene in its slice and orders it as its own
 //              sort would);
 //   mesh     : scenes grouped by mesh size (G, S, boundary), one launch per FFT
-//              pass and one gradient launch per group;
+//              pass per group;
 //   particles: one integrate launch (blocks mapped to scenes through a prefix
-//              table) and one finalize launch (a block per scene);
+//              table; the gather differentiates each scene's potential, so
+//              there is no gradient launch) and one finalize launch (a block
+//              per scene);
 // and a whole run of K steps (plus position snapshots every M steps) is
 // recorded once into a HIP graph and replayed with a single launch. Each
 // scene's arithmetic is exactly its own single-scene step, so a batched scene
@@ -248,6 +250,8 @@ static int build_tables(sd_batch* b) {
         keyPre[i + 1] = keyPre[i] + (n + 255) / 256;
         IntegrateArgs a{};
         a.accG = s->accG;
+        a.pot = s->pot;  // the gather differentiates the potential: no gradient launches
+        a.gradH = 0.5f * (float)s->G * (s->cfg.gravity / (float)(s->allocN > 0 ? s->allocN : 1));
         a.pos = s->pos;
         a.vel = s->vel;
         a.n = n;
@@ -307,7 +311,7 @@ static void record_step(sd_batch* b, long long stepIndex, int sortInterval) {
         launch_copy_f4(b->posTmp, b->pos, n, st);
         launch_copy_f4(b->velTmp, b->vel, n, st);
     }
-    for (MeshGroup* g : b->groups) mesh_group_solve(g, st);
+    for (MeshGroup* g : b->groups) mesh_group_solve(g, false, st);
     launch_integrate_batch(b->dInteg, b->dIntegPrefix, (int)b->scenes.size(), b->integBlocks, st);
     launch_finalize_batch(b->dFin, (int)b->scenes.size(), st);
 }

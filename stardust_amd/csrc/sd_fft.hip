@@ -1191,7 +1191,7 @@ void mesh_group_destroy(MeshGroup* g) {
 }
 
 template <int N>
-static void group_solve(const MeshGroup& g, hipStream_t st) {
+static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
     const dim3 blk(TB * Plan<N>::T), fblk(XFwd::LINES * Plan<N>::T), iblk(XInv::LINES * Plan<N>::T);
     const int NK = g.S / 2 + 1, colBlocks = (NK + TB - 1) / TB;
     const int fBlocks = (g.G * g.G + XFwd::ROWS - 1) / XFwd::ROWS, iBlocks = (g.G * g.G + XInv::ROWS - 1) / XInv::ROWS;
@@ -1203,13 +1203,14 @@ static void group_solve(const MeshGroup& g, hipStream_t st) {
         hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
     hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
     hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(iBlocks, g.n), iblk, 0, st, g.xi, g.tw);
+    if (!grad) return;
     const long long cells = (long long)g.G * g.G * g.G;
     hipLaunchKernelGGL(k_grid_accel_batch, dim3((unsigned)((cells + 255) / 256), g.n), dim3(256), 0, st, g.grad);
 }
 
-int mesh_group_solve(const MeshGroup* g, hipStream_t st) {
+int mesh_group_solve(const MeshGroup* g, bool grad, hipStream_t st) {
     if (!g) return -1;
-#define CALL(NN) group_solve<NN>(*g, st)
+#define CALL(NN) group_solve<NN>(*g, grad, st)
     SD_DISPATCH_N(g->S, CALL)
 #undef CALL
     return 0;

@@ -50,7 +50,8 @@ struct MeshGroupJob {
 struct MeshGroup;
 MeshGroup* mesh_group_create(const MeshGroupJob* jobs, int n);  // nullptr if the jobs differ in size
 void mesh_group_destroy(MeshGroup* g);
-int mesh_group_solve(const MeshGroup* g, hipStream_t st);
+// grad: also write each job's acceleration grid (accG) from its potential
+int mesh_group_solve(const MeshGroup* g, bool grad, hipStream_t st);
 
 // ---- keys / sort ------------------------------------------------------------
 struct SortBuffers {
