@@ -22,6 +22,7 @@
 #include "sd_internal.h"
 
 #include <climits>
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -70,6 +71,20 @@ __device__ __forceinline__ Cic cic_setup(float4 p, float fG) {
     c.fy = gy - (float)c.iy;
     c.fz = gz - (float)c.iz;
     return c;
+}
+
+// Grid addressing in 32-bit arithmetic. Meshes have G <= 512, so a cell index
+// (< 2^27) comes from 24-bit multiplies (full-rate VALU, where 32- and 64-bit
+// integer multiplies run at a quarter of the rate), and every grid fits 2^32
+// bytes, so an element is a 32-bit byte offset from the array's base, which
+// the load or atomic adds to the (uniform) base itself.
+__device__ __forceinline__ unsigned cell_idx(int x, int y, int z, int G) {
+    return __umul24(__umul24((unsigned)z, (unsigned)G) + (unsigned)y, (unsigned)G) + (unsigned)x;
+}
+template <typename T>
+__device__ __forceinline__ T* elem32(T* base, unsigned idx) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + idx * (unsigned)sizeof(T));
 }
 
 // The 2^24 mass units of a particle split exactly along x, y, z (w1 = rint(f w),
@@ -243,10 +258,8 @@ __device__ __forceinline__ void add_global8(const Cic& c, const uint32_t v[8], i
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        if (v[k] != 0u) {
-            const size_t idx = ((size_t)zs[oz] * G + ys[oy]) * G + xs[ox];
-            atomicAdd((unsigned long long*)(rho + idx), (unsigned long long)v[k]);
-        }
+        if (v[k] != 0u)
+            atomicAdd((unsigned long long*)elem32(rho, cell_idx(xs[ox], ys[oy], zs[oz], G)), (unsigned long long)v[k]);
     }
 }
 
@@ -327,13 +340,21 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
         }
     }
     __syncthreads();
+    // tile cell t -> (x, y, z): quotients from float reciprocals, exact since
+    // t < 2^10 keeps (t + 1/2) / d at least 1/(2d) from an integer, far beyond
+    // the float rounding
+    const int nxy = ndx * ndy;
+    const float invX = 1.0f / (float)ndx, invXY = 1.0f / (float)nxy;
     for (int t = threadIdx.x; t < nv; t += PT) {
         const unsigned long long v = s_tile[t];
         if (v == 0ull) continue;
-        const int x = t % ndx, y = (t / ndx) % ndy, z = t / (ndx * ndy);
+        const int z = (int)(((float)t + 0.5f) * invXY);
+        const int r = t - __mul24(z, nxy);
+        const int y = (int)(((float)r + 0.5f) * invX);
+        const int x = r - __mul24(y, ndx);
         const int gx = wrap_or_clamp(lo[0] + x, G, periodic), gy = wrap_or_clamp(lo[1] + y, G, periodic),
                   gz = wrap_or_clamp(lo[2] + z, G, periodic);
-        atomicAdd((unsigned long long*)(rho + ((size_t)gz * G + gy) * G + gx), v);
+        atomicAdd((unsigned long long*)elem32(rho, cell_idx(gx, gy, gz, G)), v);
     }
 }
 
@@ -388,7 +409,7 @@ __device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, co
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
         const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
         const int x = ox ? x1 : x0, y = oy ? y1 : y0, z = oz ? z1 : z0;
-        const float4 g = accG[((size_t)z * G + y) * G + x];
+        const float4 g = *elem32(accG, cell_idx(x, y, z, G));
         ax = fmaf(g.x, w, ax);
         ay = fmaf(g.y, w, ay);
         az = fmaf(g.z, w, az);
@@ -419,24 +440,24 @@ __device__ __forceinline__ float3 sample_acc_pot(const float* __restrict__ pot, 
     const bool inner = c.ix >= 1 && c.ix + 2 <= G - 1 && c.iy >= 1 && c.iy + 2 <= G - 1 && c.iz >= 1 &&
                        c.iz + 2 <= G - 1;
     if (inner) {
-        const size_t GG = (size_t)G * G;
-        const float* base = pot + ((size_t)c.iz * G + c.iy) * G + c.ix;
+        const unsigned g1 = (unsigned)G, g2 = (unsigned)G * (unsigned)G;
+        const unsigned b = cell_idx(c.ix, c.iy, c.iz, G);  // every offset below stays >= 0 here
         float4 r[2][2];      // [oz][oy]: x = ix - 1 .. ix + 2
         float2 fy[2][2];     // [oz][0: y = iy - 1, 1: y = iy + 2]: x = ix, ix + 1
         float2 fz[2][2];     // [0: z = iz - 1, 1: z = iz + 2][oy]
 #pragma unroll
         for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
-            for (int oy = 0; oy < 2; ++oy) r[oz][oy] = load4u(base + oz * GG + (size_t)oy * G - 1);
+            for (int oy = 0; oy < 2; ++oy) r[oz][oy] = load4u(elem32(pot, b + oz * g2 + oy * g1 - 1u));
 #pragma unroll
         for (int oz = 0; oz < 2; ++oz) {
-            fy[oz][0] = load2u(base + oz * GG - G);
-            fy[oz][1] = load2u(base + oz * GG + 2 * (size_t)G);
+            fy[oz][0] = load2u(elem32(pot, b + oz * g2 - g1));
+            fy[oz][1] = load2u(elem32(pot, b + oz * g2 + 2u * g1));
         }
 #pragma unroll
         for (int oy = 0; oy < 2; ++oy) {
-            fz[0][oy] = load2u(base - GG + (size_t)oy * G);
-            fz[1][oy] = load2u(base + 2 * GG + (size_t)oy * G);
+            fz[0][oy] = load2u(elem32(pot, b - g2 + oy * g1));
+            fz[1][oy] = load2u(elem32(pot, b + 2u * g2 + oy * g1));
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -479,10 +500,9 @@ __device__ __forceinline__ float3 sample_acc_pot(const float* __restrict__ pot, 
             ym = max(y - 1, 0); yp = min(y + 1, G - 1);
             zm = max(z - 1, 0); zp = min(z + 1, G - 1);
         }
-        const size_t row = ((size_t)z * G + y) * G;
-        const float gx = -(pot[row + xp] - pot[row + xm]) * h;
-        const float gy = -(pot[((size_t)z * G + yp) * G + x] - pot[((size_t)z * G + ym) * G + x]) * h;
-        const float gz = -(pot[((size_t)zp * G + y) * G + x] - pot[((size_t)zm * G + y) * G + x]) * h;
+        const float gx = -(*elem32(pot, cell_idx(xp, y, z, G)) - *elem32(pot, cell_idx(xm, y, z, G))) * h;
+        const float gy = -(*elem32(pot, cell_idx(x, yp, z, G)) - *elem32(pot, cell_idx(x, ym, z, G))) * h;
+        const float gz = -(*elem32(pot, cell_idx(x, y, zp, G)) - *elem32(pot, cell_idx(x, y, zm, G))) * h;
         ax = fmaf(gx, w, ax);
         ay = fmaf(gy, w, ay);
         az = fmaf(gz, w, az);
