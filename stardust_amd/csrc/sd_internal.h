@@ -60,7 +60,7 @@ struct SortBuffers {
     uint32_t* keysAlt;  // n
     int32_t* idxAlt;    // n
     uint32_t* hist;     // 256 * maxTiles
-    uint32_t* chunk;    // scan partials
+    uint32_t* chunk;    // 256 digit totals of the current pass
     int maxTiles;
 };
 void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
@@ -78,7 +78,7 @@ struct KeyJob {
 void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* prefix, int njobs,
                             int totalBlocks, uint32_t* keys, int32_t* idx, hipStream_t st);
 int sort_tiles_for(int n);
-int sort_chunks_for(int n);
+int sort_chunks_for(int n);  // entries of SortBuffers::chunk
 // Stable LSD sort of (keys, idx) by the low `bits` bits. The result is in
 // buf.keys / buf.idx on return.
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st);

@@ -162,76 +162,62 @@ __device__ uint32_t block_scan_excl(uint32_t* v4, uint32_t* s_wave) {
     return total;
 }
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ h, int total,
-                                                     uint32_t* __restrict__ chunk) {
+// Exclusive scan of each digit's row of tile counts (hist is digit-major:
+// hist[d * ntiles + tile]), in place, one block per digit; totals[d] = the
+// digit's key count. The scatter adds the digit's base (the exclusive scan of
+// totals) itself.
+__global__ void __launch_bounds__(256) k_scan_digit_rows(uint32_t* __restrict__ h, int ntiles,
+                                                         uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_wave[WAVES];
-    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4;
-    uint32_t v4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v4[k] = (base + k < total) ? h[base + k] : 0u;
-    const uint32_t t = block_scan_excl(v4, s_wave);
-    if (threadIdx.x == 0) chunk[blockIdx.x] = t;
-}
-
-__global__ void __launch_bounds__(256) k_scan_top(uint32_t* __restrict__ chunk, int nchunks) {
-    __shared__ uint32_t s_wave[WAVES];
-    __shared__ uint32_t s_carry;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (int off = 0; off < nchunks; off += SCAN_CHUNK) {
-        uint32_t v4[4];
+    uint32_t* row = h + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int off = 0; off < ntiles; off += SCAN_CHUNK) {  // uniform
         const int base = off + threadIdx.x * 4;
+        uint32_t v4[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v4[k] = (base + k < nchunks) ? chunk[base + k] : 0u;
+        for (int k = 0; k < 4; ++k) v4[k] = (base + k < ntiles) ? row[base + k] : 0u;
         const uint32_t t = block_scan_excl(v4, s_wave);
-        const uint32_t carry = s_carry;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (base + k < nchunks) chunk[base + k] = v4[k] + carry;
-        __syncthreads();
-        if (threadIdx.x == 0) s_carry = carry + t;
-        __syncthreads();
+            if (base + k < ntiles) row[base + k] = v4[k] + carry;
+        carry += t;
     }
-}
-
-__global__ void __launch_bounds__(256) k_scan_down(uint32_t* __restrict__ h, int total,
-                                                   const uint32_t* __restrict__ chunk) {
-    __shared__ uint32_t s_wave[WAVES];
-    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4;
-    uint32_t v4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v4[k] = (base + k < total) ? h[base + k] : 0u;
-    block_scan_excl(v4, s_wave);
-    const uint32_t c = chunk[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (base + k < total) h[base + k] = v4[k] + c;
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 #ifndef SD_SORT_LOCAL
 #define SD_SORT_LOCAL 1
 #endif
 
-// Scatter of one tile. Every key's rank within its digit in the tile comes
+// Scatter of one tile. A digit's first output slot is its base (exclusive scan
+// of the digit totals, done by every block) plus the tile's entry in the
+// digit's scanned row. Every key's rank within its digit in the tile comes
 // from the ballot match and running per-digit counters. LOCAL first orders the
 // tile by digit in LDS (the tile's digit counts are the differences of
-// consecutive entries of the digit-major exclusive scan), so that each digit's
+// consecutive entries of the digit's scanned row), so that each digit's
 // keys leave as one contiguous run instead of one store per key per round:
 // that pays when a round's 256 keys spread over many digits (sparse keys), and
 // costs LDS traffic and occupancy when equal digits already come in long runs.
 template <bool LOCAL>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* __restrict__ keys,
                                                                const int32_t* __restrict__ vals, int n, int shift,
-                                                               const uint32_t* __restrict__ offs, int ntiles,
+                                                               const uint32_t* __restrict__ offs,
+                                                               const uint32_t* __restrict__ totals, int ntiles,
                                                                uint32_t* __restrict__ keysOut,
                                                                int32_t* __restrict__ valsOut) {
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_run[256];
     __shared__ uint32_t s_wcnt[WAVES][256];
     __shared__ uint32_t s_wbase[WAVES][256];
+    __shared__ uint32_t s_wave[WAVES];
     const int t = threadIdx.x;
     const size_t flat = (size_t)t * ntiles + blockIdx.x;
-    s_off[t] = offs[flat];
+    const uint32_t loc = offs[flat], tot = totals[t];
+    {
+        uint32_t v4[4] = {tot, 0u, 0u, 0u};
+        block_scan_excl(v4, s_wave);
+        s_off[t] = v4[0] + loc;
+    }
     s_run[t] = 0;
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) s_wcnt[w][t] = 0;
@@ -239,9 +225,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
     __shared__ uint32_t s_lk[LOCAL ? SORT_TILE : 1];
     __shared__ int32_t s_lv[LOCAL ? SORT_TILE : 1];
     if constexpr (LOCAL) {
-        __shared__ uint32_t s_wave[WAVES];
-        const uint32_t next = flat + 1 < (size_t)256 * ntiles ? offs[flat + 1] : (uint32_t)n;
-        uint32_t v4[4] = {next - s_off[t], 0u, 0u, 0u};
+        const uint32_t next = (int)blockIdx.x + 1 < ntiles ? offs[flat + 1] : tot;
+        uint32_t v4[4] = {next - loc, 0u, 0u, 0u};
         block_scan_excl(v4, s_wave);
         s_start[t] = v4[0];
     }
@@ -358,16 +343,11 @@ void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* pr
 }
 
 int sort_tiles_for(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
-int sort_chunks_for(int n) {
-    const int total = 256 * sort_tiles_for(n);
-    return (total + SCAN_CHUNK - 1) / SCAN_CHUNK;
-}
+int sort_chunks_for(int) { return 256; }  // the digit totals
 
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
     if (n <= 1) return;
     const int ntiles = sort_tiles_for(n);
-    const int total = 256 * ntiles;
-    const int nchunks = (total + SCAN_CHUNK - 1) / SCAN_CHUNK;
     // Keys sparse in their code space (under one key per two codes) spread each
     // round over many digits: order the tile in LDS first, except in the last
     // pass, whose few digit values come in long runs. (G=512 cosmic web, 16.7 M
@@ -378,15 +358,13 @@ void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
                            buf.hist, ntiles);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, buf.chunk, nchunks);
-        hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(256), 0, st, buf.hist, total, buf.chunk);
+        hipLaunchKernelGGL(k_scan_digit_rows, dim3(256), dim3(256), 0, st, buf.hist, ntiles, buf.chunk);
         if (sparse && shift + 8 < bits)
             hipLaunchKernelGGL(k_sort_scatter<true>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
-                               shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
+                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt);
         else
             hipLaunchKernelGGL(k_sort_scatter<false>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
-                               shift, buf.hist, ntiles, buf.keysAlt, buf.idxAlt);
+                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt);
         uint32_t* tk = buf.keys;
         buf.keys = buf.keysAlt;
         buf.keysAlt = tk;
