@@ -340,8 +340,8 @@ static void sort_keys(sd_sim* s) {
     const int n = s->allocN;
     const int kmode = s->modes.keyMode == SD_KEY_MORTON ? 2 : 0;
     TIMED(s, "sort", {
-        launch_cell_keys(s->pos, n, s->G, kmode, s->sb.keys, s->sb.idx, s->st);
-        radix_sort_pairs(s->sb, n, key_bits(s), s->st);
+        launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, s->st);
+        radix_sort_pairs(s->sb, n, key_bits(s), s->st, true);
     });
     s->keysSorted = true;
     s->corrValid = false;
@@ -353,8 +353,8 @@ static void sort_particles(sd_sim* s, bool forContact = false) {
     const int n = s->allocN;
     const int kmode = s->modes.keyMode == SD_KEY_MORTON ? (forContact ? 1 : 2) : 0;
     TIMED(s, "sort", {
-        launch_cell_keys(s->pos, n, s->G, kmode, s->sb.keys, s->sb.idx, s->st);
-        radix_sort_pairs(s->sb, n, key_bits(s), s->st);
+        launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, s->st);
+        radix_sort_pairs(s->sb, n, key_bits(s), s->st, true);
         launch_reorder(s->pos, s->vel, s->sb.idx, s->posTmp, s->velTmp, n, s->st);
     });
     if (s->borrowed) {

@@ -63,8 +63,6 @@ struct SortBuffers {
     uint32_t* chunk;    // 256 digit totals of the current pass
     int maxTiles;
 };
-void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
-                      hipStream_t st);
 // Keys of many particle sets held in one array: job j covers particles
 // base .. base + n - 1 and its keys get `tag` above the cell key bits, so a
 // stable sort keeps every set in place and orders each set as its own sort would.
@@ -79,9 +77,14 @@ void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* pr
                             int totalBlocks, uint32_t* keys, int32_t* idx, hipStream_t st);
 int sort_tiles_for(int n);
 int sort_chunks_for(int n);  // entries of SortBuffers::chunk
+// Cell keys (mode 0 row-major, 1 Morton cell, 2 Morton CIC base node; key =
+// cell + 1, 0 for an inactive slot) and identity values into buf.keys /
+// buf.idx, and the per-tile histogram of the first radix digit into buf.hist.
+void launch_cell_keys_hist(const float4* pos, int n, int G, int mode, SortBuffers& buf, hipStream_t st);
 // Stable LSD sort of (keys, idx) by the low `bits` bits. The result is in
-// buf.keys / buf.idx on return.
-void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st);
+// buf.keys / buf.idx on return. firstHistReady: buf.hist already holds the
+// first pass's histogram (launch_cell_keys_hist).
+void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady = false);
 void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
                     float4* dstV, int n, hipStream_t st);
 // Cell table of sorted keys (key = cell + 1, 0 = inactive), when range is not

@@ -33,16 +33,10 @@ constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;
 constexpr int SCAN_CHUNK = 1024;  // hist entries per scan block
 constexpr int WAVES = SORT_THREADS / 64;
 
-__global__ void __launch_bounds__(256) k_cell_keys(const float4* __restrict__ pos, int n, int G, int mode,
-                                                   uint32_t* __restrict__ keys, int32_t* __restrict__ idx) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
-    idx[i] = i;
-    if (p.x < 0.f) {
-        keys[i] = 0u;
-        return;
-    }
+// Key of one particle: cell (mode 0 row-major, 1 Morton) or CIC base node
+// (mode 2, Morton), plus one; 0 for an inactive slot.
+__device__ __forceinline__ uint32_t cell_key(float4 p, int G, int mode) {
+    if (p.x < 0.f) return 0u;
     const float fG = (float)G;
     int cx, cy, cz;
     if (mode == 2) {
@@ -57,11 +51,11 @@ __global__ void __launch_bounds__(256) k_cell_keys(const float4* __restrict__ po
     }
     uint32_t k = mode ? morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)
                       : (uint32_t)((cz * G + cy) * G + cx);
-    keys[i] = k + 1u;
+    return k + 1u;
 }
 
 // Keys of many particle sets in one launch (see KeyJob): the key of set j is
-// its own k_cell_keys key with the set's tag above it; idx is the global index.
+// its own cell key with the set's tag above it; idx is the global index.
 __global__ void __launch_bounds__(256) k_cell_keys_batch(const float4* __restrict__ pos,
                                                          const KeyJob* __restrict__ jobs,
                                                          const int* __restrict__ prefix, int njobs,
@@ -72,27 +66,8 @@ __global__ void __launch_bounds__(256) k_cell_keys_batch(const float4* __restric
     const int li = ((int)blockIdx.x - prefix[j]) * 256 + (int)threadIdx.x;
     if (li >= J.n) return;
     const int i = J.base + li;
-    const float4 p = pos[i];
     idx[i] = i;
-    if (p.x < 0.f) {
-        keys[i] = J.tag;
-        return;
-    }
-    const int G = J.G;
-    const float fG = (float)G;
-    int cx, cy, cz;
-    if (J.mode == 2) {
-        cx = min(max((int)floorf(fmaf(p.x, fG, -0.5f)), 0), G - 1);
-        cy = min(max((int)floorf(fmaf(p.y, fG, -0.5f)), 0), G - 1);
-        cz = min(max((int)floorf(fmaf(p.z, fG, -0.5f)), 0), G - 1);
-    } else {
-        cx = min(max((int)(p.x * fG), 0), G - 1);
-        cy = min(max((int)(p.y * fG), 0), G - 1);
-        cz = min(max((int)(p.z * fG), 0), G - 1);
-    }
-    const uint32_t k = J.mode ? morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)
-                              : (uint32_t)((cz * G + cy) * G + cx);
-    keys[i] = J.tag | (k + 1u);
+    keys[i] = J.tag | cell_key(pos[i], J.G, J.mode);
 }
 
 // Lanes of this wavefront whose digit equals mine (8-bit digits).
@@ -127,6 +102,36 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __re
             const int lane = threadIdx.x & 63;
             // highest lane of the group adds the group size
             if ((m >> lane) == 1ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Keys and identity values of one sort tile, plus the tile's histogram of the
+// first radix digit (what k_sort_hist would count for shift 0).
+__global__ void __launch_bounds__(SORT_THREADS) k_cell_keys_hist(const float4* __restrict__ pos, int n, int G,
+                                                                 int mode, uint32_t* __restrict__ keys,
+                                                                 int32_t* __restrict__ idx,
+                                                                 uint32_t* __restrict__ hist, int ntiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int i = base + r * SORT_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t d = 0u;
+        if (valid) {
+            const uint32_t k = cell_key(pos[i], G, mode);
+            keys[i] = k;
+            idx[i] = i;
+            d = k & 255u;
+        }
+        const uint64_t active = __ballot(valid);
+        if (valid) {
+            const uint64_t m = match_digit(d, active);
+            if ((m >> (threadIdx.x & 63)) == 1ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
         }
     }
     __syncthreads();
@@ -330,11 +335,6 @@ __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-void launch_cell_keys(const float4* pos, int n, int G, int mode, uint32_t* keys, int32_t* idx,
-                      hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, pos, n, G, mode, keys, idx);
-}
 
 void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* prefix, int njobs,
                             int totalBlocks, uint32_t* keys, int32_t* idx, hipStream_t st) {
@@ -345,7 +345,14 @@ void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* pr
 int sort_tiles_for(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 int sort_chunks_for(int) { return 256; }  // the digit totals
 
-void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
+void launch_cell_keys_hist(const float4* pos, int n, int G, int mode, SortBuffers& buf, hipStream_t st) {
+    if (n <= 0) return;
+    const int ntiles = sort_tiles_for(n);
+    hipLaunchKernelGGL(k_cell_keys_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, pos, n, G, mode, buf.keys,
+                       buf.idx, buf.hist, ntiles);
+}
+
+void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady) {
     if (n <= 1) return;
     const int ntiles = sort_tiles_for(n);
     // Keys sparse in their code space (under one key per two codes) spread each
@@ -356,8 +363,9 @@ void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st) {
     // 38/34/33 ordered.)
     const bool sparse = SD_SORT_LOCAL && 2.0 * (double)n < std::ldexp(1.0, bits);
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
-                           buf.hist, ntiles);
+        if (shift > 0 || !firstHistReady)
+            hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
+                               buf.hist, ntiles);
         hipLaunchKernelGGL(k_scan_digit_rows, dim3(256), dim3(256), 0, st, buf.hist, ntiles, buf.chunk);
         if (sparse && shift + 8 < bits)
             hipLaunchKernelGGL(k_sort_scatter<true>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
