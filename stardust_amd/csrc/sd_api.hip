@@ -336,9 +336,9 @@ static int key_bits(const sd_sim* s) {
 
 // Keys and the stable radix sort only: the permutation is left in s->sb.idx
 // for an integrate launch that reads through it (the fused reorder).
-static void sort_keys(sd_sim* s) {
+static void sort_keys(sd_sim* s, bool forContact = false) {
     const int n = s->allocN;
-    const int kmode = s->modes.keyMode == SD_KEY_MORTON ? 2 : 0;
+    const int kmode = s->modes.keyMode == SD_KEY_MORTON ? (forContact ? 1 : 2) : 0;
     TIMED(s, "sort", {
         launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, s->st);
         radix_sort_pairs(s->sb, n, key_bits(s), s->st, true);
@@ -449,20 +449,24 @@ static void ensure_mesh(sd_sim* s, bool grad = true) {
 }
 
 // doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685)
-static bool contact_pass(sd_sim* s) {
+// keysOnly: sort the keys only and read the particles through the permutation
+// (the integrate that follows does the reorder itself); the forces are the same.
+static bool contact_pass(sd_sim* s, bool keysOnly) {
     const int G = s->G;
     if (!s->cfg.contactEnabled) return false;
     if ((double)s->allocN > 0.764 * (double)G * G * G) return false;
     if (!ensure_contact_buffers(s)) return false;
-    sort_particles(s, true);
+    if (keysOnly)
+        sort_keys(s, true);
+    else
+        sort_particles(s, true);
     TIMED(s, "contact", {
         const bool morton = s->modes.keyMode == SD_KEY_MORTON;
         int2* blocks = morton ? s->blockRange : nullptr;
         launch_cell_range(s->sb.keys, s->allocN, morton ? nullptr : s->cellRange, blocks, s->blockCells, s->st);
-        launch_contact(s->pos, s->vel, s->allocN, G, s->periodic, s->modes.keyMode, s->cellRange, s->sb.keys,
-                       blocks,
-                       s->cfg.contactRadiusCells / (float)G, s->cfg.contactStiffness, s->cfg.contactDamping,
-                       s->accContact, s->st);
+        launch_contact(s->pos, s->vel, keysOnly ? s->sb.idx : nullptr, s->allocN, G, s->periodic,
+                       s->modes.keyMode, s->cellRange, s->sb.keys, blocks, s->cfg.contactRadiusCells / (float)G,
+                       s->cfg.contactStiffness, s->cfg.contactDamping, s->accContact, s->st);
     });
     return true;
 }
@@ -573,12 +577,16 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     // first deposit from unsorted positions would contend on the same cells.)
     const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
                              s->modes.depositMode == SD_DEPOSIT_FIXED;
+    const bool contactDue = s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)s->G * s->G * s->G;
+    // Contact steps reorder inside the integrate too (same conditions): the
+    // contact kernel reads the particles through the permutation.
+    const bool fuseContact = SD_FUSE_REORDER && contactDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
+                             s->modes.depositMode == SD_DEPOSIT_FIXED;
     // The mesh reads only the deposited density and the sort / contact chain
     // only the particles, so on such steps the mesh runs on a second stream
     // beside them and joins before the integrate. (Not with a learned
     // correction, which reads the density that the mesh's first pass clears,
     // nor while the density still has to be deposited from the particles.)
-    const bool contactDue = s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)s->G * s->G * s->G;
     const bool overlap = SD_OVERLAP_MESH && fused && (sortDue || contactDue) && !s->mlp && !s->borrowed &&
                          s->rhoValid && !s->meshValid && s->modes.depositMode == SD_DEPOSIT_FIXED && s->st2;
     if (overlap) {
@@ -609,7 +617,8 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         ensure_rho(s);
         solve_mesh(s, s->st, true);
     }
-    const bool contact = contact_pass(s);
+    const bool contact = contact_pass(s, fuseContact);
+    const bool reorderInIntegrate = fuseReorder || (contact && fuseContact);
     if (overlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
     if (timed) HK(hipEventRecord(s->ev1, s->st));
     flush_finalize(s);  // the scalars must be final before this integrate reads them
@@ -639,7 +648,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         const bool fusedDeposit = s->modes.depositMode == SD_DEPOSIT_FIXED;
         a.depositTo = fusedDeposit ? s->rhoFix : nullptr;
         a.partials = s->partials;
-        if (fuseReorder) {
+        if (reorderInIntegrate) {
             a.order = s->sb.idx;
             a.srcPos = s->pos;
             a.srcVel = s->vel;
@@ -647,7 +656,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
             a.vel = s->velTmp;
         }
         TIMED(s, "integrate", launch_integrate(a, s->st));
-        if (fuseReorder) {
+        if (reorderInIntegrate) {
             std::swap(s->pos, s->posTmp);
             std::swap(s->vel, s->velTmp);
         }

@@ -163,10 +163,12 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
 
 // ---- contact ------------------------------------------------------------------
 // keyMode 0 (row-major keys) reads the cell table, keyMode 1 (Morton keys) the
-// 4^3-block table and the sorted keys only (cellRange unused).
-void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
-                    float stiffness, float damping, float4* accOut, hipStream_t st);
+// 4^3-block table and the sorted keys only (cellRange unused). order: nullptr
+// when pos / vel are in sorted order, else the sort's permutation (sorted
+// particle j in slot order[j]); accOut is in sorted order either way.
+void launch_contact(const float4* pos, const float4* vel, const int32_t* order, int n, int G, int periodic,
+                    int keyMode, const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange,
+                    float radius, float stiffness, float damping, float4* accOut, hipStream_t st);
 
 // ---- direct sum -------------------------------------------------------------
 void launch_direct_sum(const float4* pos, int n, const int32_t* targets, int nt, float eps2,

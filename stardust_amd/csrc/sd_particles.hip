@@ -932,15 +932,16 @@ __device__ __forceinline__ int2 run_of_key(const uint32_t* __restrict__ keys, in
 
 // One cell's candidates [s0, e0) in the reference's order (kContact, Sim.cu:1063-1100):
 // each counts toward the cap of 96 (self excluded), in range or not.
-__device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, const float4* __restrict__ vel, int i,
-                                             float4 p, float4 v, int s0, int e0, int periodic, float d0, float d02,
-                                             float stiffness, float damping, int& seen, float& ax, float& ay,
-                                             float& az) {
+__device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                             const int32_t* __restrict__ order, int i, float4 p, float4 v, int s0,
+                                             int e0, int periodic, float d0, float d02, float stiffness,
+                                             float damping, int& seen, float& ax, float& ay, float& az) {
     constexpr int kMaxPeers = 96;
     for (int j = s0; j < e0 && seen < kMaxPeers; ++j) {
         if (j == i) continue;
         ++seen;
-        const float4 q = pos[j];
+        const int sj = order ? order[j] : j;  // storage slot of sorted particle j
+        const float4 q = pos[sj];
         if (q.x < 0.f) continue;
         float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
         if (periodic) {
@@ -952,7 +953,7 @@ __device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, con
         if (dd >= d02 || dd < 1e-16f) continue;
         const float d = sqrtf(dd);
         const float nx = ex / d, ny = ey / d, nz = ez / d;
-        const float4 vj = vel[j];
+        const float4 vj = vel[sj];
         const float vn = fmaf(v.z - vj.z, nz, fmaf(v.y - vj.y, ny, (v.x - vj.x) * nx));
         float f = 0.f;
         if (d < d0) {
@@ -985,21 +986,24 @@ __device__ __forceinline__ void contact_cell(const float4* __restrict__ pos, con
 // Isolated boxes take the second visit at their edge cells, where the clamped
 // neighbour folds onto the particle's own cell (periodic meshes below 3 cells
 // likewise).
+// order (optional): the sort's permutation, sorted particle j living in slot
+// order[j] of pos / vel (the arrays not yet reordered); accOut is in sorted order.
 template <bool BLOCKS>
-__global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel, int n,
-                                               int G, int periodic, const int2* __restrict__ range,
-                                               const uint32_t* __restrict__ keys, const int2* __restrict__ blockRange,
-                                               float radius, float stiffness, float damping,
-                                               float4* __restrict__ accOut) {
+__global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                               const int32_t* __restrict__ order, int n, int G, int periodic,
+                                               const int2* __restrict__ range, const uint32_t* __restrict__ keys,
+                                               const int2* __restrict__ blockRange, float radius, float stiffness,
+                                               float damping, float4* __restrict__ accOut) {
     const int i = blockIdx.x * PT + threadIdx.x;
     if (i >= n) return;
-    const float4 p = pos[i];
+    const int si = order ? order[i] : i;
+    const float4 p = pos[si];
     float ax = 0.f, ay = 0.f, az = 0.f;
     if (p.x < 0.f) {
         accOut[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    const float4 v = vel[i];
+    const float4 v = vel[si];
     const float d0 = radius * 2.0f;
     const float d02 = d0 * d0;
     const float fG = (float)G;
@@ -1040,7 +1044,7 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
                     if (br.y <= br.x) continue;
                     const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
                     if (r.y <= r.x) continue;
-                    contact_cell(pos, vel, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
+                    contact_cell(pos, vel, order, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
                 }
     } else {
         for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz) {
@@ -1069,23 +1073,23 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
                     seen = min(seen + (e0 - s0), kMaxPeers);  // out of reach: the particle is not in this cell
                     continue;
                 }
-                contact_cell(pos, vel, i, p, v, s0, e0, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
+                contact_cell(pos, vel, order, i, p, v, s0, e0, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
             }
         }
     }
     accOut[i] = make_float4(ax, ay, az, 0.f);
 }
 
-void launch_contact(const float4* pos, const float4* vel, int n, int G, int periodic, int keyMode,
-                    const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange, float radius,
-                    float stiffness, float damping, float4* accOut, hipStream_t st) {
+void launch_contact(const float4* pos, const float4* vel, const int32_t* order, int n, int G, int periodic,
+                    int keyMode, const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange,
+                    float radius, float stiffness, float damping, float4* accOut, hipStream_t st) {
     if (n <= 0) return;
     if (keyMode)
-        hipLaunchKernelGGL(k_contact<true>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic,
-                           nullptr, sortedKeys, blockRange, radius, stiffness, damping, accOut);
+        hipLaunchKernelGGL(k_contact<true>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, order, n, G,
+                           periodic, nullptr, sortedKeys, blockRange, radius, stiffness, damping, accOut);
     else
-        hipLaunchKernelGGL(k_contact<false>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, n, G, periodic,
-                           cellRange, sortedKeys, nullptr, radius, stiffness, damping, accOut);
+        hipLaunchKernelGGL(k_contact<false>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, order, n, G,
+                           periodic, cellRange, sortedKeys, nullptr, radius, stiffness, damping, accOut);
 }
 
 // ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------

@@ -204,6 +204,38 @@ def test_fused_pipeline_matches_reference_order(gpu, oracle, integrator):
     assert abs(t0 - t1) < 1e-9
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fused_contact_matches_reference_order(gpu, periodic):
+    """Contact steps of the fused pipeline sort the keys only: the contact
+    kernel reads the particles through the permutation and the integrate
+    reorders them. Against the reference-order pipeline (sort with reorder,
+    contact on sorted arrays) under a fixed dt, clumps reaching the 96 cap."""
+    _say("fused contact", periodic)
+    n, G, steps = 30000, 32, 6
+    pos, vel = random_particles(n, seed=12, inactive=0.02)
+    pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)  # identity carried in w
+    out = []
+    for pipe, integ in ((sd.PIPELINE_REFERENCE, sd.INTEGRATOR_KD), (sd.PIPELINE_FUSED, sd.INTEGRATOR_KD)):
+        cfg = sd.default_config(particleCount=n, gridSize=G,
+                                boundary=sd.BOUNDARY_PERIODIC if periodic else sd.BOUNDARY_ISOLATED,
+                                softeningCells=1.5, gravity=0.2, sortInterval=1, contactEnabled=1,
+                                contactRadiusCells=0.4, contactStiffness=1e3)
+        s = sd.Sim(cfg, sd.default_modes(pipeline=pipe, integrator=integ, keyMode=sd.KEY_MORTON))
+        s.set_fixed_dt(1e-3)
+        s.set_particles(pos, vel)
+        s.run(steps)
+        p, v = s.get_particles()
+        live = p[:, 0] >= 0
+        o = np.argsort(np.where(live, p[:, 3], np.inf))
+        out.append((p[o], v[o], int(live.sum())))
+        s.close()
+    (p0, v0, n0), (p1, v1, n1) = out
+    assert n0 == n1
+    assert np.array_equal(p0[:n0, 3], p1[:n1, 3])
+    assert np.abs(p0[:n0, :3] - p1[:n1, :3]).max() < 1e-6
+    assert np.abs(v0[:n0, :3] - v1[:n1, :3]).max() < 1e-5 * np.abs(v0[:n0, :3]).max()
+
+
 @pytest.mark.parametrize("G,n", [(128, 2_500_000), (64, 2_500_000), (32, 300_000), (128, 300_000)])
 def test_run_equals_single_steps(gpu, G, n):
     """Within a run the partials reduction of each step is carried by the next
