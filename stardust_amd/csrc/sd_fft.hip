@@ -36,11 +36,13 @@ __device__ __forceinline__ float2 rot90(float2 a) {
 }
 
 // ---- small DFTs in registers ---------------------------------------------
-template <int R, bool INV>
+// UZ: the upper half of the inputs is zero (zero padding), so the innermost
+// 4-point transforms drop the additions of zeros.
+template <int R, bool INV, bool UZ = false>
 struct Dft;
 
 template <bool INV>
-struct Dft<2, INV> {
+struct Dft<2, INV, false> {
     __device__ __forceinline__ static void run(float2* v) {
         const float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -49,7 +51,7 @@ struct Dft<2, INV> {
 };
 
 template <bool INV>
-struct Dft<3, INV> {
+struct Dft<3, INV, false> {
     __device__ __forceinline__ static void run(float2* v) {
         const float s3 = 0.86602540378443864676f;
         const float2 t = cadd(v[1], v[2]);
@@ -68,11 +70,11 @@ struct Dft<3, INV> {
     }
 };
 
-template <bool INV>
-struct Dft<4, INV> {
+template <bool INV, bool UZ>
+struct Dft<4, INV, UZ> {
     __device__ __forceinline__ static void run(float2* v) {
-        const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        const float2 t2 = cadd(v[1], v[3]), t3 = rot90<INV>(csub(v[1], v[3]));
+        const float2 t0 = UZ ? v[0] : cadd(v[0], v[2]), t1 = UZ ? v[0] : csub(v[0], v[2]);
+        const float2 t2 = UZ ? v[1] : cadd(v[1], v[3]), t3 = rot90<INV>(UZ ? v[1] : csub(v[1], v[3]));
         v[0] = cadd(t0, t2);
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
@@ -80,14 +82,14 @@ struct Dft<4, INV> {
     }
 };
 
-template <bool INV>
-struct Dft<8, INV> {
+template <bool INV, bool UZ>
+struct Dft<8, INV, UZ> {
     __device__ __forceinline__ static void run(float2* v) {
         const float c = 0.70710678118654752440f;
         float2 e[4] = {v[0], v[2], v[4], v[6]};
         float2 o[4] = {v[1], v[3], v[5], v[7]};
-        Dft<4, INV>::run(e);
-        Dft<4, INV>::run(o);
+        Dft<4, INV, UZ>::run(e);
+        Dft<4, INV, UZ>::run(o);
         // o[k] *= W8^k, W8 = exp(-+ i pi/4)
         const float sg = INV ? 1.f : -1.f;
         o[1] = cmul(o[1], make_float2(c, sg * c));
@@ -101,8 +103,8 @@ struct Dft<8, INV> {
     }
 };
 
-template <bool INV>
-struct Dft<16, INV> {
+template <bool INV, bool UZ>
+struct Dft<16, INV, UZ> {
     __device__ __forceinline__ static void run(float2* v) {
         float2 e[8], o[8];
 #pragma unroll
@@ -110,8 +112,8 @@ struct Dft<16, INV> {
             e[k] = v[2 * k];
             o[k] = v[2 * k + 1];
         }
-        Dft<8, INV>::run(e);
-        Dft<8, INV>::run(o);
+        Dft<8, INV, UZ>::run(e);
+        Dft<8, INV, UZ>::run(o);
         const float sg = INV ? 1.f : -1.f;
         // W16^k = (cos(pi k/8), -+ sin(pi k/8))
         const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
@@ -162,18 +164,26 @@ __device__ __forceinline__ void line_sync() {
 // tw[m] = exp(-2 pi i m / N) (FP64-computed table); INV uses the conjugate.
 // The last stage of a transform ends with a block barrier: what follows reads
 // the lines with another thread mapping.
-template <int N, int R, int NS, int T, bool INV, bool LAST = false>
+// ZIN (first stage, NS = 1): elements N/2 .. N-1 of the line are zero padding,
+// i.e. inputs r >= R/2 of every butterfly; they are neither read nor added.
+// HOUT (last stage, NS = N/R): only elements 0 .. N/2-1 are used afterwards,
+// i.e. outputs r < R/2; the others are neither stored nor computed.
+template <int N, int R, int NS, int T, bool INV, bool LAST = false, bool ZIN = false, bool HOUT = false>
 __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* __restrict__ tw,
                                       int t) {
     constexpr int NBF = N / R;
     constexpr int PER = (NBF + T - 1) / T;
+    static_assert(!ZIN || (NS == 1 && R % 2 == 0), "ZIN: first stage only");
+    static_assert(!HOUT || (NS * R == N && R % 2 == 0), "HOUT: last stage only");
     float2 v[PER][R];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int j = t + q * T;
         if (PER * T == NBF || j < NBF) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = line[PADI(j + r * NBF)];
+            for (int r = 0; r < (ZIN ? R / 2 : R); ++r) v[q][r] = line[PADI(j + r * NBF)];
+#pragma unroll
+            for (int r = (ZIN ? R / 2 : R); r < R; ++r) v[q][r] = make_float2(0.f, 0.f);
         }
     }
     line_sync<T>();
@@ -190,10 +200,10 @@ __device__ __forceinline__ void stage(float2* __restrict__ line, const float2* _
                     v[q][r] = cmul(v[q][r], w);
                 }
             }
-            Dft<R, INV>::run(v[q]);
+            Dft<R, INV, ZIN>::run(v[q]);
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[PADI(base + r * NS)] = v[q][r];
+            for (int r = 0; r < (HOUT ? R / 2 : R); ++r) line[PADI(base + r * NS)] = v[q][r];
         }
     }
     if (LAST)
@@ -257,10 +267,10 @@ struct Plan<192> {
 template <>
 struct Plan<256> {
     static constexpr int T = 16;
-    template <bool INV>
+    template <bool INV, bool ZIN = false, bool HOUT = false>
     __device__ static void run(float2* l, const float2* tw, int t) {
-        stage<256, 16, 1, T, INV>(l, tw, t);
-        stage<256, 16, 16, T, INV, true>(l, tw, t);
+        stage<256, 16, 1, T, INV, false, ZIN>(l, tw, t);
+        stage<256, 16, 16, T, INV, true, false, HOUT>(l, tw, t);
     }
 };
 template <>
@@ -582,6 +592,30 @@ struct ColArgs {
     FinalizeJob fin;     // fused z pass: reduction carried by block (0, 0) (fin.partials == nullptr: none)
 };
 
+// One line transform; halfIn: the upper half of the line is zero, halfOut:
+// only its lower half is read afterwards (pruned stages where the plan has them).
+#ifndef SD_FFT_PRUNE
+#define SD_FFT_PRUNE 1
+#endif
+template <int N, bool INV>
+__device__ __forceinline__ void run_line(float2* l, const float2* tw, int t, bool halfIn, bool halfOut) {
+    if constexpr (SD_FFT_PRUNE && N == 256) {
+        if (halfIn && halfOut) {
+            Plan<N>::template run<INV, true, true>(l, tw, t);
+            return;
+        }
+        if (halfIn) {
+            Plan<N>::template run<INV, true, false>(l, tw, t);
+            return;
+        }
+        if (halfOut) {
+            Plan<N>::template run<INV, false, true>(l, tw, t);
+            return;
+        }
+    }
+    Plan<N>::template run<INV>(l, tw, t);
+}
+
 template <int N, int MODE>
 __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
     constexpr int LP = Line<N>::LP;
@@ -625,10 +659,12 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
     float2* line = s_line + c * LP;
+    // zero-padded lines (isolated meshes: G of S = 2G rows loaded) and
+    // half-stored results take the pruned first / last stage
     if (MODE == COL_INV) {
-        Plan<N>::template run<true>(line, s_tw, t);
+        run_line<N, true>(line, s_tw, t, false, 2 * a.nout <= N);
     } else {
-        Plan<N>::template run<false>(line, s_tw, t);
+        run_line<N, false>(line, s_tw, t, 2 * a.nin <= N, false);
     }
     if (MODE == COL_GREEN) {
         const int h = N / 2;
@@ -680,7 +716,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
             s_line[(col + 1) * LP + PADI(kz)] = make_float2(v.x * m1, v.y * m1);
         }
         __syncthreads();
-        Plan<N>::template run<true>(line, s_tw, t);
+        run_line<N, true>(line, s_tw, t, false, 2 * a.nout <= N);
     }
     float2* out = a.out + (size_t)slab * a.outA + b0 + col;
 #pragma unroll
