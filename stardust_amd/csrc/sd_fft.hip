@@ -347,6 +347,31 @@ struct XPass {
 using XFwd = XPass<SD_FFT_RPL_FWD, SD_FFT_LINES_FWD>;
 using XInv = XPass<SD_FFT_RPL_INV, SD_FFT_LINES_INV>;
 
+#ifndef SD_FFT_PRUNE
+#define SD_FFT_PRUNE 1
+#endif
+#ifndef SD_FFT_PRUNE_ROWS
+#define SD_FFT_PRUNE_ROWS 1
+#endif
+template <int N, bool INV>
+__device__ __forceinline__ void run_line(float2* l, const float2* tw, int t, bool halfIn, bool halfOut) {
+    if constexpr (SD_FFT_PRUNE && N == 256) {
+        if (halfIn && halfOut) {
+            Plan<N>::template run<INV, true, true>(l, tw, t);
+            return;
+        }
+        if (halfIn) {
+            Plan<N>::template run<INV, true, false>(l, tw, t);
+            return;
+        }
+        if (halfOut) {
+            Plan<N>::template run<INV, false, true>(l, tw, t);
+            return;
+        }
+    }
+    Plan<N>::template run<INV>(l, tw, t);
+}
+
 template <int N, int SRC>
 __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2* __restrict__ tw, int bx) {
     constexpr int LP = Line<N>::LP;
@@ -417,7 +442,7 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
     }
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
-    Plan<N>::template run<false>(s_line + c * LP, s_tw, t);
+    run_line<N, false>(s_line + c * LP, s_tw, t, SD_FFT_PRUNE_ROWS && 2 * a.nin <= N, false);  // zero-padded rows
     // store k = 0 .. N/2 as complex pairs (the pair's second half at N/2 + 1
     // lands in the row padding)
     constexpr int KP2 = N / 4 + 1;
@@ -543,7 +568,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2
     }
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
-    Plan<N>::template run<true>(s_line + c * LP, s_tw, t);
+    run_line<N, true>(s_line + c * LP, s_tw, t, false, SD_FFT_PRUNE_ROWS && 2 * a.nout <= N);  // only x < nout kept
     const int q4 = a.nout / 4;
     for (int i = threadIdx.x; i < TB * q4; i += THREADS) {
         const int r = i / q4, j = i - r * q4;
@@ -594,28 +619,6 @@ struct ColArgs {
 
 // One line transform; halfIn: the upper half of the line is zero, halfOut:
 // only its lower half is read afterwards (pruned stages where the plan has them).
-#ifndef SD_FFT_PRUNE
-#define SD_FFT_PRUNE 1
-#endif
-template <int N, bool INV>
-__device__ __forceinline__ void run_line(float2* l, const float2* tw, int t, bool halfIn, bool halfOut) {
-    if constexpr (SD_FFT_PRUNE && N == 256) {
-        if (halfIn && halfOut) {
-            Plan<N>::template run<INV, true, true>(l, tw, t);
-            return;
-        }
-        if (halfIn) {
-            Plan<N>::template run<INV, true, false>(l, tw, t);
-            return;
-        }
-        if (halfOut) {
-            Plan<N>::template run<INV, false, true>(l, tw, t);
-            return;
-        }
-    }
-    Plan<N>::template run<INV>(l, tw, t);
-}
-
 template <int N, int MODE>
 __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
     constexpr int LP = Line<N>::LP;
