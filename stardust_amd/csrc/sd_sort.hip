@@ -356,8 +356,9 @@ void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool fi
     if (n <= 1) return;
     const int ntiles = sort_tiles_for(n);
     // Keys sparse in their code space (under one key per two codes) spread each
-    // round over many digits: order the tile in LDS first, except in the last
-    // pass, whose few digit values come in long runs. (G=512 cosmic web, 16.7 M
+    // round over many digits, and so do the lowest digits of any keys: order
+    // the tile in LDS first there, except in the last pass, whose few digit
+    // values come in long runs. (G=512 cosmic web, 16.7 M
     // keys of 27 bits: 214/342/365/86 us per pass direct, 136/133/128/104
     // ordered; two-disk G=128, 4.2 M keys of 21 bits: 43/24/23 direct,
     // 38/34/33 ordered.)
@@ -367,7 +368,7 @@ void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool fi
             hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
                                buf.hist, ntiles);
         hipLaunchKernelGGL(k_scan_digit_rows, dim3(256), dim3(256), 0, st, buf.hist, ntiles, buf.chunk);
-        if (sparse && shift + 8 < bits)
+        if (SD_SORT_LOCAL && (sparse || shift == 0) && shift + 8 < bits)
             hipLaunchKernelGGL(k_sort_scatter<true>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
                                shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt);
         else
