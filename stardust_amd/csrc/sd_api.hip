@@ -511,7 +511,9 @@ static bool mlp_stage(sd_mlp* m, int rows) {
 
 // Runs the network on every particle of the current positions (needs the
 // fixed-point density of those positions) into s->accCorr.
-static bool mlp_correction(sd_sim* s) {
+// order: nullptr, or the permutation of a sort whose reorder the next
+// integrate does (the correction comes out in sorted order either way)
+static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     if (!s->mlp) return false;
     if (!s->accCorr && !dalloc(s->accCorr, (size_t)s->allocN)) return false;
     const bool hadRho = s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED;
@@ -522,6 +524,7 @@ static bool mlp_correction(sd_sim* s) {
     }
     MlpInferArgs a{};
     a.pos = s->pos;
+    a.order = order;
     a.n = s->allocN;
     a.rho = s->rhoFix;
     a.G = s->G;
@@ -569,13 +572,13 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     if (timed) HK(hipEventRecord(s->evA, s->st));
     const bool sortDue = s->cfg.sortInterval > 0 && (s->stepCount % s->cfg.sortInterval) == 0 &&
                          !s->cfg.contactEnabled;
-    // With nothing between the sort and the integrate that reads particles in
-    // sorted order (learned correction, contact), the integrate launch does the
-    // reorder itself: it reads through the permutation and writes the sorted
-    // arrays, which saves the separate gather pass.
+    // On a sort step the integrate launch does the reorder itself: it reads
+    // through the permutation and writes the sorted arrays, which saves the
+    // separate gather pass; the learned correction before it reads the
+    // particles through the permutation too.
     // (Only when the density of the current positions is already deposited: a
     // first deposit from unsorted positions would contend on the same cells.)
-    const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
+    const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->borrowed && s->rhoValid &&
                              s->modes.depositMode == SD_DEPOSIT_FIXED;
     const bool contactDue = s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)s->G * s->G * s->G;
     // Contact steps reorder inside the integrate too (same conditions): the
@@ -604,7 +607,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     bool corr = false;
     if (s->mlp) {
         if (!s->corrValid || s->corrAge >= s->mlpInterval) {
-            corr = mlp_correction(s);
+            corr = mlp_correction(s, fuseReorder ? s->sb.idx : nullptr);
             s->corrValid = corr;
             s->corrAge = 0;
         } else {

@@ -34,6 +34,7 @@ struct MlpDev {
 
 struct MlpInferArgs {
     const float4* pos;
+    const int32_t* order;  // nullptr, or the sort's permutation: particle i is pos[order[i]]
     int n;
     const long long* rho;  // fixed-point density of the current positions
     int G;

@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         __asm__ volatile("" ::: "memory");
         const int i = tile * 32 + (l & 31);
         float4 p = make_float4(-1.f, -1.f, -1.f, 0.f);
-        if (i < a.n) p = a.pos[i];
+        if (i < a.n) p = a.pos[a.order ? a.order[i] : i];
         infx8 B[KH / 16];
         if constexpr (BIAS_MM) {
             // inputs of the bias k-step: k = 128, 129 are 1 (lanes 0-31 hold k = 128..135)

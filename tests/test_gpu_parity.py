@@ -236,6 +236,41 @@ def test_fused_contact_matches_reference_order(gpu, periodic):
     assert np.abs(v0[:n0, :3] - v1[:n1, :3]).max() < 1e-5 * np.abs(v0[:n0, :3]).max()
 
 
+def test_fused_learned_correction_matches_reference_order(gpu):
+    """Sort steps with the learned correction: the fused pipeline sorts keys
+    only, the MLP reads the particles through the permutation and the
+    integrate reorders; the reference-order pipeline reorders first. Same
+    network (seeded weights), refreshed every step, fixed dt."""
+    _say("fused learned")
+    n, G, steps = 40000, 32, 6
+    pos, vel = random_particles(n, seed=14, inactive=0.0)
+    pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)  # identity carried in w
+    mlp = sd.Mlp(seed=5)
+    out = []
+    for pipe, use_mlp in ((sd.PIPELINE_REFERENCE, True), (sd.PIPELINE_FUSED, True), (sd.PIPELINE_FUSED, False)):
+        s = make_sim(n, G, soft=1.5, gravity=0.3, sort_interval=2, pipeline=pipe,
+                     integrator=sd.INTEGRATOR_KD, keyMode=sd.KEY_MORTON)
+        s.set_fixed_dt(1e-3)
+        s.set_particles(pos, vel)
+        if use_mlp:
+            s.attach_mlp(mlp, 2000.0)  # seeded weights give small outputs: scale them up
+            s.set_mlp_interval(1)
+        s.run(steps)
+        p, v = s.get_particles()
+        if use_mlp:
+            s.attach_mlp(None)
+        o = np.argsort(p[:, 3])
+        out.append((p[o], v[o]))
+        s.close()
+    (p0, v0), (p1, v1), (p2, v2) = out
+    assert np.array_equal(p0[:, 3], p1[:, 3])
+    assert np.abs(p0[:, :3] - p1[:, :3]).max() < 1e-6
+    assert np.abs(v0[:, :3] - v1[:, :3]).max() < 1e-5 * np.abs(v0[:, :3]).max()
+    # the correction moves the particles far beyond that tolerance (a correction
+    # applied to the wrong particles would show)
+    assert np.abs(v1[:, :3] - v2[:, :3]).max() > 1e-3 * np.abs(v1[:, :3]).max()
+
+
 @pytest.mark.parametrize("G,n", [(128, 2_500_000), (64, 2_500_000), (32, 300_000), (128, 300_000)])
 def test_run_equals_single_steps(gpu, G, n):
     """Within a run the partials reduction of each step is carried by the next
