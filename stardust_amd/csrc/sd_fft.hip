@@ -16,6 +16,10 @@
 //   X' rows     : A -> pot[z][y][x]                                  (C2R, only x < G kept;
 //                                                                    two rows per complex line)
 //
+// At S=256 the transforms are pruned too: a forward line's first radix stage
+// skips the zero-padded inputs, an inverse line's last stage the outputs that
+// are not kept.
+//
 // Complex arrays keep kx (S/2+1 values) contiguous with a row pitch P that is
 // a multiple of 16 complex (128 B), so every column tile of 16 is one cache
 // line. The isolated Green spectrum is real and even in each axis; only its

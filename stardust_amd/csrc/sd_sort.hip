@@ -13,10 +13,13 @@
 // reproduces CUB's signed order (the -1 keys first) and only
 // ceil(log2(maxKey + 1)) bits are sorted. Each 8-bit pass is three launches:
 //   hist    : per-tile digit counts (wavefront ballot match -> one LDS add per
-//             run of equal digits) written digit-major,
-//   scan    : exclusive scan of the digit-major table (reduce / top / down),
-//   scatter : stable in-tile ranks (same ballot match, running per-digit
-//             counters across rounds and waves) -> global positions.
+//             run of equal digits) written digit-major; for the first pass
+//             the key kernel counts them (k_cell_keys_hist),
+//   scan    : exclusive scan of each digit's row of tile counts, a block per
+//             digit, and the digit totals,
+//   scatter : digit bases from the totals, stable in-tile ranks (same ballot
+//             match, running per-digit counters across rounds and waves) ->
+//             global positions.
 #include <cmath>
 
 #include "sd_device.h"
