@@ -98,6 +98,7 @@ def _declare(L):
     L.sdo_poisson_periodic.argtypes = [_dp, C.c_int, C.c_float, _dp]
     L.sdo_grid_accel.argtypes = [_dp, C.c_int, C.c_float, C.c_int, _fp]
     L.sdo_sample_acc.argtypes = [_fp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _fp]
+    L.sdo_sample_acc_all.argtypes = [_fp, _fp, C.c_int, C.c_int, C.c_int, _fp]
     L.sdo_max_speed.restype = C.c_float
     L.sdo_max_speed.argtypes = [_fp, _fp, C.c_int]
     L.sdo_integrate.argtypes = [_fp, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
@@ -241,14 +242,8 @@ def particle_accel(params, pos):
     pos = _p4(pos)
     acc = compute_accel(params, pos)
     flat = np.ascontiguousarray(acc, dtype=np.float32).reshape(-1)
-    out = np.zeros((pos.shape[0], 4), np.float32)
-    o3 = np.empty(3, np.float32)
-    L = lib()
-    G, per = params.G, int(params.periodic)
-    for i in range(pos.shape[0]):
-        if pos[i, 0] >= 0.0:
-            L.sdo_sample_acc(flat, float(pos[i, 0]), float(pos[i, 1]), float(pos[i, 2]), G, per, o3)
-            out[i, :3] = o3
+    out = np.empty((pos.shape[0], 4), np.float32)
+    lib().sdo_sample_acc_all(flat, pos, pos.shape[0], params.G, int(params.periodic), out)
     return out
 
 

@@ -497,6 +497,18 @@ void sdo_sample_acc(const float* acc4, float px, float py, float pz, int G, int 
     out3[2] = az;
 }
 
+/* sampleAcc at every live particle (zero for inactive slots): out4[n*4]. */
+void sdo_sample_acc_all(const float* acc4, const float* pos4, int n, int G, int periodic, float* out4) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const float* p = pos4 + 4 * (size_t)i;
+        float* o = out4 + 4 * (size_t)i;
+        o[0] = o[1] = o[2] = o[3] = 0.f;
+        if (p[0] < 0.f) continue;
+        sdo_sample_acc(acc4, p[0], p[1], p[2], G, periodic, o);
+    }
+}
+
 /* kMaxSpeed, Sim.cu:2923-2930: max |v| over live particles. */
 float sdo_max_speed(const float* pos4, const float* vel4, int n) {
     float m = 0.f;

@@ -56,6 +56,7 @@ void sdo_grid_accel(const double* pot, int G, float potScale, int periodic, floa
 /* --- particles ----------------------------------------------------------- */
 void sdo_sample_acc(const float* acc4, float px, float py, float pz, int G, int periodic,
                     float* out3);
+void sdo_sample_acc_all(const float* acc4, const float* pos4, int n, int G, int periodic, float* out4);
 float sdo_max_speed(const float* pos4, const float* vel4, int n);
 void sdo_integrate(const float* acc4, float* pos4, float* vel4, int n, int G, float kick,
                    float drift, int periodic, float c2, const float* accContact4,

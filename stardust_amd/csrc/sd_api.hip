@@ -448,27 +448,34 @@ static void ensure_mesh(sd_sim* s, bool grad = true) {
     solve_mesh(s, s->st, grad);
 }
 
-// doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685)
-// keysOnly: sort the keys only and read the particles through the permutation
-// (the integrate that follows does the reorder itself); the forces are the same.
-static bool contact_pass(sd_sim* s, bool keysOnly) {
+// doContact, Sim.cu:3673-3721 (contact term; gate N <= 0.764 G^3, Sim.cu:3685),
+// in two parts: the cell-key sort, then the forces. keysOnly: sort the keys
+// only and read the particles through the permutation (the integrate that
+// follows does the reorder itself); the forces are the same.
+static bool contact_due(const sd_sim* s) {
     const int G = s->G;
-    if (!s->cfg.contactEnabled) return false;
-    if ((double)s->allocN > 0.764 * (double)G * G * G) return false;
+    return s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)G * G * G;
+}
+
+static bool contact_sort(sd_sim* s, bool keysOnly) {
+    if (!contact_due(s)) return false;
     if (!ensure_contact_buffers(s)) return false;
     if (keysOnly)
         sort_keys(s, true);
     else
         sort_particles(s, true);
+    return true;
+}
+
+static void contact_forces(sd_sim* s, bool keysOnly) {
     TIMED(s, "contact", {
         const bool morton = s->modes.keyMode == SD_KEY_MORTON;
         int2* blocks = morton ? s->blockRange : nullptr;
         launch_cell_range(s->sb.keys, s->allocN, morton ? nullptr : s->cellRange, blocks, s->blockCells, s->st);
-        launch_contact(s->pos, s->vel, keysOnly ? s->sb.idx : nullptr, s->allocN, G, s->periodic,
-                       s->modes.keyMode, s->cellRange, s->sb.keys, blocks, s->cfg.contactRadiusCells / (float)G,
+        launch_contact(s->pos, s->vel, keysOnly ? s->sb.idx : nullptr, s->allocN, s->G, s->periodic,
+                       s->modes.keyMode, s->cellRange, s->sb.keys, blocks, s->cfg.contactRadiusCells / (float)s->G,
                        s->cfg.contactStiffness, s->cfg.contactDamping, s->accContact, s->st);
     });
-    return true;
 }
 
 // ---- learned correction -------------------------------------------------------
@@ -580,10 +587,10 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     // first deposit from unsorted positions would contend on the same cells.)
     const bool fuseReorder = SD_FUSE_REORDER && sortDue && fused && !s->borrowed && s->rhoValid &&
                              s->modes.depositMode == SD_DEPOSIT_FIXED;
-    const bool contactDue = s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)s->G * s->G * s->G;
+    const bool contactDue = contact_due(s);
     // Contact steps reorder inside the integrate too (same conditions): the
     // contact kernel reads the particles through the permutation.
-    const bool fuseContact = SD_FUSE_REORDER && contactDue && fused && !s->mlp && !s->borrowed && s->rhoValid &&
+    const bool fuseContact = SD_FUSE_REORDER && contactDue && fused && !s->borrowed && s->rhoValid &&
                              s->modes.depositMode == SD_DEPOSIT_FIXED;
     // The mesh reads only the deposited density and the sort / contact chain
     // only the particles, so on such steps the mesh runs on a second stream
@@ -604,10 +611,15 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         else
             sort_particles(s);
     }
+    // The contact sort runs before the learned correction: the correction is
+    // stored per slot in the order the integrate writes, so it must see the
+    // permutation of this step's last sort.
+    const bool contact = contact_sort(s, fuseContact);
+    const bool reorderInIntegrate = fuseReorder || (contact && fuseContact);
     bool corr = false;
     if (s->mlp) {
         if (!s->corrValid || s->corrAge >= s->mlpInterval) {
-            corr = mlp_correction(s, fuseReorder ? s->sb.idx : nullptr);
+            corr = mlp_correction(s, reorderInIntegrate ? s->sb.idx : nullptr);
             s->corrValid = corr;
             s->corrAge = 0;
         } else {
@@ -620,8 +632,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         ensure_rho(s);
         solve_mesh(s, s->st, true);
     }
-    const bool contact = contact_pass(s, fuseContact);
-    const bool reorderInIntegrate = fuseReorder || (contact && fuseContact);
+    if (contact) contact_forces(s, fuseContact);
     if (overlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
     if (timed) HK(hipEventRecord(s->ev1, s->st));
     flush_finalize(s);  // the scalars must be final before this integrate reads them

@@ -77,6 +77,7 @@ struct sd_batch {
     int recSteps = -1, recEvery = -1;
     long long recPhase = -1;
     std::vector<sd_config> recCfg;
+    std::vector<sd_modes> recModes;  // the graph bakes in each scene's modes too
     std::vector<float> recDt;
 };
 
@@ -343,9 +344,11 @@ static int record(sd_batch* b, int steps, int every) {
     b->recEvery = every;
     b->recPhase = phase;
     b->recCfg.clear();
+    b->recModes.clear();
     b->recDt.clear();
     for (sd_sim* s : b->scenes) {
         b->recCfg.push_back(s->cfg);
+        b->recModes.push_back(s->modes);
         b->recDt.push_back(s->fixedDt);
     }
     return SD_OK;
@@ -360,7 +363,9 @@ static bool recording_matches(const sd_batch* b, int steps, int every) {
         if ((b->recPhase % si) != (b->scenes[0]->stepCount % si)) return false;
     }
     for (size_t i = 0; i < b->scenes.size(); ++i)
-        if (memcmp(&b->recCfg[i], &b->scenes[i]->cfg, sizeof(sd_config)) != 0 || b->recDt[i] != b->scenes[i]->fixedDt)
+        if (memcmp(&b->recCfg[i], &b->scenes[i]->cfg, sizeof(sd_config)) != 0 ||
+            memcmp(&b->recModes[i], &b->scenes[i]->modes, sizeof(sd_modes)) != 0 ||
+            b->recDt[i] != b->scenes[i]->fixedDt)
             return false;
     return true;
 }

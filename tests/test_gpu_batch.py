@@ -117,3 +117,69 @@ def test_batch_scene_calls_and_refusals(gpu):
     for i in range(3):
         assert b.scene(i).measure_total_grid_mass() == pytest.approx(scenes[i][0].particleCount, abs=1e-6)
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_config3_meshes_vs_oracle(gpu, oracle):
+    """Batched disks on config 3's meshes (G = 64, 96 and 128, S = 2G) against
+    the oracle: each scene's leapfrog from v0 equals the oracle's kick-drift
+    (Sim::step, Morton keys) from v0 - a(x0) dt / 2, over a run that crosses a
+    sort (steps 0 and 5)."""
+    steps, dt = 7, 1e-3
+    specs = [(40000, 64, 21), (30000, 96, 22), (50000, 128, 23), (9000, 64, 24)]
+    scenes = []
+    for n, G, seed in specs:
+        d = ic.disk_scene(n, G, seed)
+        pos = d["pos"].copy()
+        pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)
+        cfg = sd.default_config(particleCount=n, gridSize=G, gravity=d["gravity"], softeningCells=d["soft"],
+                                sortInterval=SORT)
+        scenes.append((cfg, pos, d["vel"], dt))
+    b = _batch(scenes)
+    b.run(steps, 0)
+    b.synchronize()
+    for i, (cfg, pos, vel, _) in enumerate(scenes):
+        n, G = cfg.particleCount, cfg.gridSize
+        print("[case] batch scene vs oracle", i, n, G, flush=True)
+        gp, _ = b.scene(i).get_particles()
+        gp = gp[np.argsort(gp[:, 3])]
+        prm = oracle.make_params(G, gravity=cfg.gravity, softeningCells=cfg.softeningCells, sortInterval=SORT,
+                                 keyMode=oracle.KEY_MORTON, fixedDt=dt, allocN=n)
+        a0 = oracle.particle_accel(prm, pos)
+        v0 = vel.copy()
+        v0[:, :3] -= 0.5 * dt * a0[:, :3]
+        st = oracle.Stepper(prm, pos, v0)
+        st.step(steps)
+        op = st.pos[np.argsort(st.pos[:, 3])]
+        assert np.array_equal(gp[:, 3], op[:, 3])
+        dp = np.abs(gp[:, :3] - op[:, :3]).max()
+        print("[case]   max |dx|", dp, flush=True)
+        assert dp < 1e-6  # measured 6e-8 (one ulp at 0.5-1)
+        assert b.scene(i).sim_time() == pytest.approx(st.state.simTime, rel=1e-6)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_mode_change_rerecords(gpu):
+    """The recorded graph bakes in each scene's modes (integrator, momentum
+    removal, key order); changing them between runs must not replay the old
+    recording. Against scenes stepped alone with the same change."""
+    scenes = _scenes()[:3]
+    b = _batch(scenes)
+    b.run(SORT, 0)
+    kd = sd.default_modes(exactGrid=1, integrator=sd.INTEGRATOR_KD, momentumZero=0)
+    b.scene(1).set_modes(kd)
+    b.run(SORT, 0)
+    b.synchronize()
+    for i, (cfg, p, v, dt) in enumerate(scenes):
+        s = _alone(cfg, p, v, dt)
+        s.run(SORT)
+        if i == 1:
+            s.set_modes(kd)
+        s.run(SORT)
+        pa, va = s.get_particles()
+        pb, vb = b.scene(i).get_particles()
+        np.testing.assert_array_equal(pb, pa, err_msg=f"scene {i}")
+        np.testing.assert_array_equal(vb, va, err_msg=f"scene {i}")
+        s.close()
+    b.close()

@@ -88,8 +88,8 @@ def test_deposit_fixed_bit_exact(gpu, oracle, periodic, n, G):
     s.close()
 
 
-@pytest.mark.parametrize("periodic,G", [(False, 8), (False, 16), (False, 32), (False, 64),
-                                        (True, 16), (True, 32), (True, 64)])
+@pytest.mark.parametrize("periodic,G", [(False, 8), (False, 16), (False, 32), (False, 64), (False, 96),
+                                        (False, 128), (True, 16), (True, 32), (True, 64), (True, 128)])
 def test_potential_and_accel(gpu, oracle, periodic, G):
     _say("mesh", periodic, G)
     n = 20000
@@ -271,19 +271,70 @@ def test_fused_learned_correction_matches_reference_order(gpu):
     assert np.abs(v1[:, :3] - v2[:, :3]).max() > 1e-3 * np.abs(v1[:, :3]).max()
 
 
-@pytest.mark.parametrize("G,n", [(128, 2_500_000), (64, 2_500_000), (32, 300_000), (128, 300_000)])
-def test_run_equals_single_steps(gpu, G, n):
+@pytest.mark.parametrize("pipeline", [sd.PIPELINE_REFERENCE, sd.PIPELINE_FUSED])
+def test_contact_with_learned_correction(gpu, pipeline):
+    """Contact and the learned correction together: the contact sort permutes
+    the particles within the step, and each particle must still receive its
+    own correction. One kick-drift step from the same state with and without
+    the network differs by exactly the correction: v_a - v_b = c dt, with c
+    evaluated per particle (matched through the identity in w) before the
+    step. The particles start in random order, so the sort moves them."""
+    _say("contact + learned", pipeline)
+    n, G, dt = 30000, 32, 1e-3
+    pos, vel = random_particles(n, seed=41, inactive=0.0)
+    pos[:, 3] = np.arange(1, n + 1, dtype=np.float32)
+    mlp = sd.Mlp(seed=9)
+    cfg = sd.default_config(particleCount=n, gridSize=G, softeningCells=1.5, gravity=0.2, sortInterval=1,
+                            contactEnabled=1, contactRadiusCells=0.4, contactStiffness=1e3)
+    out = []
+    for use_mlp in (True, False):
+        s = sd.Sim(cfg, sd.default_modes(pipeline=pipeline, integrator=sd.INTEGRATOR_KD, momentumZero=0))
+        s.set_fixed_dt(dt)
+        s.set_particles(pos, vel)
+        corr = None
+        if use_mlp:
+            s.attach_mlp(mlp, 2000.0)
+            s.set_mlp_interval(1)
+            # correction at the initial positions, in the initial order; this
+            # also deposits the density, so the fused pipeline then takes its
+            # keys-only contact sort on the step
+            corr = s.mlp_correction()
+        s.step()
+        p, v = s.get_particles()
+        if use_mlp:
+            s.attach_mlp(None)
+        s.close()
+        o = np.argsort(p[:, 3])
+        out.append((p[o], v[o], corr))
+    (pa, va, corr), (pb, vb, _) = out
+    assert np.array_equal(pa[:, 3], pb[:, 3])
+    dv = (va[:, :3] - vb[:, :3]).astype(np.float64)
+    want = corr[:, :3].astype(np.float64) * dt
+    scale = np.abs(want).max()
+    assert scale > 1e-3 * np.abs(vb[:, :3]).max()  # the correction is far above rounding
+    err = np.abs(dv - want).max()
+    _say("  max |dv - c dt|", err, "of", scale)
+    assert err < 1e-3 * scale + 4 * np.spacing(np.abs(vb[:, :3]).max())
+
+
+@pytest.mark.parametrize("G,n,periodic", [(128, 2_500_000, False), (64, 2_500_000, False), (32, 300_000, False),
+                                          (128, 300_000, False), (256, 2_500_000, False),
+                                          (512, 2_500_000, True)])
+def test_run_equals_single_steps(gpu, G, n, periodic):
     """Within a run the partials reduction of each step is carried by the next
     mesh solve's x pass (split over several blocks where the pass allows);
     single steps reduce in a launch of their own. Both sum in one fixed order,
     so a run and the same steps taken one at a time agree bit for bit, here
     under the CFL dt with momentum removal (more than 4 partials per virtual
-    thread at 2.5 M particles)."""
-    _say("run vs steps", G, n)
+    thread at 2.5 M particles). At S=512 (isolated G=256, periodic G=512) the
+    column passes run persistent and the x pass carries the reduction."""
+    _say("run vs steps", G, n, periodic)
     pos, vel = random_particles(n, seed=31)
+    if periodic:
+        pos[:, :3] = np.where(pos[:, 0:1] >= 0, np.mod(pos[:, :3], 1.0), pos[:, :3])
     out = []
     for single in (False, True):
-        s = make_sim(n, G, gravity=0.5, sort_interval=3)
+        s = make_sim(n, G, periodic=periodic, gravity=0.5, sort_interval=3)
         s.set_particles(pos, vel)
         if single:
             for _ in range(7):

@@ -81,6 +81,41 @@ def test_poisson_periodic_single_mode(soft):
 
 
 # ---- deposit ------------------------------------------------------------------------
+def test_point_mass_closed_form_matches_poisson_iso():
+    """The GPU tests' isolated closed form (closed_forms.point_mass_potential)
+    against the oracle's zero-padded FP64 convolution, and the deposit of
+    cell-centred particles onto single nodes."""
+    from closed_forms import point_mass_potential, point_masses
+    G, soft = 16, 1.5
+    src = point_masses(G)
+    pos = np.array([((x + 0.5) / G, (y + 0.5) / G, (z + 0.5) / G, 0.0) for x, y, z, m in src for _ in range(m)],
+                   np.float32)
+    rho = O.deposit_fixed(pos, G).astype(np.float64) / O.FIX_ONE
+    for x, y, z, m in src:
+        assert rho[z, y, x] == m
+    assert np.abs(O.poisson_iso(rho, soft) - point_mass_potential(G, src, soft)).max() < 1e-12
+    # the 0.5-cell floor of kGreen
+    assert np.abs(O.poisson_iso(rho, 0.1) - point_mass_potential(G, src, 0.1)).max() < 1e-12
+
+
+@pytest.mark.parametrize("G,soft", [(16, 1.5), (32, 0.0), (32, 2.0)])
+def test_plane_wave_closed_form_matches_poisson_periodic(G, soft):
+    """The GPU tests' periodic closed form (a density of u = (x + 3y + 7z) mod G,
+    potential from a 1-D transform) against the oracle's 3-D FP64 solve."""
+    from closed_forms import plane_particles, plane_potential
+    b, c = 3, 7
+    counts = {1: 2, 5: 1, G - 3: 3, G // 2: 1}
+    pos = plane_particles(G, b, c, counts)
+    rho = O.deposit_fixed(pos, G, periodic=True).astype(np.float64) / O.FIX_ONE
+    g = plane_potential(G, b, c, counts, soft)
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    want = g[(x + b * y + c * z) % G]
+    got = O.poisson_periodic(rho, soft)
+    scale = np.abs(want).max()
+    assert scale > 1e-8 and np.abs(want - want.mean()).max() > 0.1 * scale  # not flat
+    assert np.abs(got - want).max() < 1e-9 * scale
+
+
 def test_fixed_weights_closed_form_and_exact_sum():
     G = 32
     rng = np.random.default_rng(2)
