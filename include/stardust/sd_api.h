@@ -200,6 +200,22 @@ int sd_particle_accel(sd_sim* s, float* acc4, int n);
  * acc3[3*nt] = gm * sum_j (x_j - x_i)/(r^2 + eps^2)^(3/2), box units. */
 int sd_direct_sum(sd_sim* s, const int32_t* targets, int nt, double gm, double eps, double* acc3);
 
+/* Allocation-free variant of sd_direct_sum. No reference counterpart: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). targetsDev
+ * (nt indices) and acc3Dev (3*nt doubles) are DEVICE pointers; the call is
+ * enqueued on the handle's stream and returns without synchronising. The
+ * FP64 partial sums live in a fixed scratch array of SD_DIRECT_SCRATCH_DOUBLES
+ * doubles (16 MiB) that the library holds on each device it runs on until the
+ * process ends; targets are processed in groups that fit it, so any nt is
+ * accepted and the results equal sd_direct_sum's bit for bit. Sizes beyond the
+ * scratch (more than 11 billion particles) return SD_ERR_ARG; a target index
+ * outside [0, particleCount) yields NaN for that target. Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (other handles'
+ * streams, other host threads) race. */
+#define SD_DIRECT_SCRATCH_DOUBLES 2097152
+int sd_direct_sum_noalloc(sd_sim* s, const int32_t* targetsDev, int nt, double gm, double eps, double* acc3Dev);
+
 /* ---- learned sub-cell correction (no reference counterpart) --------------
  * MLP 30 -> 128 -> 128 -> 128 -> 3 (GELU, tanh form), BF16 weights, FP32
  * accumulation on the matrix cores. Inputs per particle (sd_features): log1p

@@ -170,6 +170,7 @@ def _declare(L):
         "sd_sort_now": (C.c_int, [P, P, P]),
         "sd_particle_accel": (C.c_int, [P, P, C.c_int]),
         "sd_direct_sum": (C.c_int, [P, P, C.c_int, C.c_double, C.c_double, P]),
+        "sd_direct_sum_noalloc": (C.c_int, [P, P, C.c_int, C.c_double, C.c_double, P]),
         "sd_mlp_param_counts": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sd_mlp_create": (P, [C.c_uint]),
         "sd_mlp_destroy": (None, [P]),

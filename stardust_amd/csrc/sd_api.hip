@@ -7,10 +7,12 @@
 // g_failed / g_failMsg (Sim.cu:52-75).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -761,6 +763,38 @@ void sd_note_fused_steps(sd_sim* s, int steps) {
 }
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free direct sum (sd_direct_sum_noalloc): the FP64
+// chunk partials live in this file-scope device array instead of memory
+// allocated per call. Each device holds its own copy, at its own address,
+// until the process ends; the address is looked up once per device when a
+// handle is first configured there. Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in symbol), only the looked-up
+// address.
+// ---------------------------------------------------------------------------
+#define SD_MAX_DEVICES 64
+static __device__ __align__(256) double direct_scratch[SD_DIRECT_SCRATCH_DOUBLES];
+static double* g_directScratch[SD_MAX_DEVICES] = {};
+static std::mutex g_directScratchMu;
+
+// With `device` current: look up this device's copy of the scratch array.
+static bool resolve_direct_scratch(int device) {
+    if (device < 0 || device >= SD_MAX_DEVICES) {
+        mark_failure("device index %d beyond the %d devices the library tracks", device, SD_MAX_DEVICES);
+        return false;
+    }
+    std::lock_guard<std::mutex> lock(g_directScratchMu);
+    if (g_directScratch[device]) return true;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(direct_scratch));
+    if (e != hipSuccess || !ptr) {
+        mark_failure("hipGetSymbolAddress(direct scratch) on device %d: %s", device, hipGetErrorString(e));
+        return false;
+    }
+    g_directScratch[device] = (double*)ptr;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // defaults / statics
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -959,6 +993,8 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
         return arg_error("grid size not supported (G = 2^k, or 3*2^k isolated; FFT side <= 512)");
     if (c->boundary == SD_BOUNDARY_PERIODIC && !is_pow2(G)) return arg_error("periodic grid must be 2^k");
     if (!s->st) {
+        HK(hipGetDevice(&s->device));
+        if (!resolve_direct_scratch(s->device)) return SD_ERR_DEVICE;
         HK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
         HK(hipEventCreate(&s->evA));
         HK(hipEventCreate(&s->ev1));
@@ -1470,6 +1506,31 @@ int sd_direct_sum(sd_sim* s, const int32_t* targets, int nt, double gm, double e
     HK(hipStreamSynchronize(s->st));
     timer_harvest(s);
     dfree(dt); dfree(part); dfree(out);
+    return g_failed ? SD_ERR_FAILED : SD_OK;
+}
+
+int sd_direct_sum_noalloc(sd_sim* s, const int32_t* targetsDev, int nt, double gm, double eps, double* acc3Dev) {
+    int rc = check_ready(s);
+    if (rc) return rc;
+    if (nt <= 0 || !targetsDev || !acc3Dev) return SD_ERR_ARG;
+    double* scratch = (s->device >= 0 && s->device < SD_MAX_DEVICES) ? g_directScratch[s->device] : nullptr;
+    if (!scratch) return arg_error("sd_direct_sum_noalloc: no scratch address for the handle's device");
+    // Targets go in groups whose chunk partials (3 doubles per target and
+    // source chunk) fit the scratch; a target's partials and their summation
+    // order do not depend on the grouping, so the sums equal sd_direct_sum's.
+    const int nch = direct_chunks_for(s->allocN);
+    const long long perTarget = 3LL * nch;
+    if (perTarget > SD_DIRECT_SCRATCH_DOUBLES)
+        return arg_error("sd_direct_sum_noalloc: too many particles for the scratch (3 doubles per 16384 "
+                         "particles per target must fit SD_DIRECT_SCRATCH_DOUBLES)");
+    const int group = (int)std::min<long long>(nt, SD_DIRECT_SCRATCH_DOUBLES / perTarget);
+    const float eps2 = (float)(eps * eps);
+    for (int t0 = 0; t0 < nt; t0 += group) {
+        const int m = std::min(group, nt - t0);
+        TIMED(s, "direct", launch_direct_sum(s->pos, s->allocN, targetsDev + t0, m, eps2, scratch, nch,
+                                             acc3Dev + 3 * (size_t)t0, gm, s->st));
+    }
+    HK(hipGetLastError());
     return g_failed ? SD_ERR_FAILED : SD_OK;
 }
 

@@ -1102,7 +1102,12 @@ __global__ void __launch_bounds__(256) k_direct_partial(const float4* __restrict
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int chunk = blockIdx.y;
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (t < nt) p = pos[targets[t]];
+    if (t < nt) {
+        // a target outside [0, n) gets NaN accelerations (the host entry point
+        // rejects such targets; device-side target lists are checked here)
+        const int ti = targets[t];
+        p = (ti >= 0 && ti < n) ? pos[ti] : make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
+    }
     float ax = 0.f, ay = 0.f, az = 0.f;
     const int j0 = chunk * DS_CHUNK, j1 = min(n, j0 + DS_CHUNK);
     for (int base = j0; base < j1; base += 256) {

@@ -45,6 +45,7 @@ struct sd_sim {
     int allocN = 0;  // slots (mass per particle = gravity / allocN)
     int active = 0;  // slots in use (prefix), for state files
     float fixedDt = 0.f;
+    int device = -1;  // HIP device current when the handle was first configured
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;  // mesh stream of overlapped steps (joins st within the step)
     hipEvent_t evFork = nullptr, evJoin = nullptr;
