@@ -7,8 +7,11 @@ fused gather / kick / drift / deposit, reductions) with inputs resident in HBM.
 
 Prints one JSON line. ``roofline`` covers the dominant kernel (k_integrate),
 timed with HIP events on the library's stream inside the timed region;
-``cpu_baseline`` times the CPU oracle's reference-order step on a bounded
-sample of the same workload.
+``force_error`` gives the force rel-RMS against the direct sum (the metric's
+second half) for the benchmarked step and for the learned-correction
+comparison (plain G=128, G=128 + network, plain G=256) with the corrected
+step's rates; ``cpu_baseline`` times the CPU oracle's reference-order step on
+a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -36,20 +39,66 @@ def integrate_bytes(n_live, G, contact=False):
     return (80.0 if contact else 64.0) * n_live + 12.0 * G ** 3
 
 
-def cpu_baseline(cfg, n_steps, threads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, n_steps):
+    """The oracle's reference-order step (FP32 particles and FP32 mesh, OpenMP
+    threads = OMP_NUM_THREADS, else every host core) on the same particles."""
     from oracle import oracle as O
-    import numpy as np
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     n = cfg["pos"].shape[0]
     prm = O.make_params(cfg["G"], gravity=cfg["gravity"], softeningCells=cfg["soft"],
                         sortInterval=cfg.get("sort_interval", 10), keyMode=O.KEY_MORTON,
-                        fixedDt=cfg["dt"], allocN=n)
+                        fixedDt=cfg["dt"], allocN=n, fp32Mesh=1)
     st = O.Stepper(prm, cfg["pos"], cfg["vel"])
     st.step(1)  # builds and caches the Green spectrum, warms the allocator
     t0 = time.perf_counter()
     st.step(n_steps)
     dt = time.perf_counter() - t0
     return n * n_steps / dt, dt
+
+
+def force_errors(cfg, sim, log_fn):
+    """Force rel-RMS against the Plummer direct sum on 16,384 fixed particles
+    (method of proto/cuda/x3.cu:109-185): the benchmarked step as run, and the
+    learned-correction comparison at the initial positions (mesh softening 1
+    cell): plain G=128, G=128 + the packaged network, plain G=256; plus the
+    corrected step's rate refreshed every step and every 5 steps against plain
+    G=256."""
+    from stardust_amd import force_error as FE
+    from stardust_amd import learned
+    eps = learned.EPS_BOX_DEFAULT
+    out = {"targets": 16384, "direct_sum_eps_box": eps}
+    r = FE.evaluate(sim, eps)
+    out["benchmarked_step"] = {"G": cfg["G"], "softening_cells": cfg["soft"], "rel_rms": round(r["rel_rms"], 5),
+                               "lsq_scale": round(r["lsq_scale"], 4), "at": "positions after the timed steps"}
+    p4, v4, M, L = cfg["pos"], cfg["vel"], cfg["M"], cfg["L"]
+    try:
+        mlp, out_scale = learned.load_model()
+    except OSError:
+        mlp, out_scale = None, 1.0
+        log_fn("no packaged network: learned-correction lines skipped")
+    err = learned.evaluate(p4, v4, M, L, 128, 256, mlp, out_scale)
+    out["softening_cells"] = 1.0
+    for tag, e in err.items():
+        out[tag.replace("+", "_")] = {"G": e["G"], "rel_rms": round(e["rel_rms"], 5), "lsq_scale": round(e["lsq_scale"], 4)}
+    if mlp is not None:
+        rate_f, ms_f = learned.time_steps(p4, v4, M, L, 256)
+        rates = {"plain_G256": {"particle_steps_per_s": round(rate_f, 1), "ms_per_step": round(ms_f, 4)}}
+        for k in (1, 5):
+            v, ms = learned.time_steps(p4, v4, M, L, 128, mlp, out_scale, interval=k)
+            rates[f"G128_mlp_refresh{k}"] = {"particle_steps_per_s": round(v, 1), "ms_per_step": round(ms, 4),
+                                             "vs_plain_G256": round(v / rate_f, 3)}
+        out["rates"] = rates
+    return out
 
 
 def copy_calibration(mib):
@@ -125,8 +174,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-force-error", action="store_true")
     ap.add_argument("--kernel", default="integrate", help="kernel timed for the roofline")
     ap.add_argument("--workload", choices=("two_disk", "cosmic_web"), default="two_disk",
                     help="two_disk: the headline 4M-particle G=128 step; cosmic_web: 16.7M particles, "
@@ -201,24 +251,36 @@ def main():
         avg_s = kms / launches / 1e3
         nbytes = integrate_bytes(n_live, G, contact)
         ach = nbytes / avg_s / 1e9
-        traffic = None
+        # HBM traffic needs counter runs (rocprofv3 --pmc), which this run does
+        # not make: `traffic` stays null and the committed counter summary of
+        # the same kernel and workload is quoted beside it, with its source
+        profiled = None
         tpath = os.path.join(ROOT, "profiles", "traffic_k_integrate_two_disk_4M_G128.json")
         if not contact and args.kernel == "integrate" and G == 128 and os.path.exists(tpath):
             with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                profiled = {"bytes_per_launch": json.load(f).get("hbm_bytes_per_launch"),
+                            "source": "profiles/traffic_k_integrate_two_disk_4M_G128.json (FETCH_SIZE and "
+                                      "WRITE_SIZE counter runs of bench.py, not of this run)"}
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "traffic_profiled": profiled,
                 "kernel": "sd::k_integrate", "avg_launch_us": round(avg_s * 1e6, 3),
                 "algorithmic_bytes_per_launch": nbytes}
 
+    ferr = None
+    if not args.no_force_error and not contact and G == 128:
+        log("force errors against the direct sum (16,384 particles)")
+        ferr = force_errors(cfg, sim, log)
+
     cpu = None
     if not args.no_cpu and not contact:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle reference step, {args.cpu_steps} steps, {threads} threads")
-        v, secs = cpu_baseline(cfg, args.cpu_steps, threads)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        log(f"cpu baseline: oracle reference step, FP32 mesh, {args.cpu_steps} steps, {threads} threads")
+        v, secs = cpu_baseline(cfg, args.cpu_steps)
         cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"two-disk 4,194,304 particles, G={G}, {args.cpu_steps} steps after one "
-                         f"warm-up step ({secs:.1f} s): FP64 FFT mesh + FP32 particle kernels"}
+               "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+               "sample": f"two-disk 4,194,304 particles, G={G}, {args.cpu_steps} steps after one warm-up step "
+                         f"({secs:.1f} s): reference kernel order, FP32 particles and FP32 FFT mesh, OpenMP"}
 
     out = {
         "metric": metric,
@@ -236,6 +298,7 @@ def main():
         "config": {"workload": desc, "particles": n, "mesh": G, "fft_side": fft_side,
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "force_error": ferr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -70,6 +70,7 @@ class Params(C.Structure):
         ("softBoundR", C.c_float),
         ("darkEnergy", C.c_float),
         ("allocN", C.c_int),
+        ("fp32Mesh", C.c_int),
     ]
 
 
@@ -96,6 +97,8 @@ def _declare(L):
     L.sdo_green_real.argtypes = [C.c_int, C.c_float, _dp]
     L.sdo_poisson_iso.argtypes = [_dp, C.c_int, C.c_float, _dp]
     L.sdo_poisson_periodic.argtypes = [_dp, C.c_int, C.c_float, _dp]
+    L.sdo_poisson_iso_f32.argtypes = [_fp, C.c_int, C.c_float, _fp]
+    L.sdo_poisson_periodic_f32.argtypes = [_fp, C.c_int, C.c_float, _fp]
     L.sdo_grid_accel.argtypes = [_dp, C.c_int, C.c_float, C.c_int, _fp]
     L.sdo_sample_acc.argtypes = [_fp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _fp]
     L.sdo_sample_acc_all.argtypes = [_fp, _fp, C.c_int, C.c_int, C.c_int, _fp]
@@ -221,6 +224,24 @@ def poisson_periodic(rho, softCells):
     return pot.reshape(G, G, G)
 
 
+def poisson_iso_f32(rho, softCells):
+    """Single-precision isolated solve (the CPU baseline's mesh)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    G = rho.shape[0]
+    pot = np.empty(G * G * G, np.float32)
+    lib().sdo_poisson_iso_f32(rho.reshape(-1), G, softCells, pot)
+    return pot.reshape(G, G, G)
+
+
+def poisson_periodic_f32(rho, softCells):
+    """Single-precision periodic solve (the CPU baseline's mesh)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    G = rho.shape[0]
+    pot = np.empty(G * G * G, np.float32)
+    lib().sdo_poisson_periodic_f32(rho.reshape(-1), G, softCells, pot)
+    return pot.reshape(G, G, G)
+
+
 def grid_accel(pot, potScale, periodic=False):
     pot = np.ascontiguousarray(pot, dtype=np.float64)
     G = pot.shape[0]
@@ -303,10 +324,10 @@ def contact(pos, vel, G, cell_start, cell_end, radius, stiffness, damping, perio
 def make_params(G, periodic=False, gravity=1.0, softeningCells=3.0, timeScale=1.0,
                 sortInterval=40, keyMode=KEY_ROWMAJOR, fixedDeposit=1, fixedDt=0.0,
                 momentumZero=1, contact=0, contactRadiusCells=0.5, contactStiffness=1e6,
-                contactDamping=0.35, c2=1e4, softBoundR=0.0, darkEnergy=0.0, allocN=1):
+                contactDamping=0.35, c2=1e4, softBoundR=0.0, darkEnergy=0.0, allocN=1, fp32Mesh=0):
     return Params(G, int(periodic), gravity, softeningCells, timeScale, sortInterval, keyMode,
                   fixedDeposit, fixedDt, momentumZero, contact, contactRadiusCells,
-                  contactStiffness, contactDamping, c2, softBoundR, darkEnergy, allocN)
+                  contactStiffness, contactDamping, c2, softBoundR, darkEnergy, allocN, int(fp32Mesh))
 
 
 def dt_policy(params, vmax):

@@ -68,6 +68,7 @@ uint32_t sdo_morton3(uint32_t x, uint32_t y, uint32_t z) {
  * CIC base node clamp(floor(p*G - 0.5), 0, G-1) instead (no reference
  * counterpart: the product's step order). */
 void sdo_cell_keys(const float* pos4, int n, int G, int mode, uint32_t* keys, int32_t* idx) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const float* p = pos4 + 4 * (size_t)i;
         idx[i] = i;
@@ -131,6 +132,7 @@ void sdo_stable_sort_pairs(uint32_t* keys, int32_t* vals, int n) {
 
 /* kReorder, Sim.cu:2488-2495: dst[i] = src[order[i]] for float4 arrays. */
 void sdo_reorder4(const float* src4, const int32_t* order, int n, float* dst4) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) memcpy(dst4 + 4 * (size_t)i, src4 + 4 * (size_t)order[i], 16);
 }
 
@@ -197,6 +199,9 @@ void sdo_cic_weights_fixed(float px, float py, float pz, int G, int* base3, int6
 void sdo_deposit_fixed(const float* pos4, int n, int G, int periodic, int64_t* rho) {
     const size_t cells = (size_t)G * G * G;
     memset(rho, 0, sizeof(int64_t) * cells);
+    /* integer adds are exact, so the atomic adds of several threads give the
+     * same grid in any order */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const float* p = pos4 + 4 * (size_t)i;
         if (p[0] < 0.f) continue;
@@ -205,7 +210,8 @@ void sdo_deposit_fixed(const float* pos4, int n, int G, int periodic, int64_t* r
         sdo_cic_weights_fixed(p[0], p[1], p[2], G, b, w);
         for (int k = 0; k < 8; ++k) {
             const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-            rho[gidx3(b[0] + ox, b[1] + oy, b[2] + oz, G, G, periodic)] += w[k];
+            __atomic_fetch_add(&rho[gidx3(b[0] + ox, b[1] + oy, b[2] + oz, G, G, periodic)], w[k],
+                               __ATOMIC_RELAXED);
         }
     }
 }
@@ -214,139 +220,18 @@ void sdo_deposit_fixed(const float* pos4, int n, int G, int periodic, int64_t* r
 /* FFT (own mixed-radix, FP64)                                               */
 /* ------------------------------------------------------------------------ */
 
-/* Recursive decimation-in-time mixed-radix DFT for n = 2^a 3^b (any other
- * factor falls back to a direct DFT of that length). Twiddles come from a
- * table of the root length N0: W_n^j = tab[j * (N0/n)]. Sign -1 forward, +1
- * inverse; no normalisation. */
-typedef struct fft_tab {
-    int n0;
-    double* c;
-    double* s;
-} fft_tab;
-
-static void tab_init(fft_tab* t, int n0, int sign) {
-    t->n0 = n0;
-    t->c = (double*)malloc(sizeof(double) * (size_t)n0);
-    t->s = (double*)malloc(sizeof(double) * (size_t)n0);
-    for (int j = 0; j < n0; ++j) {
-        const double a = sign * 2.0 * M_PI * (double)j / (double)n0;
-        t->c[j] = cos(a);
-        t->s[j] = sin(a);
-    }
-}
-static void tab_free(fft_tab* t) {
-    free(t->c);
-    free(t->s);
-}
-
-static void fft_rec(double* re, double* im, int n, int stride, const fft_tab* tb, double* tre,
-                    double* tim) {
-    if (n == 1) return;
-    int p;
-    if (n % 4 == 0) p = 4;
-    else if (n % 2 == 0) p = 2;
-    else if (n % 3 == 0) p = 3;
-    else p = n;
-    const int m = n / p;
-    const int step = tb->n0 / n;
-    for (int r = 0; r < p; ++r)
-        for (int j = 0; j < m; ++j) {
-            tre[r * m + j] = re[(size_t)(j * p + r) * stride];
-            tim[r * m + j] = im[(size_t)(j * p + r) * stride];
-        }
-    for (int r = 0; r < p; ++r) fft_rec(tre + r * m, tim + r * m, m, 1, tb, tre + n, tim + n);
-    /* X[k + q m] = sum_r W_p^{rq} (W_n^{rk} Y_r[k]) */
-    for (int k = 0; k < m; ++k) {
-        double yr[64], yi[64];
-        double* YR = yr;
-        double* YI = yi;
-        double* big = NULL;
-        if (p > 64) {
-            big = (double*)malloc(sizeof(double) * 2 * (size_t)p);
-            YR = big;
-            YI = big + p;
-        }
-        for (int r = 0; r < p; ++r) {
-            const size_t e = ((size_t)r * k % (size_t)n) * (size_t)step;
-            const double c = tb->c[e], s = tb->s[e];
-            const double xr = tre[r * m + k], xi = tim[r * m + k];
-            YR[r] = xr * c - xi * s;
-            YI[r] = xr * s + xi * c;
-        }
-        for (int q = 0; q < p; ++q) {
-            double sr = 0.0, si = 0.0;
-            for (int r = 0; r < p; ++r) {
-                const size_t e = ((size_t)r * q % (size_t)p) * (size_t)(tb->n0 / p);
-                const double c = tb->c[e], s = tb->s[e];
-                sr += YR[r] * c - YI[r] * s;
-                si += YR[r] * s + YI[r] * c;
-            }
-            re[(size_t)(k + q * m) * stride] = sr;
-            im[(size_t)(k + q * m) * stride] = si;
-        }
-        free(big);
-    }
-}
-
-void sdo_fft1d(double* re, double* im, int n, int inverse) {
-    fft_tab tb;
-    tab_init(&tb, n, inverse ? +1 : -1);
-    double* t = (double*)malloc(sizeof(double) * 4 * (size_t)n + 64);
-    fft_rec(re, im, n, 1, &tb, t, t + 2 * (size_t)n);
-    free(t);
-    tab_free(&tb);
-}
-
-/* 3D transform of an array laid out [z][y][x] (x fastest). */
-void sdo_fft3d(double* re, double* im, int nx, int ny, int nz, int inverse) {
-    const int sign = inverse ? +1 : -1;
-    const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
-    fft_tab tx, ty, tz;
-    tab_init(&tx, nx, sign);
-    tab_init(&ty, ny, sign);
-    tab_init(&tz, nz, sign);
-#pragma omp parallel
-    {
-        double* lr = (double*)malloc(sizeof(double) * (size_t)nmax * 6 + 64);
-        double* li = lr + nmax;
-        double* t = li + nmax;
-        /* along x */
-#pragma omp for schedule(static)
-        for (long long l = 0; l < (long long)ny * nz; ++l)
-            fft_rec(re + (size_t)l * nx, im + (size_t)l * nx, nx, 1, &tx, t, t + 2 * (size_t)nmax);
-        /* along y */
-#pragma omp for schedule(static)
-        for (long long l = 0; l < (long long)nx * nz; ++l) {
-            const int x = (int)(l % nx), z = (int)(l / nx);
-            for (int y = 0; y < ny; ++y) {
-                lr[y] = re[((size_t)z * ny + y) * nx + x];
-                li[y] = im[((size_t)z * ny + y) * nx + x];
-            }
-            fft_rec(lr, li, ny, 1, &ty, t, t + 2 * (size_t)nmax);
-            for (int y = 0; y < ny; ++y) {
-                re[((size_t)z * ny + y) * nx + x] = lr[y];
-                im[((size_t)z * ny + y) * nx + x] = li[y];
-            }
-        }
-        /* along z */
-#pragma omp for schedule(static)
-        for (long long l = 0; l < (long long)nx * ny; ++l) {
-            for (int z = 0; z < nz; ++z) {
-                lr[z] = re[(size_t)z * nx * ny + l];
-                li[z] = im[(size_t)z * nx * ny + l];
-            }
-            fft_rec(lr, li, nz, 1, &tz, t, t + 2 * (size_t)nmax);
-            for (int z = 0; z < nz; ++z) {
-                re[(size_t)z * nx * ny + l] = lr[z];
-                im[(size_t)z * nx * ny + l] = li[z];
-            }
-        }
-        free(lr);
-    }
-    tab_free(&tx);
-    tab_free(&ty);
-    tab_free(&tz);
-}
+/* Mixed-radix FFT in FP64 (sdo_fft1d / sdo_fft3d) and FP32 (sdo_fft3d_f32,
+ * the CPU baseline's single-precision mesh): one source, compiled twice. */
+#define REAL double
+#define SUF(name) name
+#include "sd_oracle_fft.inc"
+#undef REAL
+#undef SUF
+#define REAL float
+#define SUF(name) name##_f32
+#include "sd_oracle_fft.inc"
+#undef REAL
+#undef SUF
 
 /* kGreen, Sim.cu:563-573: g = -1/(sqrt(dx^2+dy^2+dz^2) + soft) with wrapped
  * distances d = x <= S/2 ? x : S - x. Evaluated in FP64. */
@@ -443,6 +328,86 @@ void sdo_poisson_periodic(const double* rho, int G, float softCells, double* pot
     free(ai);
 }
 
+/* Single-precision mesh of the CPU baseline (no reference counterpart beyond
+ * cuFFT's FP32 transforms at Sim.cu:3592, 3601): the same isolated and
+ * periodic solves as above with FP32 arrays and transforms (twiddles rounded
+ * from FP64), the Green spectrum cached per (S, soft). Used only for timing. */
+static float* g_greenRe32 = NULL;
+static float* g_greenIm32 = NULL;
+static int g_greenS32 = 0;
+static float g_greenSoft32 = -1.f;
+
+void sdo_poisson_iso_f32(const float* rho, int G, float softCells, float* pot) {
+    const int S = 2 * G;
+    const size_t N = (size_t)S * S * S;
+    const float soft = softCells > 0.5f ? softCells : 0.5f;
+    if (g_greenS32 != S || g_greenSoft32 != soft) {
+        free(g_greenRe32);
+        free(g_greenIm32);
+        double* g = (double*)malloc(N * sizeof(double));
+        sdo_green_real(S, soft, g);
+        g_greenRe32 = (float*)malloc(N * sizeof(float));
+        g_greenIm32 = (float*)calloc(N, sizeof(float));
+        for (size_t i = 0; i < N; ++i) g_greenRe32[i] = (float)g[i];
+        free(g);
+        sdo_fft3d_f32(g_greenRe32, g_greenIm32, S, S, S, 0);
+        g_greenS32 = S;
+        g_greenSoft32 = soft;
+    }
+    float* ar = (float*)calloc(N, sizeof(float));
+    float* ai = (float*)calloc(N, sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < G; ++z)
+        for (int y = 0; y < G; ++y)
+            memcpy(ar + ((size_t)z * S + y) * S, rho + ((size_t)z * G + y) * G, sizeof(float) * (size_t)G);
+    sdo_fft3d_f32(ar, ai, S, S, S, 0);
+    const float norm = (float)(1.0 / (double)N);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)N; ++i) {
+        const float xr = ar[i], xi = ai[i], gr = g_greenRe32[i], gi = g_greenIm32[i];
+        ar[i] = (xr * gr - xi * gi) * norm;
+        ai[i] = (xr * gi + xi * gr) * norm;
+    }
+    sdo_fft3d_f32(ar, ai, S, S, S, 1);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < G; ++z)
+        for (int y = 0; y < G; ++y)
+            memcpy(pot + ((size_t)z * G + y) * G, ar + ((size_t)z * S + y) * S, sizeof(float) * (size_t)G);
+    free(ar);
+    free(ai);
+}
+
+void sdo_poisson_periodic_f32(const float* rho, int G, float softCells, float* pot) {
+    const size_t N = (size_t)G * G * G;
+    float* ar = (float*)malloc(N * sizeof(float));
+    float* ai = (float*)calloc(N, sizeof(float));
+    memcpy(ar, rho, N * sizeof(float));
+    sdo_fft3d_f32(ar, ai, G, G, G, 0);
+    const float scale = (float)((1.0 / (double)N) / (4.0 * M_PI * M_PI));
+    const float sg = (float)(2.0 * M_PI * (double)softCells / (double)G);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < G; ++z)
+        for (int y = 0; y < G; ++y)
+            for (int x = 0; x < G; ++x) {
+                const int kx = x <= G / 2 ? x : x - G;
+                const int ky = y <= G / 2 ? y : y - G;
+                const int kz = z <= G / 2 ? z : z - G;
+                const float k2 = (float)(kx * kx + ky * ky + kz * kz);
+                const size_t i = ((size_t)z * G + y) * G + x;
+                float m = 0.f;
+                if (k2 > 0.f) {
+                    m = -scale / k2;
+                    if (softCells > 0.f) m *= expf(-0.5f * k2 * sg * sg);
+                }
+                ar[i] *= m;
+                ai[i] *= m;
+            }
+    sdo_fft3d_f32(ar, ai, G, G, G, 1);
+    memcpy(pot, ar, N * sizeof(float));
+    free(ar);
+    free(ai);
+}
+
 /* kGridAccel, Sim.cu:589-611: a = -(pot[+1] - pot[-1]) * (0.5*G*potScale);
  * isolated clamps the stencil at the edge, periodic wraps. pot is G^3. */
 void sdo_grid_accel(const double* pot, int G, float potScale, int periodic, float* acc4) {
@@ -466,6 +431,33 @@ void sdo_grid_accel(const double* pot, int G, float potScale, int periodic, floa
                 a[0] = (float)(-(P(xp, y, z) - P(xm, y, z)) * h);
                 a[1] = (float)(-(P(x, yp, z) - P(x, ym, z)) * h);
                 a[2] = (float)(-(P(x, y, zp) - P(x, y, zm)) * h);
+                a[3] = 0.f;
+#undef P
+            }
+}
+
+/* The same gradient from the single-precision potential (the CPU baseline). */
+void sdo_grid_accel_f32(const float* pot, int G, float potScale, int periodic, float* acc4) {
+    const float h = 0.5f * (float)G * potScale;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < G; ++z)
+        for (int y = 0; y < G; ++y)
+            for (int x = 0; x < G; ++x) {
+                int xm, xp, ym, yp, zm, zp;
+                if (periodic) {
+                    xm = (x - 1) & (G - 1); xp = (x + 1) & (G - 1);
+                    ym = (y - 1) & (G - 1); yp = (y + 1) & (G - 1);
+                    zm = (z - 1) & (G - 1); zp = (z + 1) & (G - 1);
+                } else {
+                    xm = x > 0 ? x - 1 : 0; xp = x < G - 1 ? x + 1 : G - 1;
+                    ym = y > 0 ? y - 1 : 0; yp = y < G - 1 ? y + 1 : G - 1;
+                    zm = z > 0 ? z - 1 : 0; zp = z < G - 1 ? z + 1 : G - 1;
+                }
+#define P(X, Y, Z) pot[((size_t)(Z) * G + (Y)) * G + (X)]
+                float* a = acc4 + 4 * (((size_t)z * G + y) * G + x);
+                a[0] = -(P(xp, y, z) - P(xm, y, z)) * h;
+                a[1] = -(P(x, yp, z) - P(x, ym, z)) * h;
+                a[2] = -(P(x, y, zp) - P(x, y, zm)) * h;
                 a[3] = 0.f;
 #undef P
             }
@@ -792,6 +784,25 @@ void sdo_compute_accel(const sdo_params* prm, const float* pos4, int n, float* a
                        int64_t* rhoFixOut, double* potOut) {
     const int G = prm->G;
     const size_t cells = (size_t)G * G * G;
+    const float potScale = prm->gravity / (float)(prm->allocN > 0 ? prm->allocN : 1);
+    if (prm->fp32Mesh && !potOut) {
+        /* the CPU baseline's single-precision mesh */
+        int64_t* rf = rhoFixOut ? rhoFixOut : (int64_t*)malloc(cells * sizeof(int64_t));
+        float* r32 = (float*)malloc(cells * sizeof(float));
+        float* p32 = (float*)malloc(cells * sizeof(float));
+        sdo_deposit_fixed(pos4, n, G, prm->periodic, rf);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)cells; ++i) r32[i] = (float)rf[i] * (1.0f / (float)SD_FIX_ONE);
+        if (prm->periodic)
+            sdo_poisson_periodic_f32(r32, G, prm->softeningCells, p32);
+        else
+            sdo_poisson_iso_f32(r32, G, prm->softeningCells, p32);
+        sdo_grid_accel_f32(p32, G, potScale, prm->periodic, accG4);
+        if (!rhoFixOut) free(rf);
+        free(r32);
+        free(p32);
+        return;
+    }
     double* rho = (double*)malloc(cells * sizeof(double));
     double* pot = potOut ? potOut : (double*)malloc(cells * sizeof(double));
     if (prm->fixedDeposit) {
@@ -809,7 +820,6 @@ void sdo_compute_accel(const sdo_params* prm, const float* pos4, int n, float* a
         sdo_poisson_periodic(rho, G, prm->softeningCells, pot);
     else
         sdo_poisson_iso(rho, G, prm->softeningCells, pot);
-    const float potScale = prm->gravity / (float)(prm->allocN > 0 ? prm->allocN : 1);
     sdo_grid_accel(pot, G, potScale, prm->periodic, accG4);
     free(rho);
     if (!potOut) free(pot);

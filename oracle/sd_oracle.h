@@ -51,6 +51,10 @@ void sdo_fft3d(double* re, double* im, int nx, int ny, int nz, int inverse);
 void sdo_green_real(int S, float soft, double* g);
 void sdo_poisson_iso(const double* rho, int G, float softCells, double* pot);
 void sdo_poisson_periodic(const double* rho, int G, float softCells, double* pot);
+void sdo_fft3d_f32(float* re, float* im, int nx, int ny, int nz, int inverse);
+void sdo_poisson_iso_f32(const float* rho, int G, float softCells, float* pot);
+void sdo_poisson_periodic_f32(const float* rho, int G, float softCells, float* pot);
+void sdo_grid_accel_f32(const float* pot, int G, float potScale, int periodic, float* acc4);
 void sdo_grid_accel(const double* pot, int G, float potScale, int periodic, float* acc4);
 
 /* --- particles ----------------------------------------------------------- */
@@ -90,6 +94,7 @@ typedef struct sdo_params {
     float softBoundR;
     float darkEnergy;
     int allocN;         /* mass per particle = gravity / allocN */
+    int fp32Mesh;       /* 1: single-precision mesh (the CPU baseline's timing path) */
 } sdo_params;
 
 typedef struct sdo_state {

@@ -21,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
@@ -125,6 +126,21 @@ def train(X, T, epochs=30, batch=8192, lr=1e-3, seed=1, log=print):
     log(f"[learned] trained {epochs} epochs on {X.shape[0]} pairs in {time.perf_counter() - t0:.1f} s; "
         f"loss {losses[0]:.4f} -> {losses[-1]:.4f}; outScale {out_scale:.4f}")
     return m, out_scale, losses
+
+
+MODEL_TWO_DISK_G128 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "mlp_two_disk_G128.npz")
+
+
+def load_model(path=MODEL_TWO_DISK_G128):
+    """A network written by ``--save-model`` (arrays only, no pickles):
+    returns (Mlp, out_scale). The packaged model was trained for the G=128
+    mesh by ``python -m stardust_amd.learned --save-model <path>`` (default
+    options: 2,359,296 pairs, 30 epochs)."""
+    with np.load(path, allow_pickle=False) as d:
+        W, b, out_scale = d["W"], d["b"], float(d["out_scale"])
+    m = Mlp(seed=1)
+    m.set_params(W, b)
+    return m, out_scale
 
 
 def evaluate(pos, vel, M, L, G_coarse, G_fine, mlp=None, out_scale=1.0, eps_box=EPS_BOX_DEFAULT,

@@ -116,6 +116,29 @@ def test_plane_wave_closed_form_matches_poisson_periodic(G, soft):
     assert np.abs(got - want).max() < 1e-9 * scale
 
 
+@pytest.mark.parametrize("periodic,G", [(False, 12), (False, 16), (True, 16), (True, 32)])
+def test_fp32_mesh_matches_fp64(periodic, G):
+    """The CPU baseline's single-precision solves against the FP64 ones, and
+    its whole step against the FP64-mesh step (FP32 tolerances)."""
+    pos = rand_pos(3000, G)
+    rho = O.deposit_fixed(pos, G, periodic).astype(np.float64) / O.FIX_ONE
+    if periodic:
+        ref, got = O.poisson_periodic(rho, 1.5), O.poisson_periodic_f32(rho, 1.5)
+    else:
+        ref, got = O.poisson_iso(rho, 1.5), O.poisson_iso_f32(rho, 1.5)
+    assert np.abs(got - ref).max() < 2e-6 * np.abs(ref).max()
+    vel = np.zeros_like(pos)
+    vel[:, :3] = np.random.default_rng(G).normal(0, 0.05, (3000, 3))
+    out = []
+    for f32 in (0, 1):
+        prm = O.make_params(G, periodic=periodic, gravity=0.5, softeningCells=1.5, sortInterval=2, fixedDt=2e-3,
+                            allocN=3000, fp32Mesh=f32)
+        st = O.Stepper(prm, pos, vel)
+        st.step(3)
+        out.append(st.pos)
+    assert np.abs(out[0][:, :3] - out[1][:, :3]).max() < 1e-6
+
+
 def test_fixed_weights_closed_form_and_exact_sum():
     G = 32
     rng = np.random.default_rng(2)
