@@ -71,8 +71,8 @@ def force_errors(cfg, sim, log_fn):
     (method of proto/cuda/x3.cu:109-185): the benchmarked step as run, and the
     learned-correction comparison at the initial positions (mesh softening 1
     cell): plain G=128, G=128 + the packaged network, plain G=256; plus the
-    corrected step's rate refreshed every step and every 5 steps against plain
-    G=256."""
+    corrected step's rate refreshed every step, every 2 and every 5 steps
+    against plain G=256."""
     from stardust_amd import force_error as FE
     from stardust_amd import learned
     eps = learned.EPS_BOX_DEFAULT
@@ -93,7 +93,7 @@ def force_errors(cfg, sim, log_fn):
     if mlp is not None:
         rate_f, ms_f = learned.time_steps(p4, v4, M, L, 256)
         rates = {"plain_G256": {"particle_steps_per_s": round(rate_f, 1), "ms_per_step": round(ms_f, 4)}}
-        for k in (1, 5):
+        for k in (1, 2, 5):
             v, ms = learned.time_steps(p4, v4, M, L, 128, mlp, out_scale, interval=k)
             rates[f"G128_mlp_refresh{k}"] = {"particle_steps_per_s": round(v, 1), "ms_per_step": round(ms, 4),
                                              "vs_plain_G256": round(v / rate_f, 3)}

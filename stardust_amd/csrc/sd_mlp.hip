@@ -79,27 +79,44 @@ typedef bf16x8 infx8;
 __device__ __forceinline__ f32x16 mfma_inf(bf16x8 a, bf16x8 b, f32x16 c) { return mfma32(a, b, c); }
 #endif
 
-// GELU on FP16 values, 8 at a time: gelu(u) = u phi(u), phi = 0.5 + uc P(x),
-// uc = clamp(u, -4, 4), x = uc^2 / 8 - 1 in [-1, 1], P degree 6 (a weighted
-// least-squares fit of tanh(sqrt(2/pi) (u + 0.044715 u^3)) / (2u) with
-// P(1) = 1/8, so phi(+-4) is exactly 1 and 0 in FP16 arithmetic). The centred
-// variable keeps the coefficients below 0.2, so Horner's rule in FP16 loses
-// little: over every FP16 input the result is within 2.4e-3 of the tanh-form
-// GELU (mean 6e-5 for |u| <= 4). Each vector operation is four independent
-// packed instructions, which keeps dependent packed operations apart (gfx950
-// needs a wait state between a packed FP16 result and its next use).
-__device__ __forceinline__ f16x8 gelu_h8(f16x8 u) {
-    const f16x8 uc = __builtin_elementwise_min(__builtin_elementwise_max(u, (f16x8)(-4.0f16)), (f16x8)(4.0f16));
-    const f16x8 x = __builtin_elementwise_fma(uc * uc, (f16x8)(0.125f16), (f16x8)(-1.0f16));
-    f16x8 r = (f16x8)(0.00838470458984375f16);
-    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.0185089111328125f16));
-    r = __builtin_elementwise_fma(r, x, (f16x8)(0.0215911865234375f16));
-    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.0335693359375f16));
-    r = __builtin_elementwise_fma(r, x, (f16x8)(0.055816650390625f16));
-    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.08477783203125f16));
-    r = __builtin_elementwise_fma(r, x, (f16x8)(0.176025390625f16));
-    return u * __builtin_elementwise_fma(uc, r, (f16x8)(0.5f16));
+// GELU on FP16 values, 8 at a time, on halved inputs: the weights and biases
+// that feed an activation are stored halved (exact: powers of two), so the
+// accumulator holds v = u / 2, and the result is gelu(u) / 2 = v phi(u), which
+// the next layer's unscaled weights (output layer: doubled) take as is. With
+// x = v^2 - 2 (in [-2, 2] for |u| <= 4), phi = clamp(0.5 + v R(x), 0, 1), R of
+// degree 6 (a minimax-weighted least-squares fit of
+// (0.5 tanh(sqrt(2/pi) (u + 0.044715 u^3))) / v with R(2) = 1/4 and a positive
+// leading coefficient: beyond |u| = 4, R(x) >= 1/4, so the clamp, which the
+// compiler folds into the packed FMA, saturates phi at exactly 1 and 0). Over
+// every finite FP16 input the result is within 2.4e-3 of the tanh-form GELU
+// (mean 6e-5 for |u| <= 4); 9 packed instructions per pair of values. Each
+// vector operation is four independent packed instructions, which keeps
+// dependent packed operations apart (gfx950 needs a wait state between a packed
+// FP16 result and its next use). Inputs are finite: the accumulators convert
+// with round-toward-zero, which saturates at 65504.
+__device__ __forceinline__ f16x8 gelu_half_h8(f16x8 v) {
+    const f16x8 x = __builtin_elementwise_fma(v, v, (f16x8)(-2.0f16));
+    f16x8 r = (f16x8)(0.00023937225341796875f16);
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.0011377334594726562f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.0028095245361328125f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.008453369140625f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.02777099609375f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(-0.084716796875f16));
+    r = __builtin_elementwise_fma(r, x, (f16x8)(0.35205078125f16));
+    const f16x8 ph = __builtin_elementwise_min(
+        __builtin_elementwise_max(__builtin_elementwise_fma(v, r, (f16x8)(0.5f16)), (f16x8)(0.0f16)), (f16x8)(1.0f16));
+    return v * ph;
 }
+
+// Scale of the stored inference weights: layer 1 weights and the biases of
+// layers 1-3 halved, the output layer's weights doubled (FP16 path only).
+#if SD_MLP_F16
+constexpr float INF_IN_SCALE = 0.5f;
+constexpr float INF_OUT_SCALE = 2.0f;
+#else
+constexpr float INF_IN_SCALE = 1.0f;
+constexpr float INF_OUT_SCALE = 1.0f;
+#endif
 
 // Eight FP32 accumulators -> FP16 (round toward zero, 2 per instruction).
 __device__ __forceinline__ f16x8 to_f16x8(const float* v) {
@@ -408,7 +425,7 @@ __device__ __forceinline__ void mlp_activate(const f32x16* acc, infx8* B) {
         float a8[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) a8[e] = acc[s >> 1][8 * (s & 1) + e];
-        v = gelu_h8(to_f16x8(a8));
+        v = gelu_half_h8(to_f16x8(a8));
 #else
 #pragma unroll
         for (int e = 0; e < 8; ++e) v[e] = (__bf16)gelu_f(acc[s >> 1][8 * (s & 1) + e]);
@@ -470,17 +487,21 @@ __device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const 
     }
 }
 
-// Eight BF16 weights (as stored) -> eight inference elements (FP16: exact).
-__device__ __forceinline__ uint4 to_inf(uint4 v) {
+// Eight BF16 weights (as stored) -> eight inference elements, times a power
+// of two `scale` (FP16: exact, except weights below 2^-16 whose halving drops
+// their last bit into the subnormal range).
+__device__ __forceinline__ uint4 to_inf(uint4 v, float scale = 1.0f) {
 #if SD_MLP_F16
     uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const f16x2 h = {(_Float16)__uint_as_float(w[k] << 16), (_Float16)__uint_as_float(w[k] & 0xffff0000u)};
+        const f16x2 h = {(_Float16)(__uint_as_float(w[k] << 16) * scale),
+                         (_Float16)(__uint_as_float(w[k] & 0xffff0000u) * scale)};
         w[k] = __builtin_bit_cast(uint32_t, h);
     }
     return make_uint4(w[0], w[1], w[2], w[3]);
 #else
+    (void)scale;
     return v;
 #endif
 }
@@ -501,8 +522,8 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     float* sB = reinterpret_cast<float*>(sW4 + MLP_OUTP * WH_LD);  // b1q | b2q | b3q | b4 (padded to 32)
     for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
         const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
-        uint4 v = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i]);
-        if (BIAS_MM && c == MLP_INP / 8 - 1) v.w = bias_halves(a.b[r]);  // columns 30, 31 (zero weights)
+        uint4 v = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i], INF_IN_SCALE);
+        if (BIAS_MM && c == MLP_INP / 8 - 1) v.w = bias_halves(a.b[r] * INF_IN_SCALE);  // columns 30, 31 (zero weights)
         *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = v;
     }
     for (int i = threadIdx.x; i < MLP_H * (KH / 8); i += blockDim.x) {
@@ -514,20 +535,21 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
         } else {  // the bias k-step: columns 128, 129 = (bh, bl), the rest zero
             const bool first = c == MLP_H / 8;
             *reinterpret_cast<uint4*>(sW2 + r * WH_LD + 8 * c) =
-                make_uint4(first ? bias_halves(a.b[MLP_H + r]) : 0u, 0u, 0u, 0u);
+                make_uint4(first ? bias_halves(a.b[MLP_H + r] * INF_IN_SCALE) : 0u, 0u, 0u, 0u);
             *reinterpret_cast<uint4*>(sW3 + r * WH_LD + 8 * c) =
-                make_uint4(first ? bias_halves(a.b[2 * MLP_H + r]) : 0u, 0u, 0u, 0u);
+                make_uint4(first ? bias_halves(a.b[2 * MLP_H + r] * INF_IN_SCALE) : 0u, 0u, 0u, 0u);
         }
     }
     for (int i = threadIdx.x; i < MLP_OUTP * (MLP_H / 8); i += blockDim.x) {
         const int r = i / (MLP_H / 8), c = i % (MLP_H / 8);
-        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) = to_inf(reinterpret_cast<const uint4*>(a.W4q)[i]);
+        *reinterpret_cast<uint4*>(sW4 + r * WH_LD + 8 * c) =
+            to_inf(reinterpret_cast<const uint4*>(a.W4q)[i], INF_OUT_SCALE);
     }
     for (int i = threadIdx.x; i < 3 * MLP_H + MLP_OUTP; i += blockDim.x) {
         if (i < 3 * MLP_H) {
             // accumulator order of layer L: [t][g][j] <- bq[L H + 32 t + 16 (j / 8) + 8 g + (j % 8)]
             const int L = i / MLP_H, r = i % MLP_H, t = r / 32, g = (r / 16) & 1, j = r & 15;
-            sB[i] = a.bq[L * MLP_H + 32 * t + 16 * (j >> 3) + 8 * g + (j & 7)];
+            sB[i] = a.bq[L * MLP_H + 32 * t + 16 * (j >> 3) + 8 * g + (j & 7)] * INF_IN_SCALE;
         } else {
             sB[i] = i - 3 * MLP_H < MLP_OUT ? a.b[i] : 0.f;
         }

@@ -140,3 +140,51 @@ def test_features_and_fused_inference(gpu, oracle):
     s.run(2)
     s.attach_mlp(None)
     s.close()
+
+
+def test_fused_inference_vs_oracle(gpu, oracle):
+    """The fused inference kernel (FP16 operands: the BF16 weights converted
+    exactly, FP16 activations, polynomial GELU on packed FP16) against the
+    FP64 oracle forward on the same BF16 weights and FP64 features, beside the
+    BF16 GEMM path (BF16 activations, FP32 tanh GELU): the FP16 kernel computes
+    the BF16 network at least as closely as the BF16-activation path does. The
+    packaged trained network on a disk of 200,000 particles, G=128."""
+    from stardust_amd import ic, learned
+    _say("fused inference vs oracle")
+    d = ic.disk_scene(200000, 128, 5)
+    pos, vel = d["pos"], d["vel"]
+    n, G = pos.shape[0], 128
+    gravity = d["gravity"]
+    s = sd.Sim(sd.default_config(particleCount=n, gridSize=G, gravity=gravity, softeningCells=1.0),
+               sd.default_modes())
+    s.set_particles(pos, vel)
+    m, out_scale = learned.load_model()
+    W, b = m.get_params()
+    targets = np.sort(np.random.default_rng(3).choice(n, 4096, replace=False)).astype(np.int32)
+    # FP64 features from the oracle's density
+    rho = oracle.deposit_fixed(pos, G).astype(np.float64) / oracle.FIX_ONE
+    p = pos[targets, :3]
+    c = np.clip(np.floor(p * np.float32(G)).astype(int), 0, G - 1)
+    X = np.zeros((targets.size, 30))
+    q = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                X[:, q] = np.log1p(rho[np.clip(c[:, 2] + dz, 0, G - 1), np.clip(c[:, 1] + dy, 0, G - 1),
+                                       np.clip(c[:, 0] + dx, 0, G - 1)])
+                q += 1
+    X[:, 27:] = 2.0 * (p.astype(np.float64) * G - 0.5 - c)
+    a_unit = float(np.float32(gravity) * G / n)
+    ref = oracle.mlp_forward(bf16_round(W).astype(np.float64), b.astype(np.float64), H, X) * out_scale * a_unit
+    s.attach_mlp(m, out_scale)
+    fused = s.mlp_correction()[targets, :3].astype(np.float64)
+    s.attach_mlp(None)
+    path = m.forward(s.features(targets)).astype(np.float64) * out_scale * a_unit
+    s.close()
+
+    def rel(a):
+        return float(np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum()))
+    e_fused, e_path = rel(fused), rel(path)
+    _say("  rel rms vs oracle: fused FP16", e_fused, "BF16 path", e_path)
+    assert e_fused < 5e-3
+    assert e_fused <= e_path
