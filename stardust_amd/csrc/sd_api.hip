@@ -1006,7 +1006,7 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
         if (g_failed) return SD_ERR_DEVICE;
     }
     const bool structural = !s->ready || G != s->G || c->boundary != (s->periodic ? 1 : 0) ||
-                            c->particleCount != s->allocN ||
+                            c->particleCount != s->requestedN ||
                             (!s->periodic && c->softeningCells != s->cfg.softeningCells) ||
                             (s->periodic && c->softeningCells != s->cfg.softeningCells);
     if (structural && s->ready && s->borrowed)
@@ -1014,17 +1014,28 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
     s->cfg = *c;
     s->cfg.gridSize = G;
     if (!structural) return SD_OK;
-    const size_t need = sd_estimate_bytes(c->particleCount, G, c->boundary);
+    // Sim::reconfigure's memory policy (Sim.cu:4120-4135): the particle count is
+    // cut to what the free device memory holds; a grid that alone does not fit
+    // is a failure. (A batch scene's slice is fixed: it is not cut.)
+    int n = c->particleCount;
     const size_t freeB = sd_device_free_bytes();
-    if (freeB > 0 && need > freeB) {
-        mark_failure("configuration needs %.1f GB, %.1f GB free", need / 1e9, freeB / 1e9);
-        return SD_ERR_FAILED;
+    if (freeB > 0 && !s->borrowed) {
+        const int cap = sd_max_particles_for(G, c->boundary, freeB);
+        if (cap <= 0) {
+            mark_failure("grid %d does not fit the device (%.1f GB free)", G, freeB / 1e9);
+            return SD_ERR_FAILED;
+        }
+        if (n > cap) {
+            fprintf(stderr, "[stardust_amd] particle count %d cut to %d (%.1f GB free)\n", n, cap, freeB / 1e9);
+            n = cap;
+        }
     }
+    s->requestedN = c->particleCount;
     s->G = G;
     s->periodic = c->boundary == SD_BOUNDARY_PERIODIC ? 1 : 0;
     s->S = s->periodic ? G : 2 * G;
     s->P = fft_pitch(s->S);
-    s->allocN = c->particleCount;
+    s->allocN = n;
     return allocate(s);
 }
 
@@ -1345,13 +1356,12 @@ double sd_measure_kinetic_energy(sd_sim* s) {
 int sd_peak_cell_count(sd_sim* s) {
     if (check_ready(s)) return -1;
     const long long cells = (long long)s->G * s->G * s->G;
-    int* cnt = nullptr;
-    int* mx = nullptr;
-    if (!dalloc(cnt, (size_t)cells) || !dalloc(mx, 1)) {
-        dfree(cnt);
-        dfree(mx);
-        return -1;
-    }
+    // The counts borrow the acceleration grid's storage (16 B per cell, 4 used,
+    // plus the maximum after them); that grid is marked stale and rebuilt from
+    // the potential when something next reads it. Nothing is allocated.
+    int* cnt = reinterpret_cast<int*>(s->accG);
+    int* mx = cnt + cells;
+    s->accGValid = false;
     launch_fill_i32(cnt, cells, 0, s->st);
     launch_fill_i32(mx, 1, 0, s->st);
     if (s->allocN > 0)
@@ -1361,8 +1371,6 @@ int sd_peak_cell_count(sd_sim* s) {
     int h = 0;
     HK(hipMemcpyAsync(&h, mx, sizeof(int), hipMemcpyDeviceToHost, s->st));
     HK(hipStreamSynchronize(s->st));
-    dfree(cnt);
-    dfree(mx);
     return g_failed ? -1 : h;
 }
 

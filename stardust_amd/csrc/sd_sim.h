@@ -43,6 +43,7 @@ struct sd_sim {
     bool ready = false;
     int G = 0, S = 0, P = 0, periodic = 0;
     int allocN = 0;  // slots (mass per particle = gravity / allocN)
+    int requestedN = -1;  // the configured particle count (allocN may be cut to fit the device)
     int active = 0;  // slots in use (prefix), for state files
     float fixedDt = 0.f;
     int device = -1;  // HIP device current when the handle was first configured

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <dlfcn.h>
 #include <string>
 #include <vector>
 
@@ -39,14 +40,11 @@ struct Rng {
     }
 };
 
-// Exponential disk of radius ~0.2 box centred at 0.5 with circular speeds for
-// the given reference gravity (G_N,box * M = gravity / G on the isolated mesh).
-static void spiral_disk(int n, int G, float gravity, uint64_t seed, std::vector<float>& p, std::vector<float>& v) {
+// Exponential disk of radius ~0.2 box centred at 0.5 (positions only).
+static void disk_positions(int n, uint64_t seed, std::vector<float>& p) {
     p.assign((size_t)n * 4, 0.f);
-    v.assign((size_t)n * 4, 0.f);
     Rng r(seed);
     const double Rd = 0.05, Rmax = 0.22, h = 0.004;
-    const double GM = gravity / (double)G;
     const double cmax = 1.0 - (1.0 + Rmax / Rd) * std::exp(-Rmax / Rd);
     for (int i = 0; i < n; ++i) {
         // invert the enclosed fraction by bisection
@@ -59,14 +57,31 @@ static void spiral_disk(int n, int G, float gravity, uint64_t seed, std::vector<
         }
         const double R = 0.5 * (lo + hi), ph = 2.0 * M_PI * r.next();
         const double z = h * std::atanh(2.0 * std::fmin(std::fmax(r.next(), 1e-9), 1.0 - 1e-9) - 1.0);
-        const double menc = (1.0 - (1.0 + R / Rd) * std::exp(-R / Rd)) / cmax;
-        const double vc = std::sqrt(GM * menc / std::fmax(R, 1e-6));
         p[4 * i + 0] = (float)(0.5 + R * std::cos(ph));
         p[4 * i + 1] = (float)(0.5 + R * std::sin(ph));
         p[4 * i + 2] = (float)(0.5 + z);
-        v[4 * i + 0] = (float)(-vc * std::sin(ph));
-        v[4 * i + 1] = (float)(vc * std::cos(ph));
     }
+}
+
+// The reference's orbit rule (kSetOrbit, Sim.cu:413-480, no dispersion):
+// solve the mesh once for the placed particles and give each spin times the
+// circular speed v = sqrt(a_r r) of the measured inward acceleration a_r in
+// the xy plane about the box centre (spin = 1 - diskSpinLag, Sim.cu:460).
+// With gravity 0 the disk starts at rest.
+static bool set_orbits(Sim& sim, const std::vector<float>& p, int n, double spin = 1.0) {
+    std::vector<float> v((size_t)n * 4, 0.f), a((size_t)n * 4, 0.f);
+    if (!sim.setParticles(p.data(), v.data(), n)) return false;
+    if (sd_particle_accel(sim.handle(), a.data(), n) != SD_OK) return false;
+    for (int i = 0; i < n; ++i) {
+        const double dx = p[4 * i] - 0.5, dy = p[4 * i + 1] - 0.5;
+        const double r = std::sqrt(dx * dx + dy * dy);
+        if (r < 1e-5) continue;
+        const double ar = -(a[4 * i] * dx + a[4 * i + 1] * dy) / r;
+        const double s = spin * std::sqrt(std::fmax(ar, 0.0) * r);
+        v[4 * i + 0] = (float)(-dy / r * s);
+        v[4 * i + 1] = (float)(dx / r * s);
+    }
+    return sim.setParticles(p.data(), v.data(), n);
 }
 
 static void cloud(int n, float radius, uint64_t seed, std::vector<float>& p, std::vector<float>& v) {
@@ -96,10 +111,12 @@ static SimConfig plain(int n, int G) {
     return c;
 }
 
+// The reference tests' SpiralDisk scene: a rotating disk on orbits measured
+// on the simulation's own mesh.
 static bool seed_disk(Sim& sim, const SimConfig& c, uint64_t seed) {
-    std::vector<float> p, v;
-    spiral_disk(c.particleCount, sim.gridSize(), c.gravity, seed, p, v);
-    return sim.setParticles(p.data(), v.data(), c.particleCount);
+    std::vector<float> p;
+    disk_positions(c.particleCount, seed, p);
+    return set_orbits(sim, p, c.particleCount);
 }
 
 // [1] sim_tests.cpp:31-52 -- total grid mass preserved over 50 steps.
@@ -120,7 +137,11 @@ static void testMassConservation() {
     }
 }
 
-// [2] sim_tests.cpp:56-82 -- stronger gravity clusters more.
+// [2] sim_tests.cpp:56-82 -- stronger gravity clusters more. A rotating disk
+// on the reference's orbit rule with diskSpinLag 0.5: with gravity 0 it starts
+// at rest and keeps its initial peak; with gravity it contracts while it turns.
+// (On exact circular orbits, spin 1, the disk is in equilibrium and 900 steps
+// are under one orbital period at a tenth of the box: ratio 1.07.)
 static void testGravityResponds() {
     std::printf("\n[2] gravity responds\n");
     auto peakAfter = [](float gravity) {
@@ -128,12 +149,9 @@ static void testGravityResponds() {
         SimConfig cfg = plain(300000, 128);
         cfg.gravity = gravity;
         sim.init(cfg);
-        // the same cold disk (no rotation) in both runs: without gravity it
-        // stays put, with gravity it contracts
-        std::vector<float> p, v;
-        spiral_disk(cfg.particleCount, sim.gridSize(), 2.5f, 2, p, v);
-        std::fill(v.begin(), v.end(), 0.f);
-        sim.setParticles(p.data(), v.data(), cfg.particleCount);
+        std::vector<float> p;
+        disk_positions(cfg.particleCount, 2, p);
+        set_orbits(sim, p, cfg.particleCount, 0.5);
         for (int i = 0; i < 900; ++i) sim.step();
         return sim.measureMaxDensity();
     };
@@ -204,6 +222,109 @@ static void testCflClamp() {
     char buf[220];
     std::snprintf(buf, sizeof(buf), "dm=%.2e dc=%.4f dt=%.2e vmax=%.2f", dm, dc, t.dtUsed, t.maxSpeed);
     check(dm < 1e-2 && dc < 0.05, "mass and centroid held under strong gravity", buf);
+}
+
+// [8] sim_tests.cpp:213-231 -- 10,000 steps keep the total mass (G=256, S=512).
+static void testLongRun() {
+    std::printf("\n[8] long run (10,000 steps)\n");
+    Sim sim;
+    SimConfig cfg = plain(200000, 256);
+    cfg.gravity = 0.6f;
+    sim.init(cfg);
+    seed_disk(sim, cfg, 10);
+    const double m0 = sim.measureTotalGridMass();
+    for (int i = 0; i < 10000; ++i) sim.step();
+    const double m1 = sim.measureTotalGridMass();
+    const double rel = std::fabs(m1 - m0) / (m0 > 0 ? m0 : 1.0);
+    char buf[180];
+    std::snprintf(buf, sizeof(buf), "m0=%.0f m10000=%.0f rel=%.2e t=%.4f", m0, m1, rel, sim.simTime());
+    check(m0 > 1.0 && rel < 1e-2, "total mass kept after 10,000 steps", buf);
+}
+
+// [9] sim_tests.cpp:236-265 -- a request the device cannot hold is cut to the
+// largest possible count and still steps. This device holds far more than the
+// reference's 500 M particles, so the test first fills all but ~8 GB of its
+// free memory (HIP runtime loaded by name), then asks for 400 M particles.
+static void testVramClamp() {
+    std::printf("\n[9] particle count cut to the free memory\n");
+    void* hip = dlopen("libamdhip64.so", RTLD_NOW | RTLD_GLOBAL);
+    typedef int (*MallocFn)(void**, size_t);
+    typedef int (*FreeFn)(void*);
+    MallocFn hipMallocFn = hip ? (MallocFn)dlsym(hip, "hipMalloc") : nullptr;
+    FreeFn hipFreeFn = hip ? (FreeFn)dlsym(hip, "hipFree") : nullptr;
+    if (!hipMallocFn || !hipFreeFn) {
+        check(false, "HIP runtime for the memory filler", dlerror() ? dlerror() : "");
+        return;
+    }
+    const size_t keep = 8ull << 30;
+    size_t freeB = Sim::deviceFreeBytes();
+    void* filler = nullptr;
+    if (freeB > keep + (1ull << 30) && hipMallocFn(&filler, freeB - keep) != 0) filler = nullptr;
+    const int gCap = Sim::maxGridSize(Boundary::Isolated);
+    freeB = Sim::deviceFreeBytes();
+    const int maxN = Sim::maxParticlesFor(gCap, Boundary::Isolated, freeB);
+    Sim sim;
+    SimConfig cfg = plain(400000000, gCap);
+    sim.init(cfg);
+    const int got = sim.particleCount();
+    char buf[220];
+    std::snprintf(buf, sizeof(buf), "asked 400000000 got %d bound %d free %.0f MB", got, maxN, freeB / 1048576.0);
+    check(!Sim::failed() && got > 1000 && got <= maxN && got < 400000000, "an oversized request is cut to fit", buf);
+    std::vector<float> p, v;
+    cloud(got, 0.2f, 11, p, v);
+    sim.setParticles(p.data(), v.data(), got);
+    sim.step();
+    const double mass = sim.measureTotalGridMass();
+    std::snprintf(buf, sizeof(buf), "grid mass %.0f for %d particles", mass, got);
+    check(std::fabs(mass - got) / got < 1e-2, "the cut simulation steps normally", buf);
+    if (filler) hipFreeFn(filler);
+}
+
+// [11] sim_tests.cpp:345-372 -- a lower time scale slows simulated time.
+static void testTimeScale() {
+    std::printf("\n[11] time scale\n");
+    auto simTimeAfter = [](float ts, float& dtOut) {
+        Sim sim;
+        SimConfig cfg = plain(200000, 256);
+        cfg.timeScale = ts;
+        sim.init(cfg);
+        seed_disk(sim, cfg, 12);
+        for (int i = 0; i < 100; ++i) sim.step();
+        dtOut = sim.timings().dtUsed;
+        return sim.simTime();
+    };
+    float dtSlow = 0.f, dtNorm = 0.f;
+    const double tSlow = simTimeAfter(0.25f, dtSlow);
+    const double tNorm = simTimeAfter(1.0f, dtNorm);
+    char buf[220];
+    std::snprintf(buf, sizeof(buf), "ts 0.25: t=%.6f dt=%.7f / ts 1: t=%.6f dt=%.7f (ratio %.2f)", tSlow, dtSlow,
+                  tNorm, dtNorm, tSlow > 0 ? tNorm / tSlow : 0.0);
+    check(tNorm > tSlow * 2.5, "a lower time scale slows simulated time", buf);
+}
+
+// [21] sim_tests.cpp:686-728 -- contact keeps particles from piling into one
+// cell: the peak density stays well below the contact-free run's, mass kept.
+static void testContactBlocksOverlap() {
+    std::printf("\n[21] contact bounds the peak density\n");
+    auto run = [](bool contact, double& maxDensity, double& mass) {
+        Sim sim;
+        SimConfig cfg = plain(200000, 128);
+        cfg.gravity = 0.6f;
+        cfg.contactEnabled = contact;
+        sim.init(cfg);
+        seed_disk(sim, cfg, 13);
+        for (int i = 0; i < 800; ++i) sim.step();
+        maxDensity = sim.measureMaxDensity();
+        mass = sim.measureTotalGridMass();
+    };
+    double dOn = 0, dOff = 0, mOn = 0, mOff = 0;
+    run(true, dOn, mOn);
+    run(false, dOff, mOff);
+    char buf[240];
+    std::snprintf(buf, sizeof(buf), "contact on: peak %.1f (mass %.0f), off: peak %.1f (mass %.0f), %.1fx", dOn, mOn,
+                  dOff, mOff, dOn > 0 ? dOff / dOn : 0.0);
+    check(dOn < dOff * 0.5 && dOn > 0.0 && mOn > 200000.0 * 0.98 && mOn <= 200000.0 * 1.001,
+          "contact bounds the peak density and keeps the mass", buf);
 }
 
 // [13] sim_tests.cpp:423-458 -- softening acts on the periodic mesh.
@@ -297,9 +418,13 @@ int main() {
     testReconfigure();
     testTimings();
     testCflClamp();
+    testLongRun();
+    testTimeScale();
+    testContactBlocksOverlap();
     testPeriodicSoftening();
     testNoReallocOnSameRequest();
     testStateRoundTrip();
+    testVramClamp();
     testRefusal();  // last: it leaves the sticky failure set
     std::printf("\n=== result: %d PASS / %d FAIL ===\n", g_pass, g_fail);
     return g_fail == 0 ? 0 : 1;

@@ -448,3 +448,76 @@ def test_step_sort_is_a_permutation():
     k, _ = O.cell_keys(st.pos, G, O.KEY_ROWMAJOR)
     # the sort ran before the drift: keys of the pre-drift positions were ascending
     assert np.mean(np.diff(k.astype(np.int64)) >= 0) > 0.95
+
+
+# ---- Stepper schedule, closed forms -------------------------------------------
+def _kd_harmonic(y0, v0, w2, dt, k):
+    """Kick-drift (v += a dt; x += v dt) of y'' = -w2 y: [y, v] <- M [y, v]."""
+    M = np.array([[1.0 - w2 * dt * dt, dt], [-w2 * dt, 1.0]])
+    Mk = np.linalg.matrix_power(M, k)
+    return Mk[0, 0] * y0 + Mk[0, 1] * v0, Mk[1, 0] * y0 + Mk[1, 1] * v0
+
+
+def test_step_harmonic_closed_form():
+    """Gravity off, dark energy -w^2: every particle is a harmonic oscillator
+    about the box centre, and the reference's kick-drift step is the linear map
+    M = [[1 - w^2 dt^2, dt], [-w^2 dt, 1]] per axis (Sim.cu:822, 842), so k
+    steps are M^k, across re-sorts (every 2 steps). Drift-kick order, a
+    dropped kick or a wrong sign fails this."""
+    n, k, dt, w2 = 3000, 9, 2e-3, 40.0
+    rng = np.random.default_rng(31)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.4, 0.6, (n, 3))
+    pos[:, 3] = np.arange(1, n + 1)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.05, (n, 3))
+    prm = O.make_params(16, gravity=0.0, softeningCells=1.0, sortInterval=2, fixedDt=dt, momentumZero=0,
+                        darkEnergy=-w2, allocN=n)
+    st = O.Stepper(prm, pos, vel)
+    st.step(k)
+    o = np.argsort(st.pos[:, 3])
+    y, v = _kd_harmonic(pos[:, :3].astype(np.float64) - 0.5, vel[:, :3].astype(np.float64), w2, dt, k)
+    assert np.abs(st.pos[o, :3] - 0.5 - y).max() < 2e-7
+    assert np.abs(st.vel[o, :3] - v).max() < 2e-6
+    assert st.state.simTime == pytest.approx(k * dt, rel=1e-6)
+
+
+def test_step_momentum_removed_after_drift():
+    """Uniform drift u with momentum removal: the first step drifts by u dt
+    with the velocities as loaded, then removes the mean (Sim.cu:4496-4501),
+    so every later step leaves the particles in place."""
+    n, dt = 2000, 1e-3
+    pos = rand_pos(n, 32, lo=0.3, hi=0.7)
+    u = np.array([0.3, -0.2, 0.1], np.float32)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = u
+    prm = O.make_params(16, gravity=0.0, softeningCells=1.0, sortInterval=0, fixedDt=dt, momentumZero=1, allocN=n)
+    st = O.Stepper(prm, pos, vel)
+    st.step(1)
+    x1 = pos[:, :3] + u * np.float32(dt)
+    assert np.array_equal(st.pos[:, :3], x1.astype(np.float32))
+    assert np.abs(st.vel[:, :3]).max() == 0.0
+    st.step(3)
+    assert np.array_equal(st.pos[:, :3], x1.astype(np.float32))
+
+
+@pytest.mark.parametrize("speed,ts", [(2.0, 1.0), (0.01, 1.0), (0.01, 0.25), (0.01, 6.0)])
+def test_step_cfl_closed_form(speed, ts):
+    """Every particle at one speed: dt = min(0.0016 dtScaleFor(ts), 0.8 / (G v))
+    each step (Sim.cu:4259-4283, Sim.h:89-93), positions x + k v dt and time
+    k dt."""
+    n, G, k = 1000, 32, 5
+    pos = rand_pos(n, 33, lo=0.4, hi=0.45)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = speed
+    prm = O.make_params(G, gravity=0.0, softeningCells=1.0, sortInterval=0, timeScale=ts, momentumZero=0, allocN=n)
+    st = O.Stepper(prm, pos, vel)
+    st.step(k)
+    scale = ts if ts <= 1.0 else (1.0 if ts <= 3.0 else ts / 3.0)
+    dt = min(np.float32(0.0016) * np.float32(scale), np.float32(0.8) * np.float32(1.0 / G) / np.float32(speed))
+    x = pos[:, 0].copy()
+    for _ in range(k):
+        x = (x.astype(np.float64) + np.float64(np.float32(speed)) * np.float64(dt)).astype(np.float32)
+    assert st.state.lastDt == pytest.approx(float(dt), rel=1e-6)
+    assert st.state.simTime == pytest.approx(k * float(dt), rel=1e-6)
+    assert np.abs(st.pos[:, 0] - x).max() <= 1e-7
