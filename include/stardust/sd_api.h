@@ -186,6 +186,21 @@ int sd_set_fixed_dt(sd_sim* s, float dt);  /* dt > 0 replaces the CFL rule; 0 re
 const float* sd_particle_pos_device_ptr(sd_sim* s);
 const float* sd_particle_vel_device_ptr(sd_sim* s);
 
+/* Density views: Sim::densityDevicePtr / Sim::fieldDevicePtr (Sim.h:1083-1084).
+ * sd_density_device_ptr: float[G^3] ([z][y][x], row pitch G), particles per
+ * cell of the current positions (the reference's rho is its S-pitched padded
+ * grid with the density in the G corner). sd_field_device_ptr(SD_FIELD_DENSITY):
+ * float[G^2] ([y][x]), the density summed along z (kProjectXY without a view
+ * rotation); the other fields of Sim::Field come from off-path physics and
+ * return NULL. Both recompute on each call, synchronise the handle's stream
+ * before returning and are invalidated by the next step or call. */
+#define SD_FIELD_DENSITY 0
+#define SD_FIELD_DISPERSION 1
+#define SD_FIELD_LIGHT 2
+#define SD_FIELD_ASH 3
+const float* sd_density_device_ptr(sd_sim* s);
+const float* sd_field_device_ptr(sd_sim* s, int field);
+
 /* ---- parity dumps (no reference counterpart) ----------------------------- */
 int sd_get_grid(sd_sim* s, int which, void* out, size_t bytes);
 int sd_compute_forces(sd_sim* s);  /* deposit + solve + gradient for current positions */

@@ -38,6 +38,7 @@ PIPELINE_REFERENCE = 0
 PIPELINE_FUSED = 1
 GRID_DENSITY_FIXED = 0
 GRID_DENSITY = 1
+FIELD_DENSITY = 0
 GRID_POTENTIAL = 2
 GRID_ACCEL = 3
 FIX_ONE = 1 << 24
@@ -171,6 +172,8 @@ def _declare(L):
         "sd_particle_accel": (C.c_int, [P, P, C.c_int]),
         "sd_direct_sum": (C.c_int, [P, P, C.c_int, C.c_double, C.c_double, P]),
         "sd_direct_sum_noalloc": (C.c_int, [P, P, C.c_int, C.c_double, C.c_double, P]),
+        "sd_density_device_ptr": (C.c_void_p, [P]),
+        "sd_field_device_ptr": (C.c_void_p, [P, C.c_int]),
         "sd_mlp_param_counts": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sd_mlp_create": (P, [C.c_uint]),
         "sd_mlp_destroy": (None, [P]),
@@ -386,6 +389,20 @@ class Sim:
             out = np.empty((G, G, G), np.float32)
         _check(lib().sd_get_grid(self._h, which, _ptr(out), out.nbytes), "sd_get_grid")
         return out
+
+    def density_device_ptr(self) -> int:
+        """Sim::densityDevicePtr: device address of float[G^3] particles per cell."""
+        p = lib().sd_density_device_ptr(self._h)
+        if not p:
+            raise StardustError("sd_density_device_ptr: " + fail_message())
+        return p
+
+    def field_device_ptr(self, field: int = 0) -> int:
+        """Sim::fieldDevicePtr: device address of float[G^2] (density summed along z)."""
+        p = lib().sd_field_device_ptr(self._h, field)
+        if not p:
+            raise StardustError("sd_field_device_ptr: " + fail_message())
+        return p
 
     def compute_forces(self):
         _check(lib().sd_compute_forces(self._h), "sd_compute_forces")

@@ -152,6 +152,8 @@ bool Sim::loadState(const std::string& path) {
     cfg_.preset = Preset::Empty;
     return true;
 }
+const float* Sim::fieldDevicePtr(Field field) { return sd_field_device_ptr(h_, (int)field); }
+const float* Sim::densityDevicePtr() const { return sd_density_device_ptr(h_); }
 const float* Sim::particlePosDevicePtr() const { return sd_particle_pos_device_ptr(h_); }
 const float* Sim::particleVelDevicePtr() const { return sd_particle_vel_device_ptr(h_); }
 

@@ -139,6 +139,10 @@ public:
     bool saveState(const std::string& path);
     bool loadState(const std::string& path);
 
+    // Sim::Field (Sim.h:1078); only Density is on this path
+    enum class Field { Density, Dispersion, Light, Ash };
+    const float* fieldDevicePtr(Field field);  // G^2 floats: density summed along z
+    const float* densityDevicePtr() const;     // G^3 floats, row pitch G
     const float* particlePosDevicePtr() const;  // float4 array as float*
     const float* particleVelDevicePtr() const;
 

@@ -140,7 +140,7 @@ static void free_buffers(sd_sim* s) {
     }
     dfree(s->keysA); dfree(s->keysB); dfree(s->idxA); dfree(s->idxB);
     dfree(s->hist); dfree(s->chunk);
-    dfree(s->rhoFix); dfree(s->rhoF); dfree(s->A); dfree(s->B); dfree(s->green);
+    dfree(s->rhoFix); dfree(s->rhoF); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
     dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
     s->cellTableCells = 0;
@@ -233,7 +233,8 @@ static int allocate(sd_sim* s) {
     const int ntiles = sort_tiles_for((int)N);
     ok = ok && dalloc(s->hist, (size_t)256 * ntiles) && dalloc(s->chunk, (size_t)sort_chunks_for((int)N) + 1);
     ok = ok && dalloc(s->rhoFix, cells) && dalloc(s->A, (size_t)G * G * P) && dalloc(s->B, (size_t)G * S * P) &&
-         dalloc(s->pot, cells) && dalloc(s->accG, cells) && dalloc(s->tw, (size_t)S);
+         dalloc(s->pot, cells) && dalloc(s->accG, cells) && dalloc(s->tw, (size_t)S) &&
+         dalloc(s->proj, (size_t)G * G);
     if (!s->periodic) ok = ok && dalloc(s->green, (size_t)(S / 2 + 1) * (S / 2 + 1) * P);
     s->nparts = particle_blocks((int)N);
     s->npartsInt = integrate_blocks((int)N);
@@ -1424,6 +1425,55 @@ int sd_set_fixed_dt(sd_sim* s, float dt) {
     s->fixedDt = dt;
     s->primed = false;
     return SD_OK;
+}
+
+// Density views (Sim::densityDevicePtr / fieldDevicePtr, Sim.cu:5151, 5229-5240)
+__global__ void k_fix_to_count(const long long* __restrict__ q, long long n, float* __restrict__ f) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = (float)((double)q[i] * (1.0 / (double)SD_FIX_ONE));
+}
+
+// kProjectXY without a view rotation: out[y][x] = sum over z of rho[z][y][x],
+// added in ascending z (one thread per column, no atomics).
+__global__ void k_project_z(const float* __restrict__ rho, int G, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G * G) return;
+    const size_t plane = (size_t)G * G;
+    float acc = 0.f;
+    for (int z = 0; z < G; ++z) acc += rho[(size_t)z * plane + i];
+    out[i] = acc;
+}
+
+// The float density of the current positions (particles per cell) in rhoF.
+static bool density_view(sd_sim* s) {
+    if (!ensure_float_rho(s)) return false;
+    const long long cells = (long long)s->G * s->G * s->G;
+    const bool clearAfter = query_density(s);
+    hipLaunchKernelGGL(k_fix_to_count, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->st, s->rhoFix, cells,
+                       s->rhoF);
+    if (clearAfter) launch_fill_i64(s->rhoFix, cells, s->st);
+    return true;
+}
+
+const float* sd_density_device_ptr(sd_sim* s) {
+    if (check_ready(s)) return nullptr;
+    if (!density_view(s)) return nullptr;
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? nullptr : s->rhoF;
+}
+
+const float* sd_field_device_ptr(sd_sim* s, int field) {
+    if (check_ready(s)) return nullptr;
+    if (field != SD_FIELD_DENSITY) {
+        arg_error("sd_field_device_ptr: only the density field is on this path (dispersion, light and ash "
+                  "come from the star and gas physics)");
+        return nullptr;
+    }
+    if (!density_view(s)) return nullptr;
+    hipLaunchKernelGGL(k_project_z, dim3((unsigned)((s->G * s->G + 255) / 256)), dim3(256), 0, s->st, s->rhoF, s->G,
+                       s->proj);
+    HK(hipStreamSynchronize(s->st));
+    return g_failed ? nullptr : s->proj;
 }
 
 const float* sd_particle_pos_device_ptr(sd_sim* s) { return s && s->ready ? (const float*)s->pos : nullptr; }

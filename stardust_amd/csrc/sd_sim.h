@@ -63,7 +63,8 @@ struct sd_sim {
 
     // mesh
     long long* rhoFix = nullptr;
-    float* rhoF = nullptr;
+    float* rhoF = nullptr;   // float density (float deposit mode; the density view)
+    float* proj = nullptr;   // G^2 projection of the density along z (the density field view)
     float2 *A = nullptr, *B = nullptr;
     float* green = nullptr;
     float* pot = nullptr;

@@ -135,7 +135,7 @@ def zeldovich(n_side=256, seed=1, rms_cells=0.3, G_mesh=512, index=-2.0):
     disp = []
     for kk in (kx, ky, kz):
         psi_k = (1j * kk / k2) * delta_k
-        disp.append(np.fft.irfftn(psi_k, s=(ns, ns, ns)).astype(np.float32))
+        disp.append(np.fft.irfftn(psi_k, s=(ns, ns, ns), axes=(0, 1, 2)).astype(np.float32))
     d = np.stack([disp[0], disp[1], disp[2]], axis=-1)
     rms = float(np.sqrt((d.astype(np.float64) ** 2).sum(axis=-1).mean()))
     d *= (rms_cells / G_mesh) / max(rms, 1e-30)

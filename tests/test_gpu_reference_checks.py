@@ -223,3 +223,37 @@ def test_peak_cell_count(gpu, oracle):
     assert s.peak_cell_count() == want
     assert np.array_equal(s.get_grid(sd.GRID_ACCEL), acc0)
     s.close()
+
+
+def test_density_device_views(gpu):
+    """Sim::densityDevicePtr / fieldDevicePtr(Density) (Sim.cu:5151, 5229-5240):
+    the G^3 density of the current positions and its sum along z, read
+    straight from device memory; the off-path fields are refused."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    n, G = 100000, 32
+    rng = np.random.default_rng(54)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.05, 0.95, (n, 3))
+    s = sd.Sim(sd.default_config(particleCount=n, gridSize=G, sortInterval=2))
+    s.set_particles(pos, np.zeros_like(pos))
+    s.run(3)
+    rho = s.get_grid(sd.GRID_DENSITY)
+    d = np.empty((G, G, G), np.float32)
+    assert hip.hipMemcpy(d.ctypes.data, s.density_device_ptr(), d.nbytes, 2) == 0
+    assert np.array_equal(d, rho)
+    proj = np.empty((G, G), np.float32)
+    assert hip.hipMemcpy(proj.ctypes.data, s.field_device_ptr(sd.FIELD_DENSITY), proj.nbytes, 2) == 0
+    want = np.zeros((G, G), np.float32)
+    for z in range(G):
+        want += rho[z]
+    assert np.array_equal(proj, want)
+    assert abs(float(proj.astype(np.float64).sum()) - n) < 1e-2
+    with pytest.raises(sd.StardustError):
+        s.field_device_ptr(2)  # light: star physics, off this path
+    assert not sd.failed()
+    # the views do not disturb the step: a run continues as before
+    s.run(2)
+    assert s.measure_total_grid_mass() == pytest.approx(n, abs=0)
+    s.close()
