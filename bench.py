@@ -31,12 +31,14 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def integrate_bytes(n_live, G, contact=False):
+def integrate_bytes(n_live, G, contact=False, deposit=True):
     """Algorithmic bytes of one k_integrate launch: pos + vel read and written
     (64 B per particle) + the potential read once (4 B per cell; the gather
     takes its differences) + the fixed-point density written once (8 B per
-    cell) (+ the contact acceleration, 16 B per particle, when contact is on)."""
-    return (80.0 if contact else 64.0) * n_live + 12.0 * G ** 3
+    cell) when the integrate deposits (+ the contact acceleration, 16 B per
+    particle, when contact is on). Sparse contact steps deposit in a gather
+    kernel of their own instead (deposit=False)."""
+    return (80.0 if contact else 64.0) * n_live + (12.0 if deposit else 4.0) * G ** 3
 
 
 def cpu_model():
@@ -249,7 +251,10 @@ def main():
     roof = None
     if launches > 0:
         avg_s = kms / launches / 1e3
-        nbytes = integrate_bytes(n_live, G, contact)
+        # the cosmic web's sparse contact steps (0.125 particles per cell) deposit
+        # in the gather kernel, not in the integrate
+        gather = contact and n < 0.5 * G ** 3
+        nbytes = integrate_bytes(n_live, G, contact, deposit=not gather)
         ach = nbytes / avg_s / 1e9
         # HBM traffic needs counter runs (rocprofv3 --pmc), which this run does
         # not make: `traffic` stays null and the committed counter summary of

@@ -134,6 +134,17 @@ static void timer_harvest(sd_sim* s) {
     } while (0)
 #define TIMED(s, name, stmt) TIMED_ON(s, (s)->st, name, stmt)
 
+// Zero the fixed-point density grid (and note it is clean).
+static void fill_rho(sd_sim* s) {
+    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    s->rhoDirty = false;
+}
+// Atomic deposits add into rhoFix and need it zero: a grid an x pass left
+// unconsumed for a gather deposit is cleared first.
+static void clean_rho(sd_sim* s) {
+    if (s->rhoDirty) fill_rho(s);
+}
+
 static void free_buffers(sd_sim* s) {
     if (!s->borrowed) {
         dfree(s->pos); dfree(s->vel); dfree(s->posTmp); dfree(s->velTmp);
@@ -267,7 +278,7 @@ static int allocate(sd_sim* s) {
         const float sg = 6.2831853f * s->cfg.softeningCells / (float)G;
         s->plan.perSoft = s->cfg.softeningCells > 0.f ? sg * sg : 0.f;
     }
-    launch_fill_i64(s->rhoFix, (long long)cells, s->st);
+    fill_rho(s);
     build_green(s);
     if (g_failed) return SD_ERR_DEVICE;
     // all slots inactive
@@ -394,6 +405,7 @@ static void deposit_now(sd_sim* s) {
         TIMED(s, "deposit", launch_deposit_float(s->pos, s->allocN, s->G, s->periodic, s->rhoF, s->st));
     } else {
         // rhoFix is zero here (allocated zero, or cleared by the mesh pass)
+        clean_rho(s);
         TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
         s->rhoValid = true;
     }
@@ -418,6 +430,8 @@ static void grad_now(sd_sim* s, hipStream_t st) {
     s->accGValid = true;
 }
 
+static bool gather_deposit(const sd_sim* s);
+
 // The fused step's integrate differentiates the potential in its gather, so
 // it solves without the gradient (grad = false); the acceleration grid is
 // then made only when something reads it (ensure_mesh).
@@ -428,7 +442,11 @@ static void solve_mesh(sd_sim* s, hipStream_t st, bool grad) {
                            s->rhoFix, cells);
     }
     const FinalizeJob* fin = s->finDeferred ? &s->finJob : nullptr;
-    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, 1, fin, st));
+    // The x pass clears the density it reads for the atomic deposit of the
+    // next integrate; a gather deposit writes every node, so it needs no clear.
+    const bool keep = gather_deposit(s);
+    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, keep ? 0 : 1, fin, st));
+    s->rhoDirty = keep;
     TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, fin, st));
     s->finDeferred = false;
     if (grad) grad_now(s, st);
@@ -458,6 +476,21 @@ static void ensure_mesh(sd_sim* s, bool grad = true) {
 static bool contact_due(const sd_sim* s) {
     const int G = s->G;
     return s->cfg.contactEnabled && (double)s->allocN <= 0.764 * (double)G * G * G;
+}
+
+// Contact steps at under SD_GATHER_DENSITY particles per cell deposit in
+// gather form (launch_deposit_gather): the contact sort's Morton block runs
+// locate each tile's particles, and every node is written once.
+#ifndef SD_GATHER_DEPOSIT
+#define SD_GATHER_DEPOSIT 1
+#endif
+#ifndef SD_GATHER_DENSITY
+#define SD_GATHER_DENSITY 0.5
+#endif
+static bool gather_deposit(const sd_sim* s) {
+    return SD_GATHER_DEPOSIT && contact_due(s) && s->modes.keyMode == SD_KEY_MORTON &&
+           s->modes.pipeline == SD_PIPELINE_FUSED && s->modes.depositMode == SD_DEPOSIT_FIXED && !s->borrowed &&
+           gather_deposit_supported(s->G, s->periodic) && (double)s->allocN < SD_GATHER_DENSITY * (double)s->G * s->G * s->G;
 }
 
 static bool contact_sort(sd_sim* s, bool keysOnly) {
@@ -529,7 +562,10 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     const bool hadRho = s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED;
     if (!hadRho) {
         // rhoFix is zero whenever it is not valid (fixed-point invariant)
-        if (s->modes.depositMode == SD_DEPOSIT_FLOAT) launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+        if (s->modes.depositMode == SD_DEPOSIT_FLOAT)
+            fill_rho(s);
+        else
+            clean_rho(s);
         TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
     }
     MlpInferArgs a{};
@@ -549,7 +585,7 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
         if (s->modes.depositMode == SD_DEPOSIT_FIXED && !s->meshValid) {
             s->rhoValid = true;
         } else {
-            launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+            fill_rho(s);
         }
     }
     return true;
@@ -663,7 +699,9 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         if (!s->primed) prime_scalars(s);
         a.applyMeanSub = s->modes.momentumZero ? 1 : 0;
         const bool fusedDeposit = s->modes.depositMode == SD_DEPOSIT_FIXED;
-        a.depositTo = fusedDeposit ? s->rhoFix : nullptr;
+        const bool gatherDep = fusedDeposit && contact && gather_deposit(s);
+        a.depositTo = fusedDeposit && !gatherDep ? s->rhoFix : nullptr;
+        if (a.depositTo) clean_rho(s);
         a.partials = s->partials;
         if (reorderInIntegrate) {
             a.order = s->sb.idx;
@@ -676,6 +714,13 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         if (reorderInIntegrate) {
             std::swap(s->pos, s->posTmp);
             std::swap(s->vel, s->velTmp);
+        }
+        if (gatherDep) {
+            // the new positions are in the contact sort's order (sb.keysAlt is
+            // the sort's free buffer: the far-mover list)
+            TIMED(s, "deposit", launch_deposit_gather(s->pos, s->blockRange, s->G, s->periodic, s->rhoFix,
+                                                      reinterpret_cast<int32_t*>(s->sb.keysAlt), s->sc, s->st));
+            s->rhoDirty = false;
         }
         const int mode = (s->modes.momentumZero ? 1 : 0) | 2 | 8 | 4;
         if (last) {
@@ -968,7 +1013,7 @@ int sd_set_modes(sd_sim* s, const sd_modes* m) {
         else
             std::swap(s->vel, s->velTmp);
         hipLaunchKernelGGL(k_restart_integrator, dim3(1), dim3(1), 0, s->st, s->sc);
-        launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+        fill_rho(s);
         s->rhoValid = false;
         s->meshValid = false;
         s->primed = false;
@@ -1054,7 +1099,7 @@ int sd_reset(sd_sim* s) {
     std::vector<float4> hv((size_t)s->allocN, make_float4(0.f, 0.f, 0.f, 0.f));
     HK(hipMemcpy(s->pos, hp.data(), sizeof(float4) * hp.size(), hipMemcpyHostToDevice));
     HK(hipMemcpy(s->vel, hv.data(), sizeof(float4) * hv.size(), hipMemcpyHostToDevice));
-    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    fill_rho(s);
     s->active = 0;
     s->rhoValid = s->meshValid = s->keysSorted = false;
     reset_scalars(s);
@@ -1234,7 +1279,7 @@ __global__ void k_gather_acc(const float4* __restrict__ accG, const float4* __re
 static bool query_density(sd_sim* s) {
     if (s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED) return false;
     const bool clearAfter = s->meshValid && s->modes.pipeline == SD_PIPELINE_FUSED;
-    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    fill_rho(s);
     TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
     if (!clearAfter && s->modes.depositMode == SD_DEPOSIT_FIXED) s->rhoValid = true;
     return clearAfter || s->modes.depositMode != SD_DEPOSIT_FIXED;
@@ -1260,7 +1305,7 @@ static GridStats grid_stats(sd_sim* s) {
     HK(hipMemcpyAsync(hs.data(), dsum, sizeof(double) * nb, hipMemcpyDeviceToHost, s->st));
     HK(hipMemcpyAsync(ho.data(), docc, sizeof(long long) * nb, hipMemcpyDeviceToHost, s->st));
     HK(hipMemcpyAsync(&hm, dmax, sizeof(hm), hipMemcpyDeviceToHost, s->st));
-    if (clearAfter) launch_fill_i64(s->rhoFix, cells, s->st);
+    if (clearAfter) fill_rho(s);
     HK(hipStreamSynchronize(s->st));
     for (int b = 0; b < nb; ++b) {
         r.sum += hs[b];
@@ -1394,7 +1439,7 @@ int sd_set_particles(sd_sim* s, const float* pos4, const float* vel4, int n) {
         HK(hipMemcpy(s->vel + n, hv.data(), sizeof(float4) * hv.size(), hipMemcpyHostToDevice));
     }
     s->active = n;
-    launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    fill_rho(s);
     s->rhoValid = s->meshValid = s->keysSorted = false;
     s->corrValid = false;
     // keep simTime, restart the integrator state
@@ -1451,7 +1496,7 @@ static bool density_view(sd_sim* s) {
     const bool clearAfter = query_density(s);
     hipLaunchKernelGGL(k_fix_to_count, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->st, s->rhoFix, cells,
                        s->rhoF);
-    if (clearAfter) launch_fill_i64(s->rhoFix, cells, s->st);
+    if (clearAfter) fill_rho(s);
     return true;
 }
 
@@ -1498,7 +1543,7 @@ int sd_get_grid(sd_sim* s, int which, void* out, size_t bytes) {
         const bool clearAfter = query_density(s);
         std::vector<long long> h(cells);
         HK(hipMemcpyAsync(h.data(), s->rhoFix, cells * 8, hipMemcpyDeviceToHost, s->st));
-        if (clearAfter) launch_fill_i64(s->rhoFix, (long long)cells, s->st);
+        if (clearAfter) fill_rho(s);
         HK(hipStreamSynchronize(s->st));
         if (which == SD_GRID_DENSITY_FIXED) {
             memcpy(out, h.data(), cells * 8);
@@ -1805,7 +1850,7 @@ int sd_features(sd_sim* s, const int32_t* targets, int nt, float* X) {
     HK(hipMemcpyAsync(dt, targets, sizeof(int32_t) * nt, hipMemcpyHostToDevice, s->st));
     launch_features(s->pos, dt, nt, s->rhoFix, s->G, s->periodic, dx, s->st);
     HK(hipMemcpyAsync(X, dx, sizeof(float) * MLP_IN * (size_t)nt, hipMemcpyDeviceToHost, s->st));
-    if (clearAfter) launch_fill_i64(s->rhoFix, (long long)s->G * s->G * s->G, s->st);
+    if (clearAfter) fill_rho(s);
     HK(hipStreamSynchronize(s->st));
     dfree(dt);
     dfree(dx);

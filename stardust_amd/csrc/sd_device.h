@@ -94,7 +94,7 @@ struct StepScalars {
     double mom[4];     // last momentum sums (x, y, z, count)
     long long stepCount;
     int kdkOpen;       // 1 until the first leapfrog step has been taken
-    int pad;
+    int farCount;      // far movers listed by the gather deposit (0 between steps)
 };
 
 struct ReducePartial {  // one per integrate block

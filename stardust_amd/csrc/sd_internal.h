@@ -104,6 +104,15 @@ void launch_deposit_fixed(const float4* pos, int n, int G, int periodic, long lo
                           hipStream_t st);
 void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* rho,
                           hipStream_t st);
+// Gather-form deposit of positions sorted by 4^3-cell Morton blocks (a contact
+// step's order, with blockRange the sorted index range of each block): each
+// 32 x 16 x 8 node tile adds the exact fixed-point weights of the particles in
+// the surrounding blocks and writes every node once (no clearing, no global
+// atomics). Particles more than the halo allows from their block are listed
+// and added by a one-block atomic pass, which also resets sc->farCount.
+bool gather_deposit_supported(int G, int periodic);
+void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int periodic, long long* rho,
+                           int32_t* farList, StepScalars* sc, hipStream_t st);
 void launch_fill_i64(long long* p, long long n, hipStream_t st);
 void launch_fill_f32(float* p, long long n, float v, hipStream_t st);
 void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st);

@@ -398,6 +398,202 @@ void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* 
     if (n > 0) hipLaunchKernelGGL(k_deposit_float, dim3((n + PT - 1) / PT), dim3(PT), 0, st, pos, n, G, periodic, rho);
 }
 
+// ---- gather-form deposit (sparse grids) ------------------------------------------
+// At under a particle per cell the block tile above overflows (a block's
+// particles span more nodes than the tile holds) and every weight becomes a
+// global int64 atomic into a grid larger than the caches. When the particles
+// are sorted by 4^3-cell Morton blocks (contact steps: blockRange holds each
+// block's sorted index range, and the integrate wrote the new positions in
+// that order), the deposit can instead be gathered: a workgroup owns a
+// 32 x 16 x 8 tile of nodes, reads the particles of the blocks around it, adds
+// the weights that fall inside the tile into LDS (exact int64) and stores each
+// node once. A particle whose new CIC base node lies within [-2, 4] cells of
+// its pre-drift block's origin on every axis (any step that moves it less than
+// a cell) touches only nodes of tiles that read its block; the rest ("far
+// movers") are listed by the tile that owns their block and added afterwards
+// with global atomics. Integer sums make the grid independent of the order, so
+// it equals the atomic deposit's bit for bit.
+#ifndef SD_GD_X
+#define SD_GD_X 32
+#endif
+#ifndef SD_GD_Y
+#define SD_GD_Y 16
+#endif
+#ifndef SD_GD_Z
+#define SD_GD_Z 8
+#endif
+#ifndef SD_GD_THREADS
+#define SD_GD_THREADS 512
+#endif
+constexpr int GDX = SD_GD_X, GDY = SD_GD_Y, GDZ = SD_GD_Z;  // tile nodes per axis
+constexpr int GBX = GDX / 4 + 2, GBY = GDY / 4 + 2, GBZ = GDZ / 4 + 2;  // blocks read per axis
+constexpr int GRUNS = GBX * GBY * GBZ;             // 240 for a 32 x 16 x 8 tile
+constexpr int GTHREADS = SD_GD_THREADS;
+constexpr int GLUT = 4096;  // particle visits per tile found through a run table (more: binary search)
+static_assert(GRUNS <= 64 * 4 && GRUNS < 256, "runs per tile: four per lane of one wavefront, byte run table");
+
+// Tiles must divide the mesh; on a periodic mesh the blocks a tile reads must
+// be distinct after wrapping (at least GBX, GBY, GBZ blocks per axis).
+bool gather_deposit_supported(int G, int periodic) {
+    if (G < 32 || G % GDX || G % GDY || G % GDZ) return false;
+    const int nb = G / 4;
+    return !periodic || (nb >= GBX && nb >= GBY && nb >= GBZ);
+}
+
+// Offset of a new base node from its block's origin (4 * bw), folded into
+// [-G/2, G/2) on a periodic mesh.
+__device__ __forceinline__ int block_offset(int base, int bw, int G, int periodic) {
+    int d = base - 4 * bw;
+    if (periodic) {
+        if (d >= G / 2) d -= G;
+        if (d < -G / 2) d += G;
+    }
+    return d;
+}
+
+__global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __restrict__ pos,
+                                                            const int2* __restrict__ blockRange, int G, int periodic,
+                                                            long long* __restrict__ rho, int32_t* __restrict__ farList,
+                                                            int* __restrict__ farCount) {
+    __shared__ unsigned long long s_tile[GDX * GDY * GDZ];
+    __shared__ int s_first[GRUNS], s_pre[GRUNS + 1];
+    __shared__ int s_blk[GRUNS];  // wrapped block coordinates, 10 bits each
+    __shared__ unsigned char s_run[GLUT];  // visit index -> run, when the visits fit
+    const int tid = threadIdx.x;
+    const int X0 = blockIdx.x * GDX, Y0 = blockIdx.y * GDY, Z0 = blockIdx.z * GDZ;
+    const int NB = G / 4;
+    for (int t = tid; t < GDX * GDY * GDZ; t += GTHREADS) s_tile[t] = 0ull;
+    if (tid < 64) {
+        // up to four runs per lane, then a wavefront scan of the lengths
+        constexpr int RPL = (GRUNS + 63) / 64;
+        int len[RPL];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            len[q] = 0;
+            const int r = RPL * tid + q;
+            if (r >= GRUNS) continue;
+            int bx = X0 / 4 - 1 + r % GBX, by = Y0 / 4 - 1 + (r / GBX) % GBY, bz = Z0 / 4 - 1 + r / (GBX * GBY);
+            bool ok = true;
+            if (periodic) {
+                bx = (bx + NB) % NB; by = (by + NB) % NB; bz = (bz + NB) % NB;
+            } else {
+                ok = bx >= 0 && bx < NB && by >= 0 && by < NB && bz >= 0 && bz < NB;
+            }
+            int2 run = make_int2(0, 0);
+            if (ok) run = blockRange[morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz)];
+            s_first[r] = run.x;
+            s_blk[r] = ok ? (bx | (by << 10) | (bz << 20)) : -1;
+            len[q] = run.y > run.x ? run.y - run.x : 0;
+        }
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) sum += len[q];
+        int inc = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (tid >= o) inc += y;
+        }
+        int ex = inc - sum;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            const int r = RPL * tid + q;
+            if (r < GRUNS) s_pre[r] = ex;
+            ex += len[q];
+        }
+        if (tid == 63) s_pre[GRUNS] = inc;
+    }
+    __syncthreads();
+    const int total = s_pre[GRUNS];
+    const bool lut = total <= GLUT;
+    if (lut) {
+        for (int r = tid; r < GRUNS; r += GTHREADS)
+            for (int j = s_pre[r]; j < s_pre[r + 1]; ++j) s_run[j] = (unsigned char)r;
+        __syncthreads();
+    }
+    const float fG = (float)G;
+    for (int j = tid; j < total; j += GTHREADS) {
+        // the run holding flattened index j: the last r with s_pre[r] <= j
+        int r;
+        if (lut) {
+            r = s_run[j];
+        } else {
+            int lo = 0, hi = GRUNS - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+            r = lo;
+        }
+        const int slot = s_first[r] + (j - s_pre[r]);
+        const float4 p = pos[slot];
+        if (p.x < 0.f) continue;
+        const Cic c = cic_setup(p, fG);
+        const int blk = s_blk[r];
+        const int bx = blk & 1023, by = (blk >> 10) & 1023, bz = blk >> 20;
+        const int dx = block_offset(c.ix, bx, G, periodic), dy = block_offset(c.iy, by, G, periodic),
+                  dz = block_offset(c.iz, bz, G, periodic);
+        if (dx < -2 || dx > 4 || dy < -2 || dy > 4 || dz < -2 || dz > 4) {
+            // a far mover: listed once, by the tile that owns its block
+            const int ri = r % GBX, rj = (r / GBX) % GBY, rk = r / (GBX * GBY);
+            if (ri >= 1 && ri < GBX - 1 && rj >= 1 && rj < GBY - 1 && rk >= 1 && rk < GBZ - 1)
+                farList[atomicAdd(farCount, 1)] = slot;
+            continue;
+        }
+        int rx[2], ry[2], rz[2];
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            rx[o] = wrap_or_clamp(c.ix + o, G, periodic) - X0;
+            ry[o] = wrap_or_clamp(c.iy + o, G, periodic) - Y0;
+            rz[o] = wrap_or_clamp(c.iz + o, G, periodic) - Z0;
+        }
+        const bool inx0 = (unsigned)rx[0] < GDX, inx1 = (unsigned)rx[1] < GDX;
+        const bool iny0 = (unsigned)ry[0] < GDY, iny1 = (unsigned)ry[1] < GDY;
+        const bool inz0 = (unsigned)rz[0] < GDZ, inz1 = (unsigned)rz[1] < GDZ;
+        if (!((inx0 || inx1) && (iny0 || iny1) && (inz0 || inz1))) continue;
+        uint32_t w[8];
+        fixed_weights(c, w);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int ox = q & 1, oy = (q >> 1) & 1, oz = (q >> 2) & 1;
+            const bool in = (ox ? inx1 : inx0) && (oy ? iny1 : iny0) && (oz ? inz1 : inz0);
+            if (in && w[q] != 0u)
+                atomicAdd(&s_tile[((oz ? rz[1] : rz[0]) * GDY + (oy ? ry[1] : ry[0])) * GDX + (ox ? rx[1] : rx[0])],
+                          (unsigned long long)w[q]);
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < GDX * GDY * GDZ; t += GTHREADS) {
+        const int x = t % GDX, y = (t / GDX) % GDY, z = t / (GDX * GDY);
+        *elem32(rho, cell_idx(X0 + x, Y0 + y, Z0 + z, G)) = (long long)s_tile[t];
+    }
+}
+
+// The far movers of the gather deposit, added with global atomics after every
+// tile has stored its nodes; one block, which then resets the count.
+__global__ void __launch_bounds__(GTHREADS) k_deposit_far(const float4* __restrict__ pos, int G, int periodic,
+                                                         long long* __restrict__ rho,
+                                                         const int32_t* __restrict__ farList, int* farCount) {
+    const int n = *(volatile int*)farCount;
+    for (int i = threadIdx.x; i < n; i += GTHREADS) {
+        const float4 p = pos[farList[i]];
+        const Cic c = cic_setup(p, (float)G);
+        uint32_t w[8];
+        fixed_weights(c, w);
+        add_global8(c, w, G, periodic, rho);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *farCount = 0;
+}
+
+void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int periodic, long long* rho,
+                           int32_t* farList, StepScalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_deposit_gather, dim3(G / GDX, G / GDY, G / GDZ), dim3(GTHREADS), 0, st, pos, blockRange, G,
+                       periodic, rho, farList, &sc->farCount);
+    hipLaunchKernelGGL(k_deposit_far, dim3(1), dim3(GTHREADS), 0, st, pos, G, periodic, rho, farList,
+                       &sc->farCount);
+}
+
 // sampleAcc, Sim.cu:614-629.
 __device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, const Cic& c, int G, int periodic) {
     const int x0 = wrap_or_clamp(c.ix, G, periodic), x1 = wrap_or_clamp(c.ix + 1, G, periodic);

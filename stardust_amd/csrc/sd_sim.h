@@ -90,6 +90,7 @@ struct sd_sim {
 
     // state
     bool rhoValid = false;   // rhoFix holds the deposit of the current positions
+    bool rhoDirty = false;   // rhoFix not zero although !rhoValid (an x pass that left it for a gather deposit)
     bool meshValid = false;  // pot matches the current positions
     bool accGValid = false;  // accG holds pot's gradient (with meshValid)
     bool primed = false;     // sc->dt / sc->kick are set for the next integrate
