@@ -435,7 +435,9 @@ static bool gather_deposit(const sd_sim* s);
 // The fused step's integrate differentiates the potential in its gather, so
 // it solves without the gradient (grad = false); the acceleration grid is
 // then made only when something reads it (ensure_mesh).
-static void solve_mesh(sd_sim* s, hipStream_t st, bool grad) {
+// keepRho: leave the density in place (a learned correction reads it beside
+// the solve); the next atomic deposit clears it first.
+static void solve_mesh(sd_sim* s, hipStream_t st, bool grad, bool keepRho = false) {
     const long long cells = (long long)s->G * s->G * s->G;
     if (s->modes.depositMode == SD_DEPOSIT_FLOAT) {
         hipLaunchKernelGGL(k_float_to_fix, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, s->rhoF,
@@ -444,7 +446,7 @@ static void solve_mesh(sd_sim* s, hipStream_t st, bool grad) {
     const FinalizeJob* fin = s->finDeferred ? &s->finJob : nullptr;
     // The x pass clears the density it reads for the atomic deposit of the
     // next integrate; a gather deposit writes every node, so it needs no clear.
-    const bool keep = gather_deposit(s);
+    const bool keep = keepRho || gather_deposit(s);
     TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, keep ? 0 : 1, fin, st));
     s->rhoDirty = keep;
     TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, fin, st));
@@ -638,8 +640,15 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
     // nor while the density still has to be deposited from the particles.)
     const bool overlap = SD_OVERLAP_MESH && fused && (sortDue || contactDue) && !s->mlp && !s->borrowed &&
                          s->rhoValid && !s->meshValid && s->modes.depositMode == SD_DEPOSIT_FIXED && s->st2;
+    // A step that recomputes the learned correction runs the mesh beside the
+    // network instead: both only read the deposited density (the x pass leaves
+    // it in place and the integrate's deposit clears it first). The fork is
+    // taken here, before the sort, so the mesh waits for neither.
+    const bool corrDue = s->mlp && (sortDue || contactDue || !s->corrValid || s->corrAge >= s->mlpInterval);
+    const bool mlpOverlap = SD_OVERLAP_MESH && fused && corrDue && !s->borrowed && s->rhoValid && !s->meshValid &&
+                            s->modes.depositMode == SD_DEPOSIT_FIXED && s->st2;
+    if (overlap || mlpOverlap) HK(hipEventRecord(s->evFork, s->st));
     if (overlap) {
-        HK(hipEventRecord(s->evFork, s->st));
         HK(hipStreamWaitEvent(s->st2, s->evFork, 0));
         solve_mesh(s, s->st2, !fused);
         HK(hipEventRecord(s->evJoin, s->st2));
@@ -665,6 +674,13 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
             corr = true;
         }
     }
+    if (mlpOverlap) {
+        // enqueued after the network (which reads rhoValid's density) on the host
+        // side; on the device it waits only for the fork
+        HK(hipStreamWaitEvent(s->st2, s->evFork, 0));
+        solve_mesh(s, s->st2, false, true);
+        HK(hipEventRecord(s->evJoin, s->st2));
+    }
     if (fused) {
         ensure_mesh(s, false);
     } else {
@@ -672,7 +688,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         solve_mesh(s, s->st, true);
     }
     if (contact) contact_forces(s, fuseContact);
-    if (overlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
+    if (overlap || mlpOverlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
     if (timed) HK(hipEventRecord(s->ev1, s->st));
     flush_finalize(s);  // the scalars must be final before this integrate reads them
 
