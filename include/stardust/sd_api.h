@@ -46,10 +46,15 @@ typedef struct sd_sim sd_sim;
 #define SD_PRESET_EMPTY 1
 
 /* Subset of SimConfig (Sim.h:157-796) that the PM step reads, with the same
- * names and defaults (sd_config_default). The off-path physics switches are
- * kept so a caller's configuration maps field for field; any of them set
- * makes sd_init / sd_reconfigure fail with SD_ERR_ARG (this library runs the
- * plain-PM path only). */
+ * names and defaults (sd_config_default), except that pressure and cooling
+ * default to off here (the plain-PM path). Pressure and cooling (the gas
+ * velocity-dispersion grids, Sim.cu:1212-1415, 3605-3633, 3732-3830) run in the
+ * reference kernel order only: they need SD_PIPELINE_REFERENCE and
+ * SD_INTEGRATOR_KD (else SD_ERR_ARG), and treat particles with pos.w <= 0 as
+ * gas, pos.w > 0 as stars (no cooling, no pressure) and vel.w < -50 as dark
+ * matter (neither), as the reference does. The other physics switches are
+ * kept so a configuration maps field for field; any of them set makes
+ * sd_init / sd_reconfigure fail with SD_ERR_ARG. */
 typedef struct sd_config {
     int particleCount;       /* 2000000 */
     int gridSize;            /* 128 */
@@ -66,9 +71,10 @@ typedef struct sd_config {
     int contactEnabled;      /* 0 */
     float contactStiffness;  /* 1e6 */
     float contactDamping;    /* 0.35 */
-    /* off-path physics (must be 0 here) */
-    int pressureEnabled;
-    int coolingEnabled;
+    /* gas physics (reference kernel order only, see above) */
+    int pressureEnabled;     /* 0 here (the reference's default is 1) */
+    int coolingEnabled;      /* 0 here (the reference's default is 1) */
+    /* other physics (must be 0 here) */
     int starFormationEnabled;
     int blackHoleEnabled;
     int strongForceEnabled;
@@ -78,6 +84,8 @@ typedef struct sd_config {
     /* extension: contact radius in cells (the reference hard-codes 0.5,
      * Sim.cu:3694) */
     float contactRadiusCells;
+    float pressureK;         /* 0.45 (Sim.h:320) */
+    float coolingRate;       /* 0.08 (Sim.h:345) */
 } sd_config;
 
 /* Per-step timings, SimTimings (Sim.h:844-852). scatterMs = gravity phase

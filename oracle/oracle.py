@@ -71,6 +71,10 @@ class Params(C.Structure):
         ("darkEnergy", C.c_float),
         ("allocN", C.c_int),
         ("fp32Mesh", C.c_int),
+        ("pressureEnabled", C.c_int),
+        ("coolingEnabled", C.c_int),
+        ("pressureK", C.c_float),
+        ("coolingRate", C.c_float),
     ]
 
 
@@ -105,7 +109,9 @@ def _declare(L):
     L.sdo_max_speed.restype = C.c_float
     L.sdo_max_speed.argtypes = [_fp, _fp, C.c_int]
     L.sdo_integrate.argtypes = [_fp, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
-                                C.c_float, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+                                C.c_float, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+    L.sdo_cool_cells.argtypes = [_fp, _fp, _u32p, C.c_int, C.c_int, C.c_float, C.c_float, _fp]
+    L.sdo_pressure.argtypes = [_fp, C.c_int, C.c_int, C.c_float, C.c_float, _fp]
     L.sdo_momentum.argtypes = [_fp, _fp, C.c_int, _dp]
     L.sdo_subtract_mean.argtypes = [_fp, _fp, C.c_int, _dp]
     L.sdo_cell_range.argtypes = [_u32p, C.c_int, C.c_int, _i32p, _i32p]
@@ -115,7 +121,7 @@ def _declare(L):
     L.sdo_dt_policy.argtypes = [C.POINTER(Params), C.c_float]
     L.sdo_compute_accel.argtypes = [C.POINTER(Params), _fp, C.c_int, _fp, C.c_void_p, C.c_void_p]
     L.sdo_step.restype = C.c_int
-    L.sdo_step.argtypes = [C.POINTER(Params), C.POINTER(State), _fp, _fp, C.c_int, _fp, _fp]
+    L.sdo_step.argtypes = [C.POINTER(Params), C.POINTER(State), _fp, _fp, C.c_int, _fp, _fp, C.c_void_p]
     L.sdo_direct_sum.argtypes = [_fp, C.c_int, _i32p, C.c_int, C.c_double, C.c_double, _dp]
     L.sdo_rel_rms.restype = C.c_double
     L.sdo_rel_rms.argtypes = [_dp, _dp, C.c_int, C.POINTER(C.c_double)]
@@ -274,7 +280,7 @@ def max_speed(pos, vel):
 
 
 def integrate(acc4, pos, vel, G, kick, drift, periodic=False, c2=1e4, acc_contact=None,
-              softBoundR=0.0, darkEnergy=0.0, mean_sub=None):
+              softBoundR=0.0, darkEnergy=0.0, mean_sub=None, acc_press=None):
     """Returns updated (pos, vel) copies."""
     acc4 = np.ascontiguousarray(acc4, dtype=np.float32).reshape(-1)
     pos, vel = _p4(pos).copy(), _p4(vel).copy()
@@ -284,10 +290,31 @@ def integrate(acc4, pos, vel, G, kick, drift, periodic=False, c2=1e4, acc_contac
     ms = None
     if mean_sub is not None:
         ms = np.ascontiguousarray(mean_sub, dtype=np.float32)
+    ap = None
+    if acc_press is not None:
+        ap = np.ascontiguousarray(acc_press, dtype=np.float32).reshape(-1)
     lib().sdo_integrate(acc4, pos, vel, pos.shape[0], G, kick, drift, int(periodic), c2,
                         ac.ctypes.data if ac is not None else None, softBoundR, darkEnergy,
-                        ms.ctypes.data if ms is not None else None)
+                        ms.ctypes.data if ms is not None else None,
+                        ap.ctypes.data if ap is not None else None)
     return pos, vel
+
+
+def cool_cells(pos, vel, keys, G, rate, dt):
+    """Cooling pass over particles sorted by cell; returns (vel, disp[4, G, G, G])."""
+    pos, vel = _p4(pos), _p4(vel).copy()
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    disp = np.zeros(4 * G ** 3, np.float32)
+    lib().sdo_cool_cells(pos, vel, k, pos.shape[0], G, rate, dt, disp)
+    return vel, disp.reshape(4, G, G, G)
+
+
+def pressure(disp, G, periodic, k, max_acc=5.0):
+    """kPressure of the dispersion grids disp[4, G, G, G] -> float32 [G, G, G, 4]."""
+    d = np.ascontiguousarray(disp, dtype=np.float32).reshape(-1)
+    out = np.empty(4 * G ** 3, np.float32)
+    lib().sdo_pressure(d, G, int(periodic), k, max_acc, out)
+    return out.reshape(G, G, G, 4)
 
 
 def momentum(pos, vel):
@@ -324,10 +351,12 @@ def contact(pos, vel, G, cell_start, cell_end, radius, stiffness, damping, perio
 def make_params(G, periodic=False, gravity=1.0, softeningCells=3.0, timeScale=1.0,
                 sortInterval=40, keyMode=KEY_ROWMAJOR, fixedDeposit=1, fixedDt=0.0,
                 momentumZero=1, contact=0, contactRadiusCells=0.5, contactStiffness=1e6,
-                contactDamping=0.35, c2=1e4, softBoundR=0.0, darkEnergy=0.0, allocN=1, fp32Mesh=0):
+                contactDamping=0.35, c2=1e4, softBoundR=0.0, darkEnergy=0.0, allocN=1, fp32Mesh=0,
+                pressure=0, cooling=0, pressureK=0.45, coolingRate=0.08):
     return Params(G, int(periodic), gravity, softeningCells, timeScale, sortInterval, keyMode,
                   fixedDeposit, fixedDt, momentumZero, contact, contactRadiusCells,
-                  contactStiffness, contactDamping, c2, softBoundR, darkEnergy, allocN, int(fp32Mesh))
+                  contactStiffness, contactDamping, c2, softBoundR, darkEnergy, allocN, int(fp32Mesh),
+                  int(pressure), int(cooling), pressureK, coolingRate)
 
 
 def dt_policy(params, vmax):
@@ -361,11 +390,16 @@ class Stepper:
         self.vel = _p4(vel).copy()
         self._tp = np.empty_like(self.pos)
         self._tv = np.empty_like(self.vel)
+        # gas dispersion grids carried from one cooling pass to the next steps
+        self.gas = None
+        if params.pressureEnabled or params.coolingEnabled:
+            self.gas = np.zeros(4 * params.G ** 3, np.float32)
 
     def step(self, k=1):
         for _ in range(k):
             lib().sdo_step(C.byref(self.params), C.byref(self.state), self.pos, self.vel,
-                           self.pos.shape[0], self._tp, self._tv)
+                           self.pos.shape[0], self._tp, self._tv,
+                           self.gas.ctypes.data if self.gas is not None else None)
 
 
 def direct_sum(pos, targets, gm, eps):

@@ -523,7 +523,7 @@ float sdo_max_speed(const float* pos4, const float* vel4, int n) {
  * is a pending momentum correction subtracted from v on load. */
 void sdo_integrate(const float* acc4, float* pos4, float* vel4, int n, int G, float kick,
                    float drift, int periodic, float c2, const float* accContact4,
-                   float softBoundR, float darkEnergy, const float* meanSub3) {
+                   float softBoundR, float darkEnergy, const float* meanSub3, const float* accPress4) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         float* P = pos4 + 4 * (size_t)i;
@@ -561,6 +561,14 @@ void sdo_integrate(const float* acc4, float* pos4, float* vel4, int n, int G, fl
                 a[1] = fmaf(darkEnergy * k, p[1] - 0.5f, a[1]);
                 a[2] = fmaf(darkEnergy * k, p[2] - 0.5f, a[2]);
             }
+        }
+        /* gas pressure, Sim.cu:740-743: gas (w <= 0), not dark matter */
+        if (accPress4 && p[3] <= 0.f && v[3] >= -50.f) {
+            float pr[3];
+            sdo_sample_acc(accPress4, p[0], p[1], p[2], G, periodic, pr);
+            a[0] += pr[0];
+            a[1] += pr[1];
+            a[2] += pr[2];
         }
         v[0] = fmaf(a[0], kick, v[0]);
         v[1] = fmaf(a[1], kick, v[1]);
@@ -756,6 +764,110 @@ void sdo_contact(const float* pos4, const float* vel4, int n, int G, int periodi
 /* Whole step (reference order, Sim.cu:4240-4525)                            */
 /* ------------------------------------------------------------------------ */
 
+/* Gas cooling, kAccumCellVel + kCoolCell (Sim.cu:1212-1366), over particles
+ * sorted by cell (keys = cell key + 1, 0 inactive), each cell's run summed in
+ * sorted order (the reference's float atomics take some order; this is one).
+ * Mean velocity over the cell's particles that are not dark matter (vel.w >=
+ * -50, finite velocities); cells under two particles are skipped. Every such
+ * particle adds (mean - v)^2 to the cell's dispersion sums (disp4: x, y, z,
+ * count grids of G^3 each, zeroed by the caller); gas (pos.w <= 0) moves
+ * toward the mean by k = min(rate dt 60, 0.5), "v + k dv" fused as nvcc does. */
+void sdo_cool_cells(const float* pos4, float* vel4, const uint32_t* keys, int n, int G, float rate, float dt,
+                    float* disp4) {
+    const size_t cells = (size_t)G * G * G;
+    int i = 0;
+    while (i < n) {
+        const uint32_t key = keys[i];
+        int end = i + 1;
+        while (end < n && keys[end] == key) ++end;
+        if (key != 0u) {
+            float sx = 0.f, sy = 0.f, sz = 0.f, nc = 0.f;
+            for (int j = i; j < end; ++j) {
+                const float* p = pos4 + 4 * (size_t)j;
+                const float* v = vel4 + 4 * (size_t)j;
+                if (p[0] < 0.f) continue;
+                if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2])) continue;
+                if (v[3] < -50.f) continue;
+                sx += v[0];
+                sy += v[1];
+                sz += v[2];
+                nc += 1.0f;
+            }
+            if (nc >= 2.f) {
+                const float* p0 = pos4 + 4 * (size_t)i;
+                const float fG = (float)G;
+                const int cx = clampi((int)(p0[0] * fG), 0, G - 1);
+                const int cy = clampi((int)(p0[1] * fG), 0, G - 1);
+                const int cz = clampi((int)(p0[2] * fG), 0, G - 1);
+                const size_t c = ((size_t)cz * G + cy) * G + cx;
+                const float inv = 1.f / nc;
+                const float mvx = sx * inv, mvy = sy * inv, mvz = sz * inv;
+                float k = rate * dt * 60.0f;
+                if (k > 0.5f) k = 0.5f;
+                float qx = 0.f, qy = 0.f, qz = 0.f, qc = 0.f;
+                for (int j = i; j < end; ++j) {
+                    const float* p = pos4 + 4 * (size_t)j;
+                    float* v = vel4 + 4 * (size_t)j;
+                    if (p[0] < 0.f) continue;
+                    if (v[3] < -50.f) continue;
+                    const float dvx = mvx - v[0], dvy = mvy - v[1], dvz = mvz - v[2];
+                    qx += dvx * dvx;
+                    qy += dvy * dvy;
+                    qz += dvz * dvz;
+                    qc += 1.0f;
+                    if (rate > 0.f && p[3] <= 0.f) {
+                        v[0] = fmaf(k, dvx, v[0]);
+                        v[1] = fmaf(k, dvy, v[1]);
+                        v[2] = fmaf(k, dvz, v[2]);
+                    }
+                }
+                disp4[c] = qx;
+                disp4[cells + c] = qy;
+                disp4[2 * cells + c] = qz;
+                disp4[3 * cells + c] = qc;
+            }
+        }
+        i = end;
+    }
+}
+
+/* kPressure, Sim.cu:1368-1399: per cell with dispersion, a = -(D[+1] - D[-1]) *
+ * (0.5 G k / n) on each axis's own dispersion grid, clamped to +-maxAcc; zero
+ * elsewhere. Isolated edges clamp the stencil, periodic wraps. */
+void sdo_pressure(const float* disp4, int G, int periodic, float k, float maxAcc, float* accP4) {
+    const size_t cells = (size_t)G * G * G;
+    const float* dX = disp4;
+    const float* dY = disp4 + cells;
+    const float* dZ = disp4 + 2 * cells;
+    const float* dC = disp4 + 3 * cells;
+    for (int z = 0; z < G; ++z)
+        for (int y = 0; y < G; ++y)
+            for (int x = 0; x < G; ++x) {
+                const size_t c = ((size_t)z * G + y) * G + x;
+                float* a = accP4 + 4 * c;
+                a[0] = a[1] = a[2] = a[3] = 0.f;
+                const float rho = dC[c];
+                if (rho < 1e-6f) continue;
+                int xm, xp, ym, yp, zm, zp;
+                if (periodic) {
+                    xm = (x - 1 + G) % G; xp = (x + 1) % G;
+                    ym = (y - 1 + G) % G; yp = (y + 1) % G;
+                    zm = (z - 1 + G) % G; zp = (z + 1) % G;
+                } else {
+                    xm = x > 0 ? x - 1 : 0; xp = x < G - 1 ? x + 1 : G - 1;
+                    ym = y > 0 ? y - 1 : 0; yp = y < G - 1 ? y + 1 : G - 1;
+                    zm = z > 0 ? z - 1 : 0; zp = z < G - 1 ? z + 1 : G - 1;
+                }
+                const float h = 0.5f * (float)G * k / rho;
+                float ax = -(dX[((size_t)z * G + y) * G + xp] - dX[((size_t)z * G + y) * G + xm]) * h;
+                float ay = -(dY[((size_t)z * G + yp) * G + x] - dY[((size_t)z * G + ym) * G + x]) * h;
+                float az = -(dZ[((size_t)zp * G + y) * G + x] - dZ[((size_t)zm * G + y) * G + x]) * h;
+                a[0] = fminf(fmaxf(ax, -maxAcc), maxAcc);
+                a[1] = fminf(fmaxf(ay, -maxAcc), maxAcc);
+                a[2] = fminf(fmaxf(az, -maxAcc), maxAcc);
+            }
+}
+
 /* dt policy, Sim.cu:4259-4283 with dtScaleFor (Sim.h:89-93):
  * dt = min(0.0016 * dtScaleFor(timeScale), 0.8 * (1/G) / vmax), contact caps
  * dt at 0.5/sqrt(stiffness). fixedDt > 0 replaces the first term and the CFL
@@ -842,7 +954,7 @@ static void sort_particles(const sdo_params* prm, float* pos4, float* vel4, int 
  * (skipped when contact sorts itself), computeAccel, contact (sort + cell
  * ranges + kContact), CFL dt, integrate, momentum removal, time advance. */
 int sdo_step(const sdo_params* prm, sdo_state* st, float* pos4, float* vel4, int n,
-             float* scratchPos4, float* scratchVel4) {
+             float* scratchPos4, float* scratchVel4, float* gas4) {
     const int G = prm->G;
     const size_t cells = (size_t)G * G * G;
     uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
@@ -851,6 +963,13 @@ int sdo_step(const sdo_params* prm, sdo_state* st, float* pos4, float* vel4, int
         sort_particles(prm, pos4, vel4, n, scratchPos4, scratchVel4, keys, idx, 0);
     float* accG = (float*)malloc(sizeof(float) * 4 * cells);
     sdo_compute_accel(prm, pos4, n, accG, NULL, NULL);
+    /* the pressure grid from the last cooling pass's dispersion (Sim.cu:3617-3631) */
+    const int gas = (prm->pressureEnabled || prm->coolingEnabled) && gas4;
+    float* accP = NULL;
+    if (gas && prm->pressureEnabled) {
+        accP = (float*)malloc(sizeof(float) * 4 * cells);
+        sdo_pressure(gas4, G, prm->periodic, prm->pressureK, 5.0f, accP);
+    }
     float* accC = NULL;
     if (prm->contact && !((double)prm->allocN > 0.764 * (double)G * G * G)) {
         sort_particles(prm, pos4, vel4, n, scratchPos4, scratchVel4, keys, idx, 1);
@@ -874,9 +993,20 @@ int sdo_step(const sdo_params* prm, sdo_state* st, float* pos4, float* vel4, int
     }
     const float vmax = sdo_max_speed(pos4, vel4, n);
     const float dt = sdo_dt_policy(prm, vmax);
+    if (gas) {
+        /* doCooling (Sim.cu:3732-3830): every 1 / 2 / 8 steps, dt times that */
+        const int every = prm->contact ? 1 : (prm->pressureEnabled ? 2 : 8);
+        if ((st->stepCount % every) == 0) {
+            if (!accC) sort_particles(prm, pos4, vel4, n, scratchPos4, scratchVel4, keys, idx, 1);
+            memset(gas4, 0, sizeof(float) * 4 * cells);
+            const float rate = prm->coolingEnabled && prm->coolingRate > 0.f ? prm->coolingRate : 0.f;
+            sdo_cool_cells(pos4, vel4, keys, n, G, rate, dt * (float)every, gas4);
+        }
+    }
     sdo_integrate(accG, pos4, vel4, n, G, dt, dt, prm->periodic, prm->c2, accC,
                   prm->periodic ? 0.f : (prm->softBoundR > 0.f ? prm->softBoundR : 0.f),
-                  prm->darkEnergy, NULL);
+                  prm->darkEnergy, NULL, accP);
+    free(accP);
     if (prm->momentumZero) {
         double m[4];
         sdo_momentum(pos4, vel4, n, m);

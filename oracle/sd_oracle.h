@@ -64,7 +64,7 @@ void sdo_sample_acc_all(const float* acc4, const float* pos4, int n, int G, int 
 float sdo_max_speed(const float* pos4, const float* vel4, int n);
 void sdo_integrate(const float* acc4, float* pos4, float* vel4, int n, int G, float kick,
                    float drift, int periodic, float c2, const float* accContact4,
-                   float softBoundR, float darkEnergy, const float* meanSub3);
+                   float softBoundR, float darkEnergy, const float* meanSub3, const float* accPress4);
 void sdo_momentum(const float* pos4, const float* vel4, int n, double* out4);
 void sdo_subtract_mean(const float* pos4, float* vel4, int n, const double* acc4);
 
@@ -95,6 +95,10 @@ typedef struct sdo_params {
     float darkEnergy;
     int allocN;         /* mass per particle = gravity / allocN */
     int fp32Mesh;       /* 1: single-precision mesh (the CPU baseline's timing path) */
+    int pressureEnabled; /* gas pressure (Sim.cu:1368-1399, 3617-3631) */
+    int coolingEnabled;  /* gas cooling (Sim.cu:1212-1366, 3732-3830) */
+    float pressureK;
+    float coolingRate;
 } sdo_params;
 
 typedef struct sdo_state {
@@ -108,7 +112,10 @@ float sdo_dt_policy(const sdo_params* prm, float vmax);
 void sdo_compute_accel(const sdo_params* prm, const float* pos4, int n, float* accG4,
                        int64_t* rhoFixOut, double* potOut);
 int sdo_step(const sdo_params* prm, sdo_state* st, float* pos4, float* vel4, int n,
-             float* scratchPos4, float* scratchVel4);
+             float* scratchPos4, float* scratchVel4, float* gas4);
+void sdo_cool_cells(const float* pos4, float* vel4, const uint32_t* keys, int n, int G, float rate, float dt,
+                    float* disp4);
+void sdo_pressure(const float* disp4, int G, int periodic, float k, float maxAcc, float* accP4);
 
 /* --- direct sum and error metric ---------------------------------------- */
 void sdo_direct_sum(const float* pos4, int n, const int32_t* targets, int nt, double gm,

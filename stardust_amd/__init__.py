@@ -76,6 +76,8 @@ class Config(C.Structure):
         ("weakForceEnabled", C.c_int),
         ("collapseEnabled", C.c_int),
         ("contactRadiusCells", C.c_float),
+        ("pressureK", C.c_float),
+        ("coolingRate", C.c_float),
     ]
 
 

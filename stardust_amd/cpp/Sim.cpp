@@ -46,7 +46,22 @@ sd_config to_c(const SimConfig& c) {
     o.weakForceEnabled = c.weakForceEnabled ? 1 : 0;
     o.collapseEnabled = c.collapseEnabled ? 1 : 0;
     o.contactRadiusCells = c.contactRadiusCells;
+    o.pressureK = c.pressureK;
+    o.coolingRate = c.coolingRate;
     return o;
+}
+
+// Pressure and cooling run in the reference kernel order with kick-drift: a
+// configuration that enables them selects those modes first (the reference's
+// own SimConfig defaults enable both).
+void gas_modes(sd_sim* h, const SimConfig& c) {
+    if (!c.pressureEnabled && !c.coolingEnabled) return;
+    sd_modes m;
+    sd_get_modes(h, &m);
+    if (m.pipeline == SD_PIPELINE_REFERENCE && m.integrator == SD_INTEGRATOR_KD) return;
+    m.pipeline = SD_PIPELINE_REFERENCE;
+    m.integrator = SD_INTEGRATOR_KD;
+    sd_set_modes(h, &m);
 }
 }  // namespace
 
@@ -82,6 +97,7 @@ void Sim::init(const SimConfig& cfg) {
     if (any_failure()) return;
     cfg_ = cfg;
     const sd_config c = to_c(cfg);
+    gas_modes(h_, cfg);
     const int rc = sd_init(h_, &c);
     if (rc != SD_OK) shim_fail(std::string("init refused: ") + sd_fail_message());
     else cfg_.gridSize = sd_grid_size(h_);
@@ -90,6 +106,7 @@ void Sim::init(const SimConfig& cfg) {
 void Sim::reconfigure(const SimConfig& cfg) {
     if (any_failure()) return;
     const sd_config c = to_c(cfg);
+    gas_modes(h_, cfg);
     const int rc = sd_reconfigure(h_, &c);
     if (rc != SD_OK) {
         shim_fail(std::string("reconfigure refused: ") + sd_fail_message());

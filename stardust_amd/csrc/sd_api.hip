@@ -152,7 +152,7 @@ static void free_buffers(sd_sim* s) {
     dfree(s->keysA); dfree(s->keysB); dfree(s->idxA); dfree(s->idxB);
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
-    dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact);
+    dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact); dfree(s->disp); dfree(s->accPress);
     dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
     s->cellTableCells = 0;
     s->blockCells = 0;
@@ -181,14 +181,24 @@ static int choose_grid(const sd_config* c, const sd_modes* m) {
     return p;
 }
 
+// Pressure and cooling (the gas dispersion grids) run in the reference kernel
+// order with kick-drift, where the stored velocities are the physical ones.
+static bool gas_physics(const sd_config* c) { return c->pressureEnabled || c->coolingEnabled; }
+static int check_gas_modes(const sd_config* c, const sd_modes* m) {
+    if (gas_physics(c) && (m->pipeline != SD_PIPELINE_REFERENCE || m->integrator != SD_INTEGRATOR_KD))
+        return arg_error("pressure and cooling run in the reference kernel order: they need "
+                         "SD_PIPELINE_REFERENCE and SD_INTEGRATOR_KD");
+    return SD_OK;
+}
+
 static int check_config(const sd_config* c) {
     if (!c) return arg_error("null config");
     if (c->boundary != SD_BOUNDARY_ISOLATED && c->boundary != SD_BOUNDARY_PERIODIC)
         return arg_error("boundary must be isolated (0) or periodic (1)");
-    if (c->pressureEnabled || c->coolingEnabled || c->starFormationEnabled || c->blackHoleEnabled ||
-        c->strongForceEnabled || c->emForceEnabled || c->weakForceEnabled || c->collapseEnabled)
-        return arg_error("only the plain-PM path is provided: pressure, cooling, star formation, "
-                         "black holes, strong/EM/weak forces and collapse must be off");
+    if (c->starFormationEnabled || c->blackHoleEnabled || c->strongForceEnabled || c->emForceEnabled ||
+        c->weakForceEnabled || c->collapseEnabled)
+        return arg_error("star formation, black holes, strong/EM/weak forces and collapse are not provided "
+                         "(the PM path with contact, pressure and cooling is)");
     if (c->preset != SD_PRESET_EMPTY)
         return arg_error("preset Filament is not provided; use SD_PRESET_EMPTY and sd_set_particles");
     if (c->particleCount < 0) return arg_error("particleCount < 0");
@@ -324,6 +334,15 @@ static bool ensure_contact_buffers(sd_sim* s) {
     return true;
 }
 
+// Gas dispersion grids (zero until the first cooling pass) and the pressure grid.
+static bool ensure_gas_buffers(sd_sim* s) {
+    if (s->disp && s->accPress) return true;
+    const long long cells = (long long)s->G * s->G * s->G;
+    if (!dalloc(s->disp, (size_t)cells * 4) || !dalloc(s->accPress, (size_t)cells)) return false;
+    launch_fill_f32(s->disp, cells * 4, 0.f, s->st);
+    return true;
+}
+
 static bool ensure_float_rho(sd_sim* s) {
     if (s->rhoF) return true;
     return dalloc(s->rhoF, (size_t)s->G * s->G * s->G);
@@ -381,6 +400,22 @@ static void sort_particles(sd_sim* s, bool forContact = false) {
     }
     s->keysSorted = true;
     s->corrValid = false;  // per-particle correction no longer in particle order
+}
+
+// The reference's doCooling (Sim.cu:3732-3830), deterministic: every 1 (contact
+// on), 2 (pressure on) or 8 steps, with dt times that interval, particles
+// sorted by cell (the contact pass already did; else a cell-key sort here),
+// dispersion sums cleared, then one pass per cell.
+static void cooling_pass(sd_sim* s, bool contactSorted) {
+    const int every = s->cfg.contactEnabled ? 1 : (s->cfg.pressureEnabled ? 2 : 8);
+    if ((s->stepCount % every) != 0) return;
+    if (!ensure_gas_buffers(s)) return;
+    if (!contactSorted) sort_particles(s, true);
+    const long long cells = (long long)s->G * s->G * s->G;
+    launch_fill_f32(s->disp, cells * 4, 0.f, s->st);
+    const float rate = s->cfg.coolingEnabled && s->cfg.coolingRate > 0.f ? s->cfg.coolingRate : 0.f;
+    TIMED(s, "cooling", launch_cool_cells(s->pos, s->vel, s->sb.keys, s->allocN, s->G, rate, (float)every, s->sc,
+                                          s->disp, s->st));
 }
 
 static DtPolicy dt_policy(const sd_sim* s) {
@@ -687,6 +722,11 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         ensure_rho(s);
         solve_mesh(s, s->st, true);
     }
+    // the pressure grid from the last cooling pass's dispersion (computeAccel,
+    // Sim.cu:3617-3631); gas physics runs in the reference order only
+    const bool gasPressure = !fused && s->cfg.pressureEnabled && ensure_gas_buffers(s);
+    if (gasPressure)
+        TIMED(s, "pressure", launch_pressure(s->disp, s->accPress, s->G, s->periodic, s->cfg.pressureK, 5.0f, s->st));
     if (contact) contact_forces(s, fuseContact);
     if (overlap || mlpOverlap) HK(hipStreamWaitEvent(s->st, s->evJoin, 0));
     if (timed) HK(hipEventRecord(s->ev1, s->st));
@@ -753,6 +793,14 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         // kMaxSpeed + dt policy, then kIntegrate, then momentum removal
         launch_vmax(s->pos, s->vel, s->allocN, s->partials, s->st);
         launch_finalize(s->partials, s->nparts, 2 | 4, pol, s->sc, s->st);
+        if (gas_physics(&s->cfg)) {
+            // cooling after the dt policy and before the kick (Sim.cu:4290); its
+            // cell sort may move the particles
+            cooling_pass(s, contact);
+            a.pos = s->pos;
+            a.vel = s->vel;
+        }
+        a.accPress = gasPressure ? s->accPress : nullptr;
         a.applyMeanSub = 0;
         a.depositTo = nullptr;
         a.partials = nullptr;
@@ -882,6 +930,8 @@ void sd_config_default(sd_config* c) {
     c->pressureEnabled = 1;  // SimConfig defaults (Sim.h); plain PM turns them off
     c->coolingEnabled = 1;
     c->contactRadiusCells = 0.5f;
+    c->pressureK = 0.45f;
+    c->coolingRate = 0.08f;
 }
 
 void sd_modes_default(sd_modes* m) {
@@ -1021,6 +1071,8 @@ int sd_set_modes(sd_sim* s, const sd_modes* m) {
     if (s->ready) {
         if (g_failed) return SD_ERR_FAILED;
         if (m->exactGrid != s->modes.exactGrid) return arg_error("exactGrid can change only before sd_init");
+        const int rc = check_gas_modes(&s->cfg, m);
+        if (rc) return rc;
         // hand the state over synchronised: pending momentum correction and the
         // leapfrog half kick applied, integrator restarted
         synced_velocity(s);
@@ -1049,6 +1101,8 @@ int sd_reconfigure(sd_sim* s, const sd_config* c) {
     if (!s) return SD_ERR_ARG;
     if (g_failed) return SD_ERR_FAILED;
     int rc = check_config(c);
+    if (rc) return rc;
+    rc = check_gas_modes(c, &s->modes);
     if (rc) return rc;
     const int G = choose_grid(c, &s->modes);
     if (!valid_grid(G, c->boundary, s->modes.exactGrid))

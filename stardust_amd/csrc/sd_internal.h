@@ -110,6 +110,12 @@ void launch_deposit_float(const float4* pos, int n, int G, int periodic, float* 
 // the surrounding blocks and writes every node once (no clearing, no global
 // atomics). Particles more than the halo allows from their block are listed
 // and added by a one-block atomic pass, which also resets sc->farCount.
+// Gas cooling and pressure (Sim.cu:1212-1415). disp holds four G^3 float grids:
+// the x, y, z dispersion sums and the particle count of each cell. Particles
+// must be sorted by cell (keys: a cell key + 1, 0 inactive).
+void launch_cool_cells(const float4* pos, float4* vel, const uint32_t* keys, int n, int G, float rate, float dtMul,
+                       const StepScalars* sc, float* disp, hipStream_t st);
+void launch_pressure(const float* disp, float4* accP, int G, int periodic, float k, float maxAcc, hipStream_t st);
 bool gather_deposit_supported(int G, int periodic);
 void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int periodic, long long* rho,
                            int32_t* farList, StepScalars* sc, hipStream_t st);
@@ -144,6 +150,8 @@ struct IntegrateArgs {
     // gradH = 0.5 * G * potScale, bit-identical to gathering the gradient grid.
     const float* pot;
     float gradH;
+    // gas pressure acceleration grid (float4[G^3]); nullptr = off
+    const float4* accPress;
 };
 int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
 int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)

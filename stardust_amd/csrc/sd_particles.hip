@@ -594,6 +594,120 @@ void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int
                        &sc->farCount);
 }
 
+// ---- gas physics: cooling and pressure (Sim.cu:1212-1415) ---------------------
+// The reference sums each cell's velocities with float atomics (an order that
+// changes run to run). Here particles are sorted by cell and the first particle
+// of each cell's run sums the run serially in sorted order, so the sums are
+// deterministic and equal a sequential CPU pass over the same order.
+//
+// k_cool_cells: kAccumCellVel + kCoolCell for one cell per run head. Mean
+// velocity over the cell's particles that are not dark matter (vel.w >= -50,
+// finite velocities); cells with fewer than two hold no dispersion and cool
+// nothing. Each such particle adds (mean - v)^2 per axis to the cell's
+// dispersion sums; gas particles (pos.w <= 0) are pulled toward the mean by
+// k = min(rate * dt * 60, 0.5), dt the step's dt times the pass interval.
+__global__ void __launch_bounds__(256) k_cool_cells(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                   const uint32_t* __restrict__ keys, int n, int G,
+                                                   float rate, float dtMul, const StepScalars* __restrict__ sc,
+                                                   float* __restrict__ dispX, float* __restrict__ dispY,
+                                                   float* __restrict__ dispZ, float* __restrict__ dispCnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = keys[i];
+    if (key == 0u || (i > 0 && keys[i - 1] == key)) return;  // inactive, or not the run's first
+    int end = i + 1;
+    while (end < n && keys[end] == key) ++end;
+    float sx = 0.f, sy = 0.f, sz = 0.f, nc = 0.f;
+    for (int j = i; j < end; ++j) {
+        const float4 p = pos[j];
+        const float4 v = vel[j];
+        if (p.x < 0.f) continue;
+        if (!isfinite(v.x) || !isfinite(v.y) || !isfinite(v.z)) continue;
+        if (v.w < -50.f) continue;
+        sx += v.x;
+        sy += v.y;
+        sz += v.z;
+        nc += 1.0f;
+    }
+    if (nc < 2.f) return;
+    const float4 p0 = pos[i];
+    const float fG = (float)G;
+    const int cx = min(max((int)(p0.x * fG), 0), G - 1);
+    const int cy = min(max((int)(p0.y * fG), 0), G - 1);
+    const int cz = min(max((int)(p0.z * fG), 0), G - 1);
+    const int c = (cz * G + cy) * G + cx;
+    const float inv = 1.f / nc;
+    const float mvx = sx * inv, mvy = sy * inv, mvz = sz * inv;
+    float k = rate * (sc->dt * dtMul) * 60.0f;
+    if (k > 0.5f) k = 0.5f;
+    float qx = 0.f, qy = 0.f, qz = 0.f, qc = 0.f;
+    for (int j = i; j < end; ++j) {
+        const float4 p = pos[j];
+        const float4 v = vel[j];
+        if (p.x < 0.f) continue;
+        if (v.w < -50.f) continue;
+        const float dvx = mvx - v.x, dvy = mvy - v.y, dvz = mvz - v.z;
+        qx += dvx * dvx;
+        qy += dvy * dvy;
+        qz += dvz * dvz;
+        qc += 1.0f;
+        if (rate > 0.f && p.w <= 0.f) vel[j] = make_float4(fmaf(k, dvx, v.x), fmaf(k, dvy, v.y), fmaf(k, dvz, v.z), v.w);
+    }
+    dispX[c] = qx;
+    dispY[c] = qy;
+    dispZ[c] = qz;
+    dispCnt[c] = qc;
+}
+
+void launch_cool_cells(const float4* pos, float4* vel, const uint32_t* keys, int n, int G, float rate, float dtMul,
+                       const StepScalars* sc, float* disp, hipStream_t st) {
+    if (n <= 0) return;
+    const size_t cells = (size_t)G * G * G;
+    hipLaunchKernelGGL(k_cool_cells, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, keys, n, G, rate, dtMul, sc,
+                       disp, disp + cells, disp + 2 * cells, disp + 3 * cells);
+}
+
+// kPressure (Sim.cu:1368-1399): a = -(disp[+1] - disp[-1]) * (0.5 G k / n) per
+// axis, the dispersion sum of that axis differenced along it, n the cell's
+// count; clamped to +-maxAcc; zero where the cell holds no dispersion. Edges as
+// kGridAccel (clamped stencil, or wrapped on a periodic mesh).
+__global__ void k_pressure(const float* __restrict__ dispX, const float* __restrict__ dispY,
+                           const float* __restrict__ dispZ, const float* __restrict__ dispCnt,
+                           float4* __restrict__ accP, int G, int periodic, float k, float maxAcc) {
+    const long long ci = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= (long long)G * G * G) return;
+    const int x = (int)(ci % G), y = (int)((ci / G) % G), z = (int)(ci / ((long long)G * G));
+    const float rho = dispCnt[ci];
+    if (rho < 1e-6f) {
+        accP[ci] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    int xm, xp, ym, yp, zm, zp;
+    if (periodic) {
+        xm = (x - 1 + G) % G; xp = (x + 1) % G;
+        ym = (y - 1 + G) % G; yp = (y + 1) % G;
+        zm = (z - 1 + G) % G; zp = (z + 1) % G;
+    } else {
+        xm = max(x - 1, 0); xp = min(x + 1, G - 1);
+        ym = max(y - 1, 0); yp = min(y + 1, G - 1);
+        zm = max(z - 1, 0); zp = min(z + 1, G - 1);
+    }
+    const float h = 0.5f * (float)G * k / rho;
+    float ax = -(dispX[((size_t)z * G + y) * G + xp] - dispX[((size_t)z * G + y) * G + xm]) * h;
+    float ay = -(dispY[((size_t)z * G + yp) * G + x] - dispY[((size_t)z * G + ym) * G + x]) * h;
+    float az = -(dispZ[((size_t)zp * G + y) * G + x] - dispZ[((size_t)zm * G + y) * G + x]) * h;
+    ax = fminf(fmaxf(ax, -maxAcc), maxAcc);
+    ay = fminf(fmaxf(ay, -maxAcc), maxAcc);
+    az = fminf(fmaxf(az, -maxAcc), maxAcc);
+    accP[ci] = make_float4(ax, ay, az, 0.f);
+}
+
+void launch_pressure(const float* disp, float4* accP, int G, int periodic, float k, float maxAcc, hipStream_t st) {
+    const long long cells = (long long)G * G * G;
+    hipLaunchKernelGGL(k_pressure, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, disp, disp + cells,
+                       disp + 2 * cells, disp + 3 * cells, accP, G, periodic, k, maxAcc);
+}
+
 // sampleAcc, Sim.cu:614-629.
 __device__ __forceinline__ float3 sample_acc(const float4* __restrict__ accG, const Cic& c, int G, int periodic) {
     const int x0 = wrap_or_clamp(c.ix, G, periodic), x1 = wrap_or_clamp(c.ix + 1, G, periodic);
@@ -834,6 +948,13 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
                     acc.y = fmaf(a.darkEnergy * k, p.y - 0.5f, acc.y);
                     acc.z = fmaf(a.darkEnergy * k, p.z - 0.5f, acc.z);
                 }
+            }
+            // gas pressure (Sim.cu:740-743): gas particles only (w <= 0), not dark matter
+            if (a.accPress && p.w <= 0.f && v.w >= -50.f) {
+                const float3 pr = sample_acc(a.accPress, cic_setup(p, fG), a.G, a.periodic);
+                acc.x += pr.x;
+                acc.y += pr.y;
+                acc.z += pr.z;
             }
             v.x = fmaf(acc.x, kick, v.x);
             v.y = fmaf(acc.y, kick, v.y);

@@ -71,6 +71,8 @@ struct sd_sim {
     float4* accG = nullptr;
     float2* tw = nullptr;
     float4* accContact = nullptr;
+    float* disp = nullptr;         // gas dispersion sums x, y, z and counts: 4 x float[G^3]
+    float4* accPress = nullptr;    // gas pressure acceleration, float4[G^3]
     int2* cellRange = nullptr;  // contact cell table: (first, one past last) sorted index per cell
     int2* blockRange = nullptr;  // contact (Morton keys): sorted index range per 4^3-cell block
     long long blockCells = 0;

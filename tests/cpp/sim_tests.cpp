@@ -375,6 +375,34 @@ static void testNoReallocOnSameRequest() {
           "same request kept, changed request reallocated", buf);
 }
 
+// [g] pressure and cooling on, as in the reference's SimConfig defaults (the
+// shim selects the reference kernel order for them): a cloud of gas with
+// random motions and gravity off loses most of its kinetic energy to cooling
+// within 60 steps, and the grid mass stays exact.
+static void testGasPhysics() {
+    std::printf("\n[g] gas pressure and cooling\n");
+    Sim sim;
+    SimConfig cfg = plain(200000, 64);
+    cfg.gravity = 0.0f;
+    cfg.pressureEnabled = true;
+    cfg.coolingEnabled = true;
+    cfg.coolingRate = 0.9f;
+    sim.init(cfg);
+    std::vector<float> p, v;
+    cloud(200000, 0.2f, 14, p, v);
+    Rng r(15);
+    for (size_t i = 0; i < v.size(); i += 4)
+        for (int d = 0; d < 3; ++d) v[i + d] = (float)(0.05 * (2.0 * r.next() - 1.0));
+    sim.setParticles(p.data(), v.data(), 200000);
+    const double e0 = sim.measureKineticEnergy();
+    for (int i = 0; i < 60; ++i) sim.step();
+    const double e1 = sim.measureKineticEnergy();
+    const double m = sim.measureTotalGridMass();
+    char buf[200];
+    std::snprintf(buf, sizeof(buf), "KE %.4e -> %.4e (%.2f), mass %.0f, failed %d", e0, e1, e1 / e0, m, (int)Sim::failed());
+    check(!Sim::failed() && e1 < 0.5 * e0 && m == 200000.0, "cooling removes random motion, mass kept", buf);
+}
+
 // State files round-trip (Sim.cu:4985-5120).
 static void testStateRoundTrip() {
     std::printf("\n[s] STARDUST state file\n");
@@ -424,6 +452,7 @@ int main() {
     testPeriodicSoftening();
     testNoReallocOnSameRequest();
     testStateRoundTrip();
+    testGasPhysics();
     testVramClamp();
     testRefusal();  // last: it leaves the sticky failure set
     std::printf("\n=== result: %d PASS / %d FAIL ===\n", g_pass, g_fail);

@@ -257,3 +257,80 @@ def test_density_device_views(gpu):
     s.run(2)
     assert s.measure_total_grid_mass() == pytest.approx(n, abs=0)
     s.close()
+
+
+def _gas_particles(n, seed):
+    """Clumped particles (several per cell): 70% gas (pos.w 0), 20% stars
+    (pos.w 1), 10% dark matter (vel.w -100)."""
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    centres = rng.uniform(0.3, 0.7, (6, 3))
+    pos[:, :3] = centres[rng.integers(0, 6, n)] + rng.normal(0, 0.03, (n, 3))
+    pos[:, :3] = np.clip(pos[:, :3], 0.01, 0.99)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.05, (n, 3))
+    kind = rng.random(n)
+    pos[(kind >= 0.7) & (kind < 0.9), 3] = 1.0
+    vel[kind >= 0.9, 3] = -100.0
+    return pos, vel
+
+
+@pytest.mark.parametrize("contact,periodic,key", [(False, False, "rowmajor"), (False, True, "morton"),
+                                                  (True, False, "morton"), (True, True, "rowmajor")])
+def test_gas_physics_vs_oracle(gpu, oracle, contact, periodic, key):
+    """Pressure and cooling (Sim.cu:1212-1415, 3605-3633, 3732-3830) in the
+    reference kernel order: the cooling pass runs every 2 steps (every step
+    with contact), sums each cell's velocities in sorted order and pulls gas
+    toward the cell mean; the pressure grid from its dispersion sums acts on
+    gas. Against the oracle's Sim::step with the same key order."""
+    n, G, steps, dt = 20000, 32, 6, 2e-3
+    pos, vel = _gas_particles(n, 81)
+    if periodic:
+        pos[:, :3] = np.mod(pos[:, :3], 1.0)
+    skey, okey = (sd.KEY_ROWMAJOR, oracle.KEY_ROWMAJOR) if key == "rowmajor" else (sd.KEY_MORTON, oracle.KEY_MORTON)
+    kw = dict(pressure=1, cooling=1, pressureK=0.45, coolingRate=0.9)
+    prm = oracle.make_params(G, periodic=periodic, gravity=0.3, softeningCells=1.5, sortInterval=3, keyMode=okey,
+                             fixedDt=dt, contact=int(contact), contactRadiusCells=0.3, contactStiffness=1e3,
+                             allocN=n, **kw)
+    st = oracle.Stepper(prm, pos, vel)
+    st.step(steps)
+    cfg = sd.default_config(particleCount=n, gridSize=G, boundary=int(periodic), gravity=0.3, softeningCells=1.5,
+                            sortInterval=3, contactEnabled=int(contact), contactRadiusCells=0.3,
+                            contactStiffness=1e3, pressureEnabled=1, coolingEnabled=1, pressureK=0.45,
+                            coolingRate=0.9)
+    out = {}
+    for gas in (True, False):
+        c = cfg
+        if not gas:
+            c = sd.default_config(**{f: getattr(cfg, f) for f, _ in sd.Config._fields_})
+            c.pressureEnabled = 0
+            c.coolingEnabled = 0
+        s = sd.Sim(c, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD, keyMode=skey))
+        s.set_fixed_dt(dt)
+        s.set_particles(pos, vel)
+        s.run(steps)
+        out[gas] = s.get_particles()
+        s.close()
+    gp, gv = out[True]
+    _say("gas", contact, periodic, key)
+    assert np.array_equal(gp[:, 0] < 0, st.pos[:, 0] < 0)
+    assert np.array_equal(gp[:, 3], st.pos[:, 3]) and np.array_equal(gv[:, 3], st.vel[:, 3])  # same order
+    dp = np.abs(gp[:, :3] - st.pos[:, :3]).max()
+    dv = np.abs(gv[:, :3] - st.vel[:, :3]).max() / np.abs(st.vel[:, :3]).max()
+    _say("  max |dx|", dp, "max rel dv", dv)
+    assert dp < 2e-6
+    assert dv < 1e-4
+    # the gas terms change the motion far beyond that tolerance
+    pv, vv = out[False]
+    assert np.abs(vv[:, :3] - gv[:, :3]).max() > 1e-2 * np.abs(gv[:, :3]).max()
+
+
+def test_gas_physics_needs_reference_order(gpu):
+    cfg = sd.default_config(particleCount=1000, gridSize=16, pressureEnabled=1)
+    with pytest.raises(sd.StardustError):
+        sd.Sim(cfg, sd.default_modes())  # fused pipeline, leapfrog
+    s = sd.Sim(cfg, sd.default_modes(pipeline=sd.PIPELINE_REFERENCE, integrator=sd.INTEGRATOR_KD))
+    with pytest.raises(sd.StardustError):
+        s.set_modes(sd.default_modes())
+    assert not sd.failed()
+    s.close()

@@ -521,3 +521,94 @@ def test_step_cfl_closed_form(speed, ts):
     assert st.state.lastDt == pytest.approx(float(dt), rel=1e-6)
     assert st.state.simTime == pytest.approx(k * float(dt), rel=1e-6)
     assert np.abs(st.pos[:, 0] - x).max() <= 1e-7
+
+
+# ---- gas cooling and pressure --------------------------------------------------
+def test_cool_cells_closed_form():
+    """kAccumCellVel + kCoolCell (Sim.cu:1212-1366) on hand-built cells: the mean
+    skips dark matter (vel.w < -50) but counts stars; gas (pos.w <= 0) moves
+    toward it by k = min(rate dt 60, 0.5), stars and dark matter keep their
+    velocities; a cell under two particles is left alone; dispersion sums are
+    sum (mean - v)^2 over the non-dark particles."""
+    G = 8
+    rng = np.random.default_rng(71)
+    cellA, cellB, cellC = (1, 2, 3), (5, 5, 5), (6, 0, 7)
+    def at(c, k):
+        return [(c[0] + 0.2 + 0.15 * k) / G, (c[1] + 0.5) / G, (c[2] + 0.5) / G]
+    pos, vel = [], []
+    for k, (w, vw) in enumerate([(0, 0), (0, 0), (0, 0), (1.0, 0), (0, -100.0)]):  # 3 gas, star, dark
+        pos.append(at(cellA, k) + [w]); vel.append(list(rng.normal(0, 0.3, 3)) + [vw])
+    pos.append(at(cellB, 0) + [0]); vel.append(list(rng.normal(0, 0.3, 3)) + [0])       # alone
+    for k in range(2):
+        pos.append(at(cellC, k) + [0]); vel.append(list(rng.normal(0, 0.3, 3)) + [0])
+    pos, vel = np.array(pos, np.float32), np.array(vel, np.float32)
+    keys, idx = O.cell_keys(pos, G, O.KEY_ROWMAJOR)
+    keys, idx = O.stable_sort_pairs(keys, idx)
+    pos, vel = pos[idx], vel[idx]
+    for rate, dt in ((0.08, 1e-3), (0.9, 0.02)):  # k = 0.0048 and k clamped to 0.5
+        v2, disp = O.cool_cells(pos, vel, keys, G, rate, dt)
+        k = min(rate * dt * 60.0, 0.5)
+        want = vel.astype(np.float64).copy()
+        dispw = np.zeros((4, G, G, G))
+        cells = np.clip((pos[:, :3] * G).astype(int), 0, G - 1)
+        for c in {tuple(x) for x in cells}:
+            sel = np.all(cells == c, axis=1) & (vel[:, 3] >= -50)
+            if sel.sum() < 2:
+                continue
+            m = vel[sel, :3].astype(np.float64).mean(axis=0)
+            dv = m - vel[sel, :3].astype(np.float64)
+            dispw[:3, c[2], c[1], c[0]] = (dv ** 2).sum(axis=0)
+            dispw[3, c[2], c[1], c[0]] = sel.sum()
+            gas = sel & (pos[:, 3] <= 0)
+            want[gas, :3] += k * (m - vel[gas, :3].astype(np.float64))
+        assert np.abs(v2 - want).max() < 1e-6
+        assert np.abs(disp - dispw).max() < 1e-6 * max(1.0, np.abs(dispw).max())
+        star = pos[:, 3] > 0
+        dark = vel[:, 3] < -50
+        assert np.array_equal(v2[star | dark], vel[star | dark])
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_pressure_closed_form(periodic):
+    """kPressure (Sim.cu:1368-1399): dispersion sums linear in x, D(x) = 2x + 1,
+    with 4 particles per cell give -(D(x+1) - D(x-1)) 0.5 G k / 4 = -G k / 2 in
+    the interior; the isolated edge uses the clamped half-width difference, the
+    periodic edge wraps (and is capped at maxAcc); an empty cell gets zero."""
+    G, k, cap = 8, 0.45, 5.0
+    disp = np.zeros((4, G, G, G), np.float32)
+    disp[0] = (2.0 * np.arange(G) + 1.0)[None, None, :]
+    disp[3] = 4.0
+    disp[3, 3, 4, 5] = 0.0
+    a = O.pressure(disp, G, periodic, k, cap)
+    h = 0.5 * G * k / 4.0
+    assert np.allclose(a[2, 2, 1:G - 1, 0], -4.0 * h, rtol=1e-6)
+    assert np.all(a[..., 1:] == 0)
+    assert np.all(a[3, 4, 5] == 0)
+    if periodic:
+        # x = 0: -(D(1) - D(G-1)) h = 2 (G - 2) h; x = G-1: -(D(0) - D(G-2)) h, the same
+        assert a[2, 2, 0, 0] == pytest.approx(min(cap, 2 * (G - 2) * h), rel=1e-6)
+        assert a[2, 2, G - 1, 0] == pytest.approx(min(cap, 2 * (G - 2) * h), rel=1e-6)
+        assert 2 * (G - 2) * h > cap  # the cap binds here
+    else:
+        assert a[2, 2, 0, 0] == pytest.approx(-2.0 * h, rel=1e-6)
+        assert a[2, 2, G - 1, 0] == pytest.approx(-2.0 * h, rel=1e-6)
+
+
+def test_step_cooling_keeps_momentum():
+    """Gravity off, cooling on, every particle gas: each cooling pass moves a
+    cell's velocities toward their mean, which leaves the total momentum
+    unchanged (to FP32 rounding) while the dispersion falls."""
+    n, G = 4000, 8
+    rng = np.random.default_rng(72)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.3, 0.7, (n, 3))
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.01, (n, 3))
+    prm = O.make_params(G, gravity=0.0, softeningCells=1.0, sortInterval=0, fixedDt=1e-3, momentumZero=0,
+                        cooling=1, coolingRate=2.0, allocN=n)
+    st = O.Stepper(prm, pos, vel)
+    p0 = vel[:, :3].astype(np.float64).sum(axis=0)
+    st.step(16)
+    p1 = st.vel[:, :3].astype(np.float64).sum(axis=0)
+    assert np.abs(p1 - p0).max() < 1e-5  # total |v| is ~30: FP32 rounding of the updates
+    assert (st.vel[:, :3].astype(np.float64) ** 2).sum() < 0.9 * (vel[:, :3].astype(np.float64) ** 2).sum()
