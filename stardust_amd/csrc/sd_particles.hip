@@ -869,9 +869,12 @@ __device__ __forceinline__ float speed3(float x, float y, float z) {
     return sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
 }
 
-// kIntegrate, Sim.cu:685-906 (plain PM: no black holes, no pressure). Each
-// thread advances KP particles (i = block * BP + k * PT + thread, coalesced for
-// every k); the new positions feed the block deposit when depositTo is set.
+// kIntegrate, Sim.cu:685-906 (no black holes). Each thread advances KP
+// particles (i = block * BP + k * PT + thread, coalesced for every k); the new
+// positions feed the block deposit when depositTo is set. GAS adds the gas
+// pressure gather (a.accPress); it is a separate instantiation because its
+// registers would cost the plain step a wave per SIMD (79 against 71 VGPRs).
+template <bool GAS>
 __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) {
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm = 0.f;
@@ -950,7 +953,7 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
                 }
             }
             // gas pressure (Sim.cu:740-743): gas particles only (w <= 0), not dark matter
-            if (a.accPress && p.w <= 0.f && v.w >= -50.f) {
+            if (GAS && p.w <= 0.f && v.w >= -50.f) {
                 const float3 pr = sample_acc(a.accPress, cic_setup(p, fG), a.G, a.periodic);
                 acc.x += pr.x;
                 acc.y += pr.y;
@@ -1038,7 +1041,8 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
     if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials, blk);
 }
 
-__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) { integrate_body(a, blockIdx.x); }
+template <bool GAS>
+__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) { integrate_body<GAS>(a, blockIdx.x); }
 
 // Integrate of many simulations in one launch: block b belongs to job j with
 // prefix[j] <= b < prefix[j + 1] and runs that job's block b - prefix[j], so
@@ -1047,7 +1051,7 @@ __global__ void __launch_bounds__(PT) k_integrate_batch(const IntegrateArgs* __r
                                                         const int* __restrict__ prefix, int njobs) {
     const int j = find_job(prefix, njobs, (int)blockIdx.x);
     const IntegrateArgs a = jobs[j];
-    integrate_body(a, (int)blockIdx.x - prefix[j]);
+    integrate_body<false>(a, (int)blockIdx.x - prefix[j]);
 }
 
 int integrate_blocks(int n) { return (n + BP - 1) / BP; }
@@ -1055,7 +1059,10 @@ int particle_blocks(int n) { return (n + PT - 1) / PT; }
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t st) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_integrate, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
+    if (a.accPress)
+        hipLaunchKernelGGL(k_integrate<true>, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_integrate<false>, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
 }
 
 // kMaxSpeed (Sim.cu:2923-2930) as block partials.
