@@ -22,7 +22,14 @@
 //
 // Complex arrays keep kx (S/2+1 values) contiguous with a row pitch P that is
 // a multiple of 16 complex (128 B), so every column tile of 16 is one cache
-// line. The isolated Green spectrum is real and even in each axis; only its
+// line. Packed layout (the one-tile column kernels, S < 512): the real
+// kx = S/2 bin rides in the imaginary part of the real kx = 0 bin, so the
+// column passes see S/2 columns (whole tiles) instead of S/2 + 1 (a ninth
+// tile with one live column). The y pass separates the two real sequences by
+// Hermitian symmetry within its line (ky with S-ky): kx = 0 keeps ky <= S/2 in
+// place, the kx = S/2 values of ky = 1 .. S/2-1 go to slots S-ky (slots 0 and
+// S/2 hold both, as real and imaginary parts); the z pass applies each its own
+// multiplier, and the inverse y pass puts the packed line back together. The isolated Green spectrum is real and even in each axis; only its
 // octant kx, ky, kz <= S/2 is stored, pre-multiplied by 1/S^3.
 #include "sd_device.h"
 
@@ -315,6 +322,7 @@ struct RowsFwdArgs {
     int nin;       // input values per row (even)
     int P;         // output row pitch (complex, > N/2 + 1)
     int clearSrc;
+    int pack;      // packed layout: bin N/2 in the imaginary part of bin 0
     int S;         // Green: grid side
     float soft;    // Green: softening (cells)
     FinalizeJob fin;  // reduction carried by block 0 (fin.partials == nullptr: none)
@@ -455,11 +463,13 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
     for (int it = 0; it < OITERS; ++it) {
         const int q = threadIdx.x + it * THREADS;
         const int cc = q / KP2, j = q - cc * KP2;
-        if (cc >= TBR) continue;
+        if (cc >= TBR || (a.pack && j == N / 4)) continue;
         const float2* l = s_line + cc * LP;
         if constexpr (RPL == 1) {
             if (row0 + cc < a.rows) {
-                const float2 u = l[PADI(2 * j)], v = l[PADI(2 * j + 1)];
+                float2 u = l[PADI(2 * j)];
+                const float2 v = l[PADI(2 * j + 1)];
+                if (a.pack && j == 0) u.y = l[PADI(N / 2)].x;  // both bins are real
                 reinterpret_cast<float4*>(a.out + (size_t)(row0 + cc) * a.P)[j] = make_float4(u.x, u.y, v.x, v.y);
             }
         } else {
@@ -470,6 +480,11 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
                 const float2 z = l[PADI(k)], m = l[PADI(k == 0 ? 0 : N - k)];  // k <= N/2 + 1
                 A[e] = make_float2(0.5f * (z.x + m.x), 0.5f * (z.y - m.y));
                 B[e] = make_float2(0.5f * (z.y + m.y), 0.5f * (m.x - z.x));
+            }
+            if (a.pack && j == 0) {  // bin N/2 of each row: the real and imaginary parts of Z[N/2]
+                const float2 zh = l[PADI(N / 2)];
+                A[0].y = zh.x;
+                B[0].y = zh.y;
             }
             const int ra = row0 + 2 * cc;
             if (ra < a.rows)
@@ -523,6 +538,7 @@ struct RowsInvArgs {
     int rows;
     int P;
     int nout;  // real values kept per row (row stride of out), multiple of 4
+    int pack;  // packed layout: bin N/2 in the imaginary part of bin 0
 };
 
 template <int N>
@@ -545,7 +561,8 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2
         for (int h = 0; h < RPL; ++h) {
             const int row = row0 + RPL * c + h;
             buf[it][h] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (c < TBR && row < a.rows) buf[it][h] = reinterpret_cast<const float4*>(a.in + (size_t)row * a.P)[j];
+            if (c < TBR && row < a.rows && !(a.pack && j == N / 4))
+                buf[it][h] = reinterpret_cast<const float4*>(a.in + (size_t)row * a.P)[j];
         }
     }
 #pragma unroll
@@ -556,13 +573,18 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs& a, const float2
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int k = 2 * j + e;
-                if (k > N / 2) continue;
+                if (k > N / 2 || (a.pack && k == N / 2)) continue;
                 float2 A = e ? make_float2(buf[it][0].z, buf[it][0].w) : make_float2(buf[it][0].x, buf[it][0].y);
                 if constexpr (RPL == 1) {
+                    if (a.pack && k == 0) {  // bins 0 and N/2, both real
+                        s_line[c * LP + PADI(N / 2)] = make_float2(A.y, 0.f);
+                        A.y = 0.f;
+                    }
                     s_line[c * LP + PADI(k)] = A;
                     if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(A.x, -A.y);
                 } else {
                     float2 B = e ? make_float2(buf[it][1].z, buf[it][1].w) : make_float2(buf[it][1].x, buf[it][1].y);
+                    if (a.pack && k == 0) s_line[c * LP + PADI(N / 2)] = make_float2(A.y, B.y);
                     if (k == 0 || k == N / 2) A.y = B.y = 0.f;  // the real bins of a real row
                     s_line[c * LP + PADI(k)] = make_float2(A.x - B.y, A.y + B.x);
                     if (k > 0 && k < N / 2) s_line[c * LP + PADI(N - k)] = make_float2(A.x + B.y, B.x - A.y);
@@ -619,7 +641,26 @@ struct ColArgs {
     float perSoft;       // periodic: (2 pi soft / G)^2 (0 disables)
     float greenNorm;     // COL_GREEN: 1/S^3
     FinalizeJob fin;     // fused z pass: reduction carried by block (0, 0) (fin.partials == nullptr: none)
+    int pack;            // packed layout (nb = S/2; column 0 carries kx = 0 and kx = S/2)
 };
+
+// The z pass multiplier at (kx, ky, kz), kx <= N/2: the isolated Green
+// octant (folded ky, kz) or kPoissonPeriodic (Sim.cu:528-557).
+template <int MODE>
+__device__ __forceinline__ float z_mult(const ColArgs& a, int N, int kx, int ky, int kz) {
+    const int h = N / 2;
+    if constexpr (MODE == COL_Z_ISO) {
+        const int kyf = ky <= h ? ky : N - ky, kzf = kz <= h ? kz : N - kz;
+        return a.green[((size_t)kzf * (h + 1) + kyf) * a.P + kx];
+    } else {
+        const int sky = ky <= h ? ky : ky - N, skz = kz <= h ? kz : kz - N;
+        const float k2 = (float)(kx * kx + sky * sky + skz * skz);
+        if (k2 < 1e-6f) return 0.f;
+        float m = a.perScale / k2;
+        if (a.perSoft > 0.f) m *= expf(-0.5f * k2 * a.perSoft);
+        return m;
+    }
+}
 
 // One line transform; halfIn: the upper half of the line is zero, halfOut:
 // only its lower half is read afterwards (pruned stages where the plan has them).
@@ -666,12 +707,35 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
     __syncthreads();
     const int c = threadIdx.x / Plan<N>::T, t = threadIdx.x % Plan<N>::T;
     float2* line = s_line + c * LP;
+    // packed column 0 (tile 0): the inverse y line is U = W0 + i WN, where slot
+    // k <= N/2 holds W0(k) and slot N-k holds WN(k) (0 < k < N/2), and
+    // W(N-k) = conj W(k) for both real-signal spectra; slots 0 and N/2 already
+    // hold W0 + i WN
+    if (MODE == COL_INV && a.pack && bx == 0) {
+        for (int k = 1 + threadIdx.x; k < N / 2; k += THREADS) {
+            const float2 w0 = s_line[PADI(k)], wn = s_line[PADI(N - k)];
+            s_line[PADI(k)] = make_float2(w0.x - wn.y, w0.y + wn.x);
+            s_line[PADI(N - k)] = make_float2(w0.x + wn.y, wn.x - w0.y);
+        }
+        __syncthreads();
+    }
     // zero-padded lines (isolated meshes: G of S = 2G rows loaded) and
     // half-stored results take the pruned first / last stage
     if (MODE == COL_INV) {
         run_line<N, true>(line, s_tw, t, false, 2 * a.nout <= N);
     } else {
         run_line<N, false>(line, s_tw, t, 2 * a.nin <= N, false);
+    }
+    // packed column 0 (tile 0) after the forward y transform: Z = F0 + i FN of
+    // the two real columns; F0(k) = (Z(k) + conj Z(N-k)) / 2 stays in slot k,
+    // FN(k) = (Z(k) - conj Z(N-k)) / 2i goes to slot N-k (0 < k < N/2)
+    if (MODE == COL_FWD && a.pack && bx == 0) {
+        for (int k = 1 + threadIdx.x; k < N / 2; k += THREADS) {
+            const float2 z = s_line[PADI(k)], m = s_line[PADI(N - k)];
+            s_line[PADI(k)] = make_float2(0.5f * (z.x + m.x), 0.5f * (z.y - m.y));
+            s_line[PADI(N - k)] = make_float2(0.5f * (z.y + m.y), 0.5f * (m.x - z.x));
+        }
+        __syncthreads();
     }
     if (MODE == COL_GREEN) {
         const int h = N / 2;
@@ -688,6 +752,12 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
         const int ky = slab;
         const int kyf = ky <= h ? ky : N - ky;
         const int kx0 = b0 + col;
+        // packed column 0: slabs 0 < ky < N/2 hold kx = 0, slabs ky > N/2 hold
+        // kx = N/2 at ky' = N - ky (the same folded ky and |sky|), slabs 0 and
+        // N/2 hold both (done below)
+        const bool col0 = a.pack && kx0 == 0;
+        const bool both = col0 && (ky == 0 || ky == h);
+        const int kxa = col0 && ky > h ? h : kx0;
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int kz = r0 + it * RPI;
@@ -696,9 +766,11 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
             if (MODE == COL_Z_ISO) {
                 const int kzf = kz <= h ? kz : N - kz;
                 // the table row has pitch P > nb, so the pair never leaves it
-                const float2 g = *reinterpret_cast<const float2*>(a.green + ((size_t)kzf * (h + 1) + kyf) * a.P + kx0);
+                const float* grow = a.green + ((size_t)kzf * (h + 1) + kyf) * a.P;
+                const float2 g = *reinterpret_cast<const float2*>(grow + kx0);
                 m0 = g.x;
                 m1 = g.y;
+                if (kxa != kx0) m0 = grow[kxa];
             } else {
                 // kPoissonPeriodic, Sim.cu:528-557
                 const int sky = ky <= h ? ky : ky - N;
@@ -706,7 +778,7 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
                 float mm[2];
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
-                    const int kx = kx0 + e;
+                    const int kx = e ? kx0 + 1 : kxa;
                     const float k2 = (float)(kx * kx + sky * sky + skz * skz);
                     if (k2 < 1e-6f) {
                         mm[e] = 0.f;
@@ -719,8 +791,26 @@ __device__ __forceinline__ void cols_body(const ColArgs& a, const float2* __rest
                 m1 = mm[1];
             }
             const float2 u = s_line[col * LP + PADI(kz)], v = s_line[(col + 1) * LP + PADI(kz)];
-            s_line[col * LP + PADI(kz)] = make_float2(u.x * m0, u.y * m0);
+            if (!both) s_line[col * LP + PADI(kz)] = make_float2(u.x * m0, u.y * m0);
             s_line[(col + 1) * LP + PADI(kz)] = make_float2(v.x * m1, v.y * m1);
+        }
+        // ky = 0 and N/2 of packed column 0: the line is Z = a + i b of two real
+        // z-sequences (kx = 0 and kx = N/2), whose spectra take multipliers m0
+        // and mN; with both even in kz, W(kz) = Z(kz) (m0 + mN) / 2 +
+        // conj Z(N - kz) (m0 - mN) / 2 (real and imaginary parts apart at kz = 0, N/2)
+        if (a.pack && bx == 0 && (ky == 0 || ky == h)) {
+            for (int kz = threadIdx.x; kz <= h; kz += THREADS) {
+                const float mz0 = z_mult<MODE>(a, N, 0, ky, kz), mzn = z_mult<MODE>(a, N, h, ky, kz);
+                const int km = kz == 0 ? 0 : N - kz;
+                const float2 z1 = s_line[PADI(kz)], z2 = s_line[PADI(km)];
+                if (kz == 0 || kz == h) {
+                    s_line[PADI(kz)] = make_float2(z1.x * mz0, z1.y * mzn);
+                } else {
+                    const float sp = 0.5f * (mz0 + mzn), sm = 0.5f * (mz0 - mzn);
+                    s_line[PADI(kz)] = make_float2(z1.x * sp + z2.x * sm, z1.y * sp - z2.y * sm);
+                    s_line[PADI(km)] = make_float2(z2.x * sp + z1.x * sm, z2.y * sp - z1.y * sm);
+                }
+            }
         }
         __syncthreads();
         run_line<N, true>(line, s_tw, t, false, 2 * a.nout <= N);
@@ -1014,6 +1104,10 @@ bool fft_length_supported(int n) {
 
 int fft_pitch(int S) { return ((S / 2 + 1) + 15) / 16 * 16; }
 
+// Whether the solve at FFT side S uses the packed half-spectrum layout: the
+// one-tile column kernels implement it, the persistent ones (S >= 512) not.
+static bool fft_packs(int S) { return S < SD_COLS_STREAM_MIN_N; }
+
 // The pass that carries a partials reduction at FFT side N: 1 the x pass, 2
 // the fused z pass (one-tile form), 0 none (it is launched before the passes).
 template <int N>
@@ -1043,6 +1137,7 @@ int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc,
     a.nin = m.G;
     a.P = m.P;
     a.clearSrc = clearSrc;
+    a.pack = fft_packs(m.S);
 #define CALL(NN) launch_rows_fwd<NN>(a, 0, m.tw, st)
     SD_DISPATCH_N(m.S, CALL)
 #undef CALL
@@ -1051,14 +1146,14 @@ int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc,
 
 int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
                     const FinalizeJob* fin, hipStream_t st) {
-    const int G = m.G, S = m.S, P = m.P, NK = S / 2 + 1;
+    const int G = m.G, S = m.S, P = m.P, pack = fft_packs(S) ? 1 : 0, NK = S / 2 + 1 - pack;
     // Y forward: A[z][y][kx] (y < G) -> B[z][ky][kx]
     {
         ColArgs a{};
         a.in = A; a.out = B;
         a.inA = (long long)G * P; a.inN = P;
         a.outA = (long long)S * P; a.outN = P;
-        a.nb = NK; a.nin = G; a.nout = S; a.P = P; a.S = S;
+        a.nb = NK; a.nin = G; a.nout = S; a.P = P; a.S = S; a.pack = pack;
 #define CALL(NN) launch_cols<NN>(a, COL_FWD, G, m.tw, st)
         SD_DISPATCH_N(S, CALL)
 #undef CALL
@@ -1069,7 +1164,7 @@ int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green,
         a.in = B; a.out = B;
         a.inA = P; a.inN = (long long)S * P;
         a.outA = P; a.outN = (long long)S * P;
-        a.nb = NK; a.nin = G; a.nout = G; a.P = P; a.S = S;
+        a.nb = NK; a.nin = G; a.nout = G; a.P = P; a.S = S; a.pack = pack;
         a.green = green;
         a.perScale = m.perScale;
         a.perSoft = m.perSoft;
@@ -1085,7 +1180,7 @@ int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green,
         a.in = B; a.out = A;
         a.inA = (long long)S * P; a.inN = P;
         a.outA = (long long)G * P; a.outN = P;
-        a.nb = NK; a.nin = S; a.nout = G; a.P = P; a.S = S;
+        a.nb = NK; a.nin = S; a.nout = G; a.P = P; a.S = S; a.pack = pack;
 #define CALL(NN) launch_cols<NN>(a, COL_INV, G, m.tw, st)
         SD_DISPATCH_N(S, CALL)
 #undef CALL
@@ -1093,7 +1188,7 @@ int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green,
     // X inverse: A rows -> pot (x < G)
     {
         RowsInvArgs a{};
-        a.in = A; a.out = pot; a.rows = G * G; a.P = P; a.nout = G;
+        a.in = A; a.out = pot; a.rows = G * G; a.P = P; a.nout = G; a.pack = pack;
 #define CALL(NN) launch_rows_inv<NN>(a, m.tw, st)
         SD_DISPATCH_N(S, CALL)
 #undef CALL
@@ -1167,7 +1262,7 @@ MeshGroup* mesh_group_create(const MeshGroupJob* jobs, int n) {
         if (m.G != m0.G || m.S != m0.S || m.P != m0.P || m.periodic != m0.periodic) return nullptr;
     }
     if (!fft_length_supported(m0.S)) return nullptr;
-    const int G = m0.G, S = m0.S, P = m0.P, NK = S / 2 + 1;
+    const int G = m0.G, S = m0.S, P = m0.P, pack = fft_packs(S) ? 1 : 0, NK = S / 2 + 1 - pack;
     std::vector<RowsFwdArgs> xf(n);
     std::vector<ColArgs> yf(n), z(n), yi(n);
     std::vector<RowsInvArgs> xi(n);
@@ -1177,26 +1272,26 @@ MeshGroup* mesh_group_create(const MeshGroupJob* jobs, int n) {
         const MeshPlan& m = J.plan;
         RowsFwdArgs& a = xf[j];
         a = RowsFwdArgs{};
-        a.src = J.rhoFix; a.out = J.A; a.rows = G * G; a.nin = G; a.P = P; a.clearSrc = 1;
+        a.src = J.rhoFix; a.out = J.A; a.rows = G * G; a.nin = G; a.P = P; a.clearSrc = 1; a.pack = pack;
         ColArgs& b = yf[j];
         b = ColArgs{};
         b.in = J.A; b.out = J.B;
         b.inA = (long long)G * P; b.inN = P; b.outA = (long long)S * P; b.outN = P;
-        b.nb = NK; b.nin = G; b.nout = S; b.P = P; b.S = S;
+        b.nb = NK; b.nin = G; b.nout = S; b.P = P; b.S = S; b.pack = pack;
         ColArgs& c = z[j];
         c = ColArgs{};
         c.in = J.B; c.out = J.B;
         c.inA = P; c.inN = (long long)S * P; c.outA = P; c.outN = (long long)S * P;
-        c.nb = NK; c.nin = G; c.nout = G; c.P = P; c.S = S;
+        c.nb = NK; c.nin = G; c.nout = G; c.P = P; c.S = S; c.pack = pack;
         c.green = J.green; c.perScale = m.perScale; c.perSoft = m.perSoft;
         ColArgs& d = yi[j];
         d = ColArgs{};
         d.in = J.B; d.out = J.A;
         d.inA = (long long)S * P; d.inN = P; d.outA = (long long)G * P; d.outN = P;
-        d.nb = NK; d.nin = S; d.nout = G; d.P = P; d.S = S;
+        d.nb = NK; d.nin = S; d.nout = G; d.P = P; d.S = S; d.pack = pack;
         RowsInvArgs& e = xi[j];
         e = RowsInvArgs{};
-        e.in = J.A; e.out = J.pot; e.rows = G * G; e.P = P; e.nout = G;
+        e.in = J.A; e.out = J.pot; e.rows = G * G; e.P = P; e.nout = G; e.pack = pack;
         gr[j] = GradJob{J.pot, J.accG, G, 0.5f * (float)G * J.potScale, m.periodic, 0};
     }
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -1236,7 +1331,7 @@ void mesh_group_destroy(MeshGroup* g) {
 template <int N>
 static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
     const dim3 blk(TB * Plan<N>::T), fblk(XFwd::LINES * Plan<N>::T), iblk(XInv::LINES * Plan<N>::T);
-    const int NK = g.S / 2 + 1, colBlocks = (NK + TB - 1) / TB;
+    const int NK = g.S / 2 + 1 - (fft_packs(g.S) ? 1 : 0), colBlocks = (NK + TB - 1) / TB;
     const int fBlocks = (g.G * g.G + XFwd::ROWS - 1) / XFwd::ROWS, iBlocks = (g.G * g.G + XInv::ROWS - 1) / XInv::ROWS;
     hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(fBlocks, g.n), fblk, 0, st, g.xf, g.tw);
     hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
