@@ -97,10 +97,18 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __re
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * SORT_THREADS * R;
+    // every key load of the thread in flight at once, then the counting
+    uint32_t kk[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = base + r * SORT_THREADS + threadIdx.x;
+        kk[r] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = base + r * SORT_THREADS + threadIdx.x;
         const bool valid = i < n;
-        const uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
+        const uint32_t d = (kk[r] >> shift) & 255u;
         const uint64_t active = __ballot(valid);
         if (valid) {
             const uint64_t m = match_digit(d, active);
@@ -124,12 +132,20 @@ __global__ void __launch_bounds__(SORT_THREADS) k_cell_keys_hist(const float4* _
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * SORT_THREADS * R;
+    // every position load of the thread in flight at once, then keys and counts
+    float4 pp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = base + r * SORT_THREADS + threadIdx.x;
+        pp[r] = i < n ? pos[i] : make_float4(-1.f, -1.f, -1.f, 0.f);
+    }
+#pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = base + r * SORT_THREADS + threadIdx.x;
         const bool valid = i < n;
         uint32_t d = 0u;
         if (valid) {
-            const uint32_t k = cell_key(pos[i], G, mode);
+            const uint32_t k = cell_key(pp[r], G, mode);
             keys[i] = k;
             idx[i] = i;
             d = k & 255u;
@@ -244,15 +260,25 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
     __syncthreads();
     const int base = blockIdx.x * SORT_THREADS * R;
     const int lane = t & 63, wv = t >> 6;
+    // every key and value load of the thread in flight at once
+    uint32_t kk[R];
+    int32_t vv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        kk[r] = 0u;
+        vv[r] = 0;
+        if (i < n) {
+            kk[r] = keys[i];
+            vv[r] = vals[i];
+        }
+    }
+#pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = base + r * SORT_THREADS + t;
         const bool valid = i < n;
-        uint32_t key = 0;
-        int32_t val = 0;
-        if (valid) {
-            key = keys[i];
-            val = vals[i];
-        }
+        const uint32_t key = kk[r];
+        const int32_t val = vv[r];
         const uint32_t d = (key >> shift) & 255u;
         const uint64_t active = __ballot(valid);
         uint32_t rank = 0;
