@@ -138,28 +138,40 @@ def _counter_rows(path, counter):
     return out
 
 
-def traffic_summary(fetch_csv, write_csv, mib, out_path, command):
-    """HBM bytes per k_integrate launch from two counter runs (FETCH_SIZE and
-    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
-    for the copy kernel of `mib` MiB (reads mib, writes mib)."""
-    res = {}
-    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
-        rows = _counter_rows(path, name)
-        integ = [v for (d, k), v in rows.items() if "k_integrate" in k]
-        # the calibration copies are the last three copy-kernel dispatches
-        copy = [v for (d, k), v in sorted(rows.items(), key=lambda kv: int(kv[0][0])) if "copyBuffer" in k][-3:]
-        if not integ or len(copy) < 3:
-            raise RuntimeError(f"{path}: no k_integrate or calibration copy dispatches for {name}")
-        integ_kib = sorted(integ)[len(integ) // 2]
-        copy_kib = sorted(copy)[1]
-        factor = (mib * 1024.0) / copy_kib
-        res[name] = {"k_integrate_kib_median": integ_kib, "copy_kib_median": copy_kib,
-                     "copy_kib_last3": copy, "scale": factor, "bytes": integ_kib * 1024.0 * factor,
-                     "dispatches": len(integ)}
-    total = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
-    doc = {"kernel": "sd::k_integrate", "workload": "two-disk 4,194,304 particles, G=128",
-           "copy_calibration_mib": mib, "counters": res, "hbm_bytes_per_launch": round(total),
+TWO_DISK = "two-disk 4,194,304 particles, G=128"
+
+
+def _kernel_counters(path, name, kernel, mib):
+    rows = _counter_rows(path, name)
+    sel = [v for (d, k), v in rows.items() if kernel in k]
+    # the calibration copies are the last three copy-kernel dispatches
+    copy = [v for (d, k), v in sorted(rows.items(), key=lambda kv: int(kv[0][0])) if "copyBuffer" in k][-3:]
+    if not sel or len(copy) < 3:
+        raise RuntimeError(f"{path}: no {kernel} or calibration copy dispatches for {name}")
+    kib = sorted(sel)[len(sel) // 2]
+    copy_kib = sorted(copy)[1]
+    factor = (mib * 1024.0) / copy_kib
+    return {"kernel_kib_median": kib, "copy_kib_median": copy_kib, "copy_kib_last3": copy, "scale": factor,
+            "bytes": kib * 1024.0 * factor, "dispatches": len(sel)}
+
+
+def traffic_summary(fetch_csv, write_csv, mib, out_path, command, kernels=("k_integrate",), workload=TWO_DISK):
+    """HBM bytes per launch of each kernel (name substrings, the first one the
+    roofline kernel) from two counter runs (FETCH_SIZE and WRITE_SIZE, KiB per
+    dispatch, median over dispatches), each scaled by what the same counter
+    reads for the copy kernel of `mib` MiB (reads mib, writes mib)."""
+    per = {}
+    for kern in kernels:
+        res = {name: _kernel_counters(path, name, kern, mib)
+               for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv))}
+        per[kern] = {"counters": res,
+                     "hbm_bytes_per_launch": round(res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"])}
+    first = per[kernels[0]]
+    doc = {"kernel": "sd::" + kernels[0], "workload": workload, "copy_calibration_mib": mib,
+           "counters": first["counters"], "hbm_bytes_per_launch": first["hbm_bytes_per_launch"],
            "command": command}
+    if len(kernels) > 1:
+        doc["other_kernels"] = {k: per[k] for k in kernels[1:]}
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1)
     print(json.dumps(doc, indent=1))
@@ -168,8 +180,16 @@ def traffic_summary(fetch_csv, write_csv, mib, out_path, command):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
         # offline: bench.py --traffic-summary FETCH.csv WRITE.csv MIB OUT.json "COMMAND"
+        #          [KERNEL ...] [--workload TEXT]   (KERNEL: name substrings, default k_integrate)
         fetch_csv, write_csv, mib, out_path, command = sys.argv[2:7]
-        traffic_summary(fetch_csv, write_csv, int(mib), out_path, command)
+        rest = sys.argv[7:]
+        workload = TWO_DISK
+        if "--workload" in rest:
+            i = rest.index("--workload")
+            workload = rest[i + 1]
+            rest = rest[:i] + rest[i + 2:]
+        traffic_summary(fetch_csv, write_csv, int(mib), out_path, command, tuple(rest) or ("k_integrate",),
+                        workload)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,12 +280,14 @@ def main():
         # not make: `traffic` stays null and the committed counter summary of
         # the same kernel and workload is quoted beside it, with its source
         profiled = None
-        tpath = os.path.join(ROOT, "profiles", "traffic_k_integrate_two_disk_4M_G128.json")
-        if not contact and args.kernel == "integrate" and G == 128 and os.path.exists(tpath):
+        tname = ("traffic_cosmic_web_16M_G512.json" if contact and G == 512
+                 else "traffic_k_integrate_two_disk_4M_G128.json" if not contact and G == 128 else None)
+        tpath = os.path.join(ROOT, "profiles", tname) if tname else None
+        if args.kernel == "integrate" and tpath and os.path.exists(tpath):
             with open(tpath) as f:
                 profiled = {"bytes_per_launch": json.load(f).get("hbm_bytes_per_launch"),
-                            "source": "profiles/traffic_k_integrate_two_disk_4M_G128.json (FETCH_SIZE and "
-                                      "WRITE_SIZE counter runs of bench.py, not of this run)"}
+                            "source": f"profiles/{tname} (FETCH_SIZE and WRITE_SIZE counter runs of bench.py, "
+                                      "not of this run)"}
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "traffic_profiled": profiled,
                 "kernel": "sd::k_integrate", "avg_launch_us": round(avg_s * 1e6, 3),
