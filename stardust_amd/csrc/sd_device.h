@@ -10,6 +10,20 @@
 namespace sd {
 
 // Cell-index clamp/wrap of the reference's gidx3 (Sim.cu:96-100) on a G-sided grid.
+// XCD-aware block order. Workgroups of a launch are dealt to the 8 XCDs in
+// turn (block b to XCD b % 8), and each XCD has its own L2. Renumbering so
+// that XCD x runs one contiguous range of logical blocks keeps neighbouring
+// tiles (and the data they share, e.g. a potential stencil halo) in one L2.
+// A bijection on [0, nb): the first nb % 8 XCDs (all, when 8 divides nb) get
+// ceil(nb / 8) blocks, the others one fewer.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int per = (nb + NUM_XCD - 1) / NUM_XCD;
+    const int tall = nb % NUM_XCD == 0 ? NUM_XCD : nb % NUM_XCD;
+    const int x = b % NUM_XCD, local = b / NUM_XCD;
+    return x < tall ? x * per + local : tall * per + (x - tall) * (per - 1) + local;
+}
+
 __device__ __forceinline__ int wrap_or_clamp(int c, int G, int periodic) {
     if (periodic) {
         // G is a power of two whenever the boundary is periodic; the modulo form

@@ -26,6 +26,12 @@
 
 #pragma clang fp contract(off)
 
+// Block renumbering so that each XCD runs a contiguous range of tiles
+// (xcd_block); 0 keeps the hardware order.
+#ifndef SD_XCD_REMAP
+#define SD_XCD_REMAP 1
+#endif
+
 namespace sd {
 
 #ifndef SD_PT
@@ -1041,8 +1047,12 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
     if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials, blk);
 }
 
+// Consecutive particle tiles (Morton-sorted: neighbouring stencils) run on one
+// XCD, so the potential lines their gathers share stay in that XCD's L2.
 template <bool GAS>
-__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) { integrate_body<GAS>(a, blockIdx.x); }
+__global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
+    integrate_body<GAS>(a, SD_XCD_REMAP ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x);
+}
 
 // Integrate of many simulations in one launch: block b belongs to job j with
 // prefix[j] <= b < prefix[j + 1] and runs that job's block b - prefix[j], so
@@ -1318,7 +1328,8 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
                                                const int2* __restrict__ range, const uint32_t* __restrict__ keys,
                                                const int2* __restrict__ blockRange, float radius, float stiffness,
                                                float damping, float4* __restrict__ accOut) {
-    const int i = blockIdx.x * PT + threadIdx.x;
+    // consecutive (sorted: neighbouring) particles on one XCD share candidates in its L2
+    const int i = (SD_XCD_REMAP ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * PT + threadIdx.x;
     if (i >= n) return;
     const int si = order ? order[i] : i;
     const float4 p = pos[si];
