@@ -990,6 +990,376 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols_batch(const ColArgs* _
     cols_body<N, MODE>(a, tw, blockIdx.x, blockIdx.y);
 }
 
+// ---- register-resident column passes at N = 256 --------------------------------
+// 256 = 16 x 16 four-step transform (Bailey, "FFTs in external or hierarchical
+// memory", J. Supercomputing 4, 1990) held in registers. Thread (c, t) of a
+// 256-thread block owns column c = threadIdx % 16 of the tile and elements
+// n = t + 16 r (t = threadIdx / 16, r = 0 .. 15) of that column's line, loaded
+// straight from HBM (the 16 lanes of one row read one 128-B line). It
+// transforms over r in registers, multiplies by W^(t k2), and exchanges
+// through LDS once (element (t, k2) to thread k2), then transforms over t:
+//   X[k2 + 16 k1] = sum_t W16^(t k1) W^(t k2) sum_r W16^(r k2) x[t + 16 r].
+// Thread t then holds outputs k = t + 16 k1, the distribution its inputs had,
+// so the fused z pass multiplies and transforms back in place and every result
+// is stored straight from registers: a line crosses LDS once per transform
+// instead of once per radix stage plus the staging in and out.
+// HIN: inputs n >= 128 are zero padding (the first transform over r skips
+// them); HOUT: only outputs < 128 are stored (the last transform over t of the
+// inverse computes only those).
+constexpr int RLP = 257;  // LDS pitch of one line (float2): odd, so the 16 lines of a row fall in distinct banks
+
+template <bool INV, bool ZIN>
+__device__ __forceinline__ void fft256_regs(float2 v[16], float2* __restrict__ s_x, const float2* __restrict__ s_tw,
+                                            int c, int t, bool syncFirst) {
+    Dft<16, INV, ZIN>::run(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        float2 w = s_tw[t * k];
+        if (INV) w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+    if (syncFirst) __syncthreads();  // earlier readers of s_x are done
+    float2* line = s_x + c * RLP;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) line[k * 16 + t] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = line[t * 16 + k];
+    Dft<16, INV, false>::run(v);
+}
+
+template <int MODE, bool HIN, bool HOUT>
+__device__ __forceinline__ void cols_reg_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
+    static_assert(MODE == COL_FWD || MODE == COL_INV || MODE == COL_Z_ISO || MODE == COL_Z_PER, "mode");
+    constexpr int N = 256, h = N / 2;
+    constexpr bool ZP = MODE == COL_Z_ISO || MODE == COL_Z_PER;
+    __shared__ float2 s_x[16 * RLP];
+    __shared__ float2 s_tw[N];
+    s_tw[threadIdx.x] = tw[threadIdx.x];
+    const int c = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int kx0 = bx * 16 + c;
+    const bool okc = kx0 < a.nb;
+    float2 v[16];
+    {
+        const float2* in = a.in + (size_t)slab * a.inA + kx0;
+        const unsigned sN = (unsigned)a.inN;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int n = t + 16 * r;
+            v[r] = make_float2(0.f, 0.f);
+            if ((!HIN || r < 8) && okc && n < a.nin) v[r] = in[(unsigned)n * sN];
+        }
+    }
+    // fused z pass: the multiplier of output kz = t + 16 k1 (isolated: the
+    // Green octant at folded ky, kz; periodic: kPoissonPeriodic, Sim.cu:528-557).
+    // Packed column 0: slabs ky > N/2 hold kx = N/2.
+    float m[ZP ? 16 : 1];
+    const bool col0 = a.pack && kx0 == 0;
+    if constexpr (ZP) {
+        const int ky = slab, kyf = ky <= h ? ky : N - ky;
+        const int kxa = col0 && ky > h ? h : kx0;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const int kz = t + 16 * k1;
+            m[k1] = 0.f;
+            if (!okc) continue;
+            if (MODE == COL_Z_ISO) {
+                const int kzf = kz <= h ? kz : N - kz;
+                m[k1] = a.green[((unsigned)kzf * (h + 1) + kyf) * (unsigned)a.P + kxa];
+            } else {
+                m[k1] = z_mult<COL_Z_PER>(a, N, kxa, ky, kz);
+            }
+        }
+    }
+    __syncthreads();  // s_tw
+    const bool packTile = a.pack && bx == 0;  // block-uniform
+    // packed column 0 of the inverse y pass: rebuild U = W0 + i WN (slot k < N/2
+    // holds W0(k), slot N-k holds WN(k); W(N-k) = conj W(k) for both)
+    if (MODE == COL_INV && packTile) {
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s_x[t + 16 * r] = v[r];
+        }
+        __syncthreads();
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int k = t + 16 * r;
+                if (k == 0 || k == h) continue;
+                const float2 o = v[r], p = s_x[N - k];
+                v[r] = k < h ? make_float2(o.x - p.y, o.y + p.x) : make_float2(p.x + o.y, o.x - p.y);
+            }
+        }
+    }
+    fft256_regs<MODE == COL_INV, HIN && MODE != COL_INV>(v, s_x, s_tw, c, t, MODE == COL_INV && packTile);
+    // packed column 0 after the forward y transform: Z = F0 + i FN of two real
+    // columns; F0(k) = (Z(k) + conj Z(N-k)) / 2 stays in slot k, FN(k) =
+    // (Z(k) - conj Z(N-k)) / 2i goes to slot N-k (0 < k < N/2)
+    if (MODE == COL_FWD && packTile) {
+        __syncthreads();
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s_x[t + 16 * r] = v[r];
+        }
+        __syncthreads();
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int k = t + 16 * r;
+                if (k == 0 || k == h) continue;
+                const float2 o = v[r], p = s_x[N - k];
+                v[r] = k < h ? make_float2(0.5f * (o.x + p.x), 0.5f * (o.y - p.y))
+                             : make_float2(0.5f * (p.y + o.y), 0.5f * (o.x - p.x));
+            }
+        }
+    }
+    if constexpr (ZP) {
+        // ky = 0 and N/2 of packed column 0: the line is Z = a + i b of two real
+        // z-sequences (kx = 0 and kx = N/2) with multipliers m0 and mN, both even
+        // in kz: W(kz) = Z(kz) (m0 + mN) / 2 + conj Z(N - kz) (m0 - mN) / 2
+        // (real and imaginary parts apart at kz = 0, N/2)
+        const bool both = packTile && (slab == 0 || slab == h);  // block-uniform
+        if (!(both && c == 0)) {
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) v[k1] = make_float2(v[k1].x * m[k1], v[k1].y * m[k1]);
+        }
+        if (both) {
+            __syncthreads();
+            if (c == 0) {
+#pragma unroll
+                for (int k1 = 0; k1 < 16; ++k1) s_x[t + 16 * k1] = v[k1];
+            }
+            __syncthreads();
+            if (c == 0) {
+#pragma unroll
+                for (int k1 = 0; k1 < 16; ++k1) {
+                    const int kz = t + 16 * k1;
+                    const float mz0 = z_mult<MODE>(a, N, 0, slab, kz), mzn = z_mult<MODE>(a, N, h, slab, kz);
+                    const float2 o = v[k1];
+                    if (kz == 0 || kz == h) {
+                        v[k1] = make_float2(o.x * mz0, o.y * mzn);
+                    } else {
+                        const float2 p = s_x[N - kz];
+                        const float sp = 0.5f * (mz0 + mzn), sm = 0.5f * (mz0 - mzn);
+                        v[k1] = make_float2(o.x * sp + p.x * sm, o.y * sp - p.y * sm);
+                    }
+                }
+            }
+        }
+        fft256_regs<true, false>(v, s_x, s_tw, c, t, true);
+    }
+    float2* out = a.out + (size_t)slab * a.outA + kx0;
+    const unsigned oN = (unsigned)a.outN;
+#pragma unroll
+    for (int r = 0; r < (HOUT ? 8 : 16); ++r) {
+        const int k = t + 16 * r;
+        if (okc && k < a.nout) out[(unsigned)k * oN] = v[r];
+    }
+}
+
+template <int MODE, bool HIN, bool HOUT>
+__global__ void __launch_bounds__(256) k_cols_reg(ColArgs a, const float2* __restrict__ tw) {
+    if constexpr (MODE == COL_Z_ISO || MODE == COL_Z_PER) {
+        if (a.fin.partials) {  // one more row of blocks in front: block (0, 0) reduces, the rest exit
+            if (blockIdx.y == 0) {
+                if (blockIdx.x == 0) finalize_body<256>(a.fin.partials, a.fin.nparts, a.fin.mode, a.fin.pol, a.fin.sc);
+                return;
+            }
+            cols_reg_body<MODE, HIN, HOUT>(a, tw, blockIdx.x, blockIdx.y - 1);
+            return;
+        }
+    }
+    cols_reg_body<MODE, HIN, HOUT>(a, tw, blockIdx.x, blockIdx.y);
+}
+
+template <int MODE, bool HIN, bool HOUT>
+__global__ void __launch_bounds__(256) k_cols_reg_batch(const ColArgs* __restrict__ jobs,
+                                                        const float2* __restrict__ tw) {
+    const ColArgs a = jobs[blockIdx.z];
+    cols_reg_body<MODE, HIN, HOUT>(a, tw, blockIdx.x, blockIdx.y);
+}
+
+// ---- register-resident row passes at N = 256 -----------------------------------
+// The same four-step transform with lines along x: the 16 threads of a line are
+// 16 consecutive lanes (t = lane % 16), which read one row's consecutive values
+// and exchange through LDS inside their wavefront, so every exchange is ordered
+// by wavefront-scope fences (no block barrier). Element (k2, t) of a line sits
+// at k2 * 17 + t: the writes of one k2 and the reads of one t both fall in 16
+// distinct banks. Two real rows share a line (Z = a + i b): the forward pass
+// splits the two half spectra by Hermitian symmetry (one exchange of Z(N - k)),
+// the inverse pass builds Z(k) = A(k) + i B(k) from the stored half spectra
+// (Z(N - k) = conj A(k) + i conj B(k)) and keeps the real part for one row and
+// the imaginary part for the other.
+constexpr int XRLP = 16 * 17;  // LDS floats2 per line
+__device__ __forceinline__ int xr_pos(int k) { return (k >> 4) * 17 + (k & 15); }
+
+#ifndef SD_XREG_LINES_FWD
+#define SD_XREG_LINES_FWD 16
+#endif
+#ifndef SD_XREG_RPL_FWD
+#define SD_XREG_RPL_FWD 1
+#endif
+#ifndef SD_XREG_LINES_INV
+#define SD_XREG_LINES_INV 8
+#endif
+
+template <bool INV, bool ZIN>
+__device__ __forceinline__ void fft256_regs_wave(float2 v[16], float2* __restrict__ line,
+                                                 const float2* __restrict__ s_tw, int t) {
+    Dft<16, INV, ZIN>::run(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        float2 w = s_tw[t * k];
+        if (INV) w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+    line_sync<16>();  // earlier reads of this line are done
+#pragma unroll
+    for (int k = 0; k < 16; ++k) line[k * 17 + t] = v[k];
+    line_sync<16>();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = line[t * 17 + k];
+    Dft<16, INV, false>::run(v);
+}
+
+// x pass, fixed-point density rows -> packed half spectra (SRC 0 of rows_fwd_body).
+// RPL rows per line: 1 (imaginary part zero; Z(k), k <= N/2, is the row's
+// half spectrum) or 2 (split by Hermitian symmetry).
+template <int LINES, int RPL, bool HIN>
+__device__ __forceinline__ void rows_fwd_reg_body(const RowsFwdArgs& a, const float2* __restrict__ tw, int bx) {
+    constexpr int N = 256, h = N / 2, NR = HIN ? 8 : 16;
+    static_assert(RPL == 1 || RPL == 2, "rows per line");
+    __shared__ float2 s_x[LINES * XRLP];
+    __shared__ float2 s_tw[N];
+    for (int i = threadIdx.x; i < N; i += LINES * 16) s_tw[i] = tw[i];
+    const int t = threadIdx.x & 15, c = threadIdx.x >> 4;
+    const int ra = bx * (RPL * LINES) + RPL * c, rb = ra + 1;
+    const bool okA = ra < a.rows, okB = RPL == 2 && rb < a.rows;
+    long long qa[NR], qb[RPL == 2 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int x = t + 16 * r;
+        qa[r] = okA && x < a.nin ? a.src[(size_t)ra * a.nin + x] : 0ll;
+        if constexpr (RPL == 2) qb[r] = okB && x < a.nin ? a.src[(size_t)rb * a.nin + x] : 0ll;
+    }
+    if (a.clearSrc) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int x = t + 16 * r;
+            if (okA && x < a.nin) a.src[(size_t)ra * a.nin + x] = 0ll;
+            if (okB && x < a.nin) a.src[(size_t)rb * a.nin + x] = 0ll;
+        }
+    }
+    float2 v[16];
+    const float sc = 1.0f / (float)SD_FIX_ONE;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        v[r] = make_float2(0.f, 0.f);
+        if (r < NR) v[r] = make_float2((float)qa[r] * sc, RPL == 2 ? (float)qb[RPL == 2 ? r : 0] * sc : 0.f);
+    }
+    __syncthreads();  // s_tw
+    float2* line = s_x + c * XRLP;
+    fft256_regs_wave<false, HIN>(v, line, s_tw, t);
+    float2* oa = a.out + (size_t)ra * a.P;
+    float2* ob = a.out + (size_t)rb * a.P;
+    const float2 zh = v[8];  // Z(N/2) on thread 0
+    if constexpr (RPL == 1) {
+#pragma unroll
+        for (int k1 = 0; k1 <= 8; ++k1) {
+            const int k = t + 16 * k1;
+            if (k > h || (a.pack && k == h)) continue;
+            float2 z = v[k1];
+            if (a.pack && k == 0) z.y = zh.x;  // bins 0 and N/2, both real
+            if (okA) oa[k] = z;
+        }
+    } else {
+        // Z(k) = A(k) + i B(k): A = (Z(k) + conj Z(N-k)) / 2, B = -i (Z(k) - conj Z(N-k)) / 2
+        line_sync<16>();
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) line[xr_pos(t + 16 * k1)] = v[k1];
+        line_sync<16>();
+#pragma unroll
+        for (int k1 = 0; k1 <= 8; ++k1) {
+            const int k = t + 16 * k1;
+            if (k > h || (a.pack && k == h)) continue;
+            const float2 z = v[k1], m = line[xr_pos((N - k) & (N - 1))];
+            float2 A = make_float2(0.5f * (z.x + m.x), 0.5f * (z.y - m.y));
+            float2 B = make_float2(0.5f * (z.y + m.y), 0.5f * (m.x - z.x));
+            if (a.pack && k == 0) {  // bin N/2 of each row: the real and imaginary parts of Z(N/2)
+                A.y = zh.x;
+                B.y = zh.y;
+            }
+            if (okA) oa[k] = A;
+            if (okB) ob[k] = B;
+        }
+    }
+}
+
+template <int LINES, int RPL, bool HIN>
+__global__ void __launch_bounds__(LINES * 16) k_rows_fwd_reg(RowsFwdArgs a, const float2* __restrict__ tw) {
+    rows_fwd_reg_body<LINES, RPL, HIN>(a, tw, blockIdx.x);
+}
+template <int LINES, int RPL, bool HIN>
+__global__ void __launch_bounds__(LINES * 16) k_rows_fwd_reg_batch(const RowsFwdArgs* __restrict__ jobs,
+                                                                   const float2* __restrict__ tw) {
+    const RowsFwdArgs a = jobs[blockIdx.y];
+    rows_fwd_reg_body<LINES, RPL, HIN>(a, tw, blockIdx.x);
+}
+
+// x' pass, packed half spectra -> real rows (x < nout kept).
+template <int LINES, bool HOUT>
+__device__ __forceinline__ void rows_inv_reg_body(const RowsInvArgs& a, const float2* __restrict__ tw, int bx) {
+    constexpr int N = 256, h = N / 2;
+    __shared__ float2 s_x[LINES * XRLP];
+    __shared__ float2 s_tw[N];
+    for (int i = threadIdx.x; i < N; i += LINES * 16) s_tw[i] = tw[i];
+    const int t = threadIdx.x & 15, c = threadIdx.x >> 4;
+    const int ra = bx * (2 * LINES) + 2 * c, rb = ra + 1;
+    const bool okA = ra < a.rows, okB = rb < a.rows;
+    const float2* ia = a.in + (size_t)ra * a.P;
+    const float2* ib = a.in + (size_t)rb * a.P;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int k = t + 16 * r;
+        const int kk = k <= h ? k : N - k;
+        const int src = a.pack && kk == h ? 0 : kk;
+        float2 A = okA ? ia[src] : make_float2(0.f, 0.f);
+        float2 B = okB ? ib[src] : make_float2(0.f, 0.f);
+        if (kk == 0 || kk == h) {  // the real bins of a real row (packed: bin N/2 in the imaginary part of bin 0)
+            if (a.pack && kk == h) {
+                A.x = A.y;
+                B.x = B.y;
+            }
+            A.y = B.y = 0.f;
+        }
+        v[r] = k <= h ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();  // s_tw
+    fft256_regs_wave<true, false>(v, s_x + c * XRLP, s_tw, t);
+    float* oa = a.out + (size_t)ra * a.nout;
+    float* ob = a.out + (size_t)rb * a.nout;
+#pragma unroll
+    for (int r = 0; r < (HOUT ? 8 : 16); ++r) {
+        const int x = t + 16 * r;
+        if (x < a.nout) {
+            if (okA) oa[x] = v[r].x;
+            if (okB) ob[x] = v[r].y;
+        }
+    }
+}
+
+template <int LINES, bool HOUT>
+__global__ void __launch_bounds__(LINES * 16) k_rows_inv_reg(RowsInvArgs a, const float2* __restrict__ tw) {
+    rows_inv_reg_body<LINES, HOUT>(a, tw, blockIdx.x);
+}
+template <int LINES, bool HOUT>
+__global__ void __launch_bounds__(LINES * 16) k_rows_inv_reg_batch(const RowsInvArgs* __restrict__ jobs,
+                                                                   const float2* __restrict__ tw) {
+    const RowsInvArgs a = jobs[blockIdx.y];
+    rows_inv_reg_body<LINES, HOUT>(a, tw, blockIdx.x);
+}
+
 struct GradJob {
     const float* pot;
     float4* acc;
@@ -1022,8 +1392,40 @@ __global__ void __launch_bounds__(256) k_grid_accel_batch(const GradJob* __restr
 
 namespace sd {
 
+// Register-resident x passes at N = 256 (k_rows_fwd_reg, k_rows_inv_reg): the
+// density rows (not the Green build) and the inverse rows, pruned when the rows
+// hold G = S/2 values.
+#ifndef SD_ROWS_REG_FWD
+#define SD_ROWS_REG_FWD 1
+#endif
+#ifndef SD_ROWS_REG_INV
+#define SD_ROWS_REG_INV 1
+#endif
+constexpr int XREG_ROWS_FWD = SD_XREG_RPL_FWD * SD_XREG_LINES_FWD, XREG_ROWS_INV = 2 * SD_XREG_LINES_INV;
+static void launch_rows_fwd_reg(const RowsFwdArgs& a, const float2* tw, hipStream_t st) {
+    const dim3 grid((a.rows + XREG_ROWS_FWD - 1) / XREG_ROWS_FWD), block(SD_XREG_LINES_FWD * 16);
+    if (2 * a.nin <= 256)
+        hipLaunchKernelGGL((k_rows_fwd_reg<SD_XREG_LINES_FWD, SD_XREG_RPL_FWD, true>), grid, block, 0, st, a, tw);
+    else
+        hipLaunchKernelGGL((k_rows_fwd_reg<SD_XREG_LINES_FWD, SD_XREG_RPL_FWD, false>), grid, block, 0, st, a, tw);
+}
+static void launch_rows_inv_reg(const RowsInvArgs& a, const float2* tw, hipStream_t st) {
+    const dim3 grid((a.rows + XREG_ROWS_INV - 1) / XREG_ROWS_INV), block(SD_XREG_LINES_INV * 16);
+    if (2 * a.nout <= 256)
+        hipLaunchKernelGGL((k_rows_inv_reg<SD_XREG_LINES_INV, true>), grid, block, 0, st, a, tw);
+    else
+        hipLaunchKernelGGL((k_rows_inv_reg<SD_XREG_LINES_INV, false>), grid, block, 0, st, a, tw);
+}
+
 template <int N>
 static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hipStream_t st) {
+    if constexpr (N == 256 && SD_ROWS_REG_FWD) {
+        static_assert(!rows_fwd_carries_finalize<N>(), "the register x pass carries no reduction");
+        if (src == 0) {
+            launch_rows_fwd_reg(a, tw, st);
+            return;
+        }
+    }
     const int carry = a.fin.partials ? 1 : 0;  // only where rows_fwd_carries_finalize<N>()
     dim3 grid((a.rows + XFwd::ROWS - 1) / XFwd::ROWS + carry), block(XFwd::LINES * Plan<N>::T);
     if (src == 0)
@@ -1033,6 +1435,10 @@ static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hip
 }
 template <int N>
 static void launch_rows_inv(const RowsInvArgs& a, const float2* tw, hipStream_t st) {
+    if constexpr (N == 256 && SD_ROWS_REG_INV) {
+        launch_rows_inv_reg(a, tw, st);
+        return;
+    }
     dim3 grid((a.rows + XInv::ROWS - 1) / XInv::ROWS), block(XInv::LINES * Plan<N>::T);
     hipLaunchKernelGGL((k_rows_inv<N>), grid, block, 0, st, a, tw);
 }
@@ -1064,10 +1470,59 @@ static void launch_cols_stream(const ColArgs& a, int slabs, const float2* tw, hi
     hipLaunchKernelGGL((k_cols_stream<N, MODE>), dim3(grid), dim3(threads), 0, st, a, tw, colBlocks, ntiles);
 }
 
+// Register-resident column passes at N = 256 (k_cols_reg), with the pruned
+// forms the zero padding allows: forward transforms of G = S/2 input rows
+// (y forward, fused z) and inverse transforms of which only G output rows are
+// stored (fused z, y inverse). The unpruned form serves every other shape.
+#ifndef SD_COLS_REG
+#define SD_COLS_REG 1
+#endif
+// Which pruned form a pass takes: forward transforms prune their input
+// (y forward, fused z), inverse ones their output (fused z, y inverse); the
+// fused z pass is instantiated pruned on both sides or on neither.
+template <int MODE>
+static bool cols_reg_pruned(int nin, int nout) {
+    if (MODE == COL_FWD) return 2 * nin <= 256;
+    if (MODE == COL_INV) return 2 * nout <= 256;
+    return 2 * nin <= 256 && 2 * nout <= 256;
+}
+template <int MODE, bool P>
+struct ColsRegForm {
+    static constexpr bool HIN = P && MODE != COL_INV;
+    static constexpr bool HOUT = P && MODE != COL_FWD;
+};
+template <int MODE>
+static void launch_cols_reg(const ColArgs& a, dim3 grid, const float2* tw, hipStream_t st) {
+    using Pr = ColsRegForm<MODE, true>;
+    if (cols_reg_pruned<MODE>(a.nin, a.nout))
+        hipLaunchKernelGGL((k_cols_reg<MODE, Pr::HIN, Pr::HOUT>), grid, dim3(256), 0, st, a, tw);
+    else
+        hipLaunchKernelGGL((k_cols_reg<MODE, false, false>), grid, dim3(256), 0, st, a, tw);
+}
+template <int MODE>
+static void launch_cols_reg_batch(const ColArgs* jobs, int nin, int nout, dim3 grid, const float2* tw,
+                                  hipStream_t st) {
+    using Pr = ColsRegForm<MODE, true>;
+    if (cols_reg_pruned<MODE>(nin, nout))
+        hipLaunchKernelGGL((k_cols_reg_batch<MODE, Pr::HIN, Pr::HOUT>), grid, dim3(256), 0, st, jobs, tw);
+    else
+        hipLaunchKernelGGL((k_cols_reg_batch<MODE, false, false>), grid, dim3(256), 0, st, jobs, tw);
+}
+
 template <int N>
 static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw, hipStream_t st) {
     // a carried reduction (fused z pass, no persistent form) adds a row of blocks
     dim3 grid((a.nb + TB - 1) / TB, slabs + (a.fin.partials ? 1 : 0)), block(TB * Plan<N>::T);
+    if constexpr (N == 256 && SD_COLS_REG) {
+        static_assert(TB == 16, "register-resident tiles are 16 columns wide");
+        switch (mode) {
+            case COL_FWD: launch_cols_reg<COL_FWD>(a, grid, tw, st); return;
+            case COL_INV: launch_cols_reg<COL_INV>(a, grid, tw, st); return;
+            case COL_Z_ISO: launch_cols_reg<COL_Z_ISO>(a, grid, tw, st); return;
+            case COL_Z_PER: launch_cols_reg<COL_Z_PER>(a, grid, tw, st); return;
+            default: break;  // the Green build keeps the staged kernel
+        }
+    }
     if constexpr (N >= SD_COLS_STREAM_MIN_N) {
         switch (mode) {
             case COL_FWD: launch_cols_stream<N, COL_FWD>(a, slabs, tw, st); return;
@@ -1333,14 +1788,40 @@ static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
     const dim3 blk(TB * Plan<N>::T), fblk(XFwd::LINES * Plan<N>::T), iblk(XInv::LINES * Plan<N>::T);
     const int NK = g.S / 2 + 1 - (fft_packs(g.S) ? 1 : 0), colBlocks = (NK + TB - 1) / TB;
     const int fBlocks = (g.G * g.G + XFwd::ROWS - 1) / XFwd::ROWS, iBlocks = (g.G * g.G + XInv::ROWS - 1) / XInv::ROWS;
-    hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(fBlocks, g.n), fblk, 0, st, g.xf, g.tw);
-    hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
-    if (g.periodic)
-        hipLaunchKernelGGL((k_cols_batch<N, COL_Z_PER>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
-    else
-        hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
-    hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
-    hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(iBlocks, g.n), iblk, 0, st, g.xi, g.tw);
+    if constexpr (N == 256 && SD_ROWS_REG_FWD) {
+        const dim3 grid((g.G * g.G + XREG_ROWS_FWD - 1) / XREG_ROWS_FWD, g.n), block(SD_XREG_LINES_FWD * 16);
+        if (2 * g.G <= 256)
+            hipLaunchKernelGGL((k_rows_fwd_reg_batch<SD_XREG_LINES_FWD, SD_XREG_RPL_FWD, true>), grid, block, 0, st, g.xf, g.tw);
+        else
+            hipLaunchKernelGGL((k_rows_fwd_reg_batch<SD_XREG_LINES_FWD, SD_XREG_RPL_FWD, false>), grid, block, 0, st, g.xf, g.tw);
+    } else {
+        hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(fBlocks, g.n), fblk, 0, st, g.xf, g.tw);
+    }
+    if constexpr (N == 256 && SD_COLS_REG) {
+        // the single-mesh solve's kernels and pruned forms (nin / nout as there)
+        launch_cols_reg_batch<COL_FWD>(g.yf, g.G, g.S, dim3(colBlocks, g.G, g.n), g.tw, st);
+        if (g.periodic)
+            launch_cols_reg_batch<COL_Z_PER>(g.z, g.G, g.G, dim3(colBlocks, g.S, g.n), g.tw, st);
+        else
+            launch_cols_reg_batch<COL_Z_ISO>(g.z, g.G, g.G, dim3(colBlocks, g.S, g.n), g.tw, st);
+        launch_cols_reg_batch<COL_INV>(g.yi, g.S, g.G, dim3(colBlocks, g.G, g.n), g.tw, st);
+    } else {
+        hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
+        if (g.periodic)
+            hipLaunchKernelGGL((k_cols_batch<N, COL_Z_PER>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
+        else
+            hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
+        hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
+    }
+    if constexpr (N == 256 && SD_ROWS_REG_INV) {
+        const dim3 grid((g.G * g.G + XREG_ROWS_INV - 1) / XREG_ROWS_INV, g.n), block(SD_XREG_LINES_INV * 16);
+        if (2 * g.G <= 256)
+            hipLaunchKernelGGL((k_rows_inv_reg_batch<SD_XREG_LINES_INV, true>), grid, block, 0, st, g.xi, g.tw);
+        else
+            hipLaunchKernelGGL((k_rows_inv_reg_batch<SD_XREG_LINES_INV, false>), grid, block, 0, st, g.xi, g.tw);
+    } else {
+        hipLaunchKernelGGL((k_rows_inv_batch<N>), dim3(iBlocks, g.n), iblk, 0, st, g.xi, g.tw);
+    }
     if (!grad) return;
     const long long cells = (long long)g.G * g.G * g.G;
     hipLaunchKernelGGL(k_grid_accel_batch, dim3((unsigned)((cells + 255) / 256), g.n), dim3(256), 0, st, g.grad);
