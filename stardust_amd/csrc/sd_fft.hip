@@ -1006,6 +1006,16 @@ __global__ void __launch_bounds__(TB * Plan<N>::T) k_cols_batch(const ColArgs* _
 // HIN: inputs n >= 128 are zero padding (the first transform over r skips
 // them); HOUT: only outputs < 128 are stored (the last transform over t of the
 // inverse computes only those).
+// Element access by a 32-bit element offset from a uniform base (the load or
+// store adds the offset to the scalar base itself).
+template <typename T>
+__device__ __forceinline__ T ld_c(const T* base, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_c(T* base, unsigned i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T)) = v;
+}
 constexpr int RLP = 257;  // LDS pitch of one line (float2): odd, so the 16 lines of a row fall in distinct banks
 
 template <bool INV, bool ZIN>
@@ -1039,15 +1049,19 @@ __device__ __forceinline__ void cols_reg_body(const ColArgs& a, const float2* __
     const int c = threadIdx.x & 15, t = threadIdx.x >> 4;
     const int kx0 = bx * 16 + c;
     const bool okc = kx0 < a.nb;
+    // element offsets (< 2^24 at N = 256: 24-bit multiplies) from the uniform
+    // base; a partial tile's dead columns load a live one and are dropped
+    const int kxl = okc ? kx0 : a.nb - 1;
     float2 v[16];
     {
-        const float2* in = a.in + (size_t)slab * a.inA + kx0;
-        const unsigned sN = (unsigned)a.inN;
+        const unsigned off = __umul24((unsigned)slab, (unsigned)a.inA) + (unsigned)kxl +
+                             __umul24((unsigned)t, (unsigned)a.inN),
+                       step = (unsigned)a.inN << 4;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int n = t + 16 * r;
-            v[r] = make_float2(0.f, 0.f);
-            if ((!HIN || r < 8) && okc && n < a.nin) v[r] = in[(unsigned)n * sN];
+        for (int r = 0; r < 16; ++r) v[r] = r < (HIN ? 8 : 16) ? ld_c(a.in, off + r * step) : make_float2(0.f, 0.f);
+        if (!okc) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = make_float2(0.f, 0.f);
         }
     }
     // fused z pass: the multiplier of output kz = t + 16 k1 (isolated: the
@@ -1057,15 +1071,15 @@ __device__ __forceinline__ void cols_reg_body(const ColArgs& a, const float2* __
     const bool col0 = a.pack && kx0 == 0;
     if constexpr (ZP) {
         const int ky = slab, kyf = ky <= h ? ky : N - ky;
-        const int kxa = col0 && ky > h ? h : kx0;
+        const int kxa = col0 && ky > h ? h : kxl;
+        const unsigned P = (unsigned)a.P, gbase = __umul24((unsigned)kyf, P) + (unsigned)kxa,
+                       gplane = __umul24((unsigned)(h + 1), P);
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) {
             const int kz = t + 16 * k1;
-            m[k1] = 0.f;
-            if (!okc) continue;
             if (MODE == COL_Z_ISO) {
                 const int kzf = kz <= h ? kz : N - kz;
-                m[k1] = a.green[((unsigned)kzf * (h + 1) + kyf) * (unsigned)a.P + kxa];
+                m[k1] = ld_c(a.green, gbase + __umul24((unsigned)kzf, gplane));
             } else {
                 m[k1] = z_mult<COL_Z_PER>(a, N, kxa, ky, kz);
             }
@@ -1148,12 +1162,12 @@ __device__ __forceinline__ void cols_reg_body(const ColArgs& a, const float2* __
         }
         fft256_regs<true, false>(v, s_x, s_tw, c, t, true);
     }
-    float2* out = a.out + (size_t)slab * a.outA + kx0;
-    const unsigned oN = (unsigned)a.outN;
+    if (okc) {
+        const unsigned off = __umul24((unsigned)slab, (unsigned)a.outA) + (unsigned)kx0 +
+                             __umul24((unsigned)t, (unsigned)a.outN),
+                       step = (unsigned)a.outN << 4;
 #pragma unroll
-    for (int r = 0; r < (HOUT ? 8 : 16); ++r) {
-        const int k = t + 16 * r;
-        if (okc && k < a.nout) out[(unsigned)k * oN] = v[r];
+        for (int r = 0; r < (HOUT ? 8 : 16); ++r) st_c(a.out, off + r * step, v[r]);
     }
 }
 
@@ -1233,21 +1247,21 @@ __device__ __forceinline__ void rows_fwd_reg_body(const RowsFwdArgs& a, const fl
     __shared__ float2 s_tw[N];
     for (int i = threadIdx.x; i < N; i += LINES * 16) s_tw[i] = tw[i];
     const int t = threadIdx.x & 15, c = threadIdx.x >> 4;
-    const int ra = bx * (RPL * LINES) + RPL * c, rb = ra + 1;
-    const bool okA = ra < a.rows, okB = RPL == 2 && rb < a.rows;
+    const int ra = bx * (RPL * LINES) + RPL * c;
+    // whole blocks of rows (the launcher checks), each nin = N/2 (HIN) or N
+    // values; element offsets < 2^24
+    const unsigned ia = __umul24((unsigned)ra, (unsigned)a.nin) + (unsigned)t, ib = ia + (unsigned)a.nin;
     long long qa[NR], qb[RPL == 2 ? NR : 1];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        const int x = t + 16 * r;
-        qa[r] = okA && x < a.nin ? a.src[(size_t)ra * a.nin + x] : 0ll;
-        if constexpr (RPL == 2) qb[r] = okB && x < a.nin ? a.src[(size_t)rb * a.nin + x] : 0ll;
+        qa[r] = ld_c(a.src, ia + 16u * r);
+        if constexpr (RPL == 2) qb[r] = ld_c(a.src, ib + 16u * r);
     }
     if (a.clearSrc) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const int x = t + 16 * r;
-            if (okA && x < a.nin) a.src[(size_t)ra * a.nin + x] = 0ll;
-            if (okB && x < a.nin) a.src[(size_t)rb * a.nin + x] = 0ll;
+            st_c(a.src, ia + 16u * r, 0ll);
+            if constexpr (RPL == 2) st_c(a.src, ib + 16u * r, 0ll);
         }
     }
     float2 v[16];
@@ -1260,8 +1274,7 @@ __device__ __forceinline__ void rows_fwd_reg_body(const RowsFwdArgs& a, const fl
     __syncthreads();  // s_tw
     float2* line = s_x + c * XRLP;
     fft256_regs_wave<false, HIN>(v, line, s_tw, t);
-    float2* oa = a.out + (size_t)ra * a.P;
-    float2* ob = a.out + (size_t)rb * a.P;
+    const unsigned oa = __umul24((unsigned)ra, (unsigned)a.P), ob = oa + (unsigned)a.P;
     const float2 zh = v[8];  // Z(N/2) on thread 0
     if constexpr (RPL == 1) {
 #pragma unroll
@@ -1270,7 +1283,7 @@ __device__ __forceinline__ void rows_fwd_reg_body(const RowsFwdArgs& a, const fl
             if (k > h || (a.pack && k == h)) continue;
             float2 z = v[k1];
             if (a.pack && k == 0) z.y = zh.x;  // bins 0 and N/2, both real
-            if (okA) oa[k] = z;
+            st_c(a.out, oa + k, z);
         }
     } else {
         // Z(k) = A(k) + i B(k): A = (Z(k) + conj Z(N-k)) / 2, B = -i (Z(k) - conj Z(N-k)) / 2
@@ -1289,8 +1302,8 @@ __device__ __forceinline__ void rows_fwd_reg_body(const RowsFwdArgs& a, const fl
                 A.y = zh.x;
                 B.y = zh.y;
             }
-            if (okA) oa[k] = A;
-            if (okB) ob[k] = B;
+            st_c(a.out, oa + k, A);
+            st_c(a.out, ob + k, B);
         }
     }
 }
@@ -1314,18 +1327,16 @@ __device__ __forceinline__ void rows_inv_reg_body(const RowsInvArgs& a, const fl
     __shared__ float2 s_tw[N];
     for (int i = threadIdx.x; i < N; i += LINES * 16) s_tw[i] = tw[i];
     const int t = threadIdx.x & 15, c = threadIdx.x >> 4;
-    const int ra = bx * (2 * LINES) + 2 * c, rb = ra + 1;
-    const bool okA = ra < a.rows, okB = rb < a.rows;
-    const float2* ia = a.in + (size_t)ra * a.P;
-    const float2* ib = a.in + (size_t)rb * a.P;
+    const int ra = bx * (2 * LINES) + 2 * c;
+    // whole blocks of rows (the launcher checks); element offsets < 2^24
+    const unsigned ia = __umul24((unsigned)ra, (unsigned)a.P), ib = ia + (unsigned)a.P;
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int k = t + 16 * r;
         const int kk = k <= h ? k : N - k;
-        const int src = a.pack && kk == h ? 0 : kk;
-        float2 A = okA ? ia[src] : make_float2(0.f, 0.f);
-        float2 B = okB ? ib[src] : make_float2(0.f, 0.f);
+        const unsigned src = a.pack && kk == h ? 0u : (unsigned)kk;
+        float2 A = ld_c(a.in, ia + src), B = ld_c(a.in, ib + src);
         if (kk == 0 || kk == h) {  // the real bins of a real row (packed: bin N/2 in the imaginary part of bin 0)
             if (a.pack && kk == h) {
                 A.x = A.y;
@@ -1337,15 +1348,11 @@ __device__ __forceinline__ void rows_inv_reg_body(const RowsInvArgs& a, const fl
     }
     __syncthreads();  // s_tw
     fft256_regs_wave<true, false>(v, s_x + c * XRLP, s_tw, t);
-    float* oa = a.out + (size_t)ra * a.nout;
-    float* ob = a.out + (size_t)rb * a.nout;
+    const unsigned oa = __umul24((unsigned)ra, (unsigned)a.nout) + (unsigned)t, ob = oa + (unsigned)a.nout;
 #pragma unroll
     for (int r = 0; r < (HOUT ? 8 : 16); ++r) {
-        const int x = t + 16 * r;
-        if (x < a.nout) {
-            if (okA) oa[x] = v[r].x;
-            if (okB) ob[x] = v[r].y;
-        }
+        st_c(a.out, oa + 16u * r, v[r].x);
+        st_c(a.out, ob + 16u * r, v[r].y);
     }
 }
 
@@ -1402,6 +1409,13 @@ namespace sd {
 #define SD_ROWS_REG_INV 1
 #endif
 constexpr int XREG_ROWS_FWD = SD_XREG_RPL_FWD * SD_XREG_LINES_FWD, XREG_ROWS_INV = 2 * SD_XREG_LINES_INV;
+// The register x passes take whole blocks of rows of N/2 (pruned) or N values.
+static bool rows_fwd_reg_fits(int rows, int nin) {
+    return rows % XREG_ROWS_FWD == 0 && (nin == 128 || nin == 256);
+}
+static bool rows_inv_reg_fits(int rows, int nout) {
+    return rows % XREG_ROWS_INV == 0 && (nout == 128 || nout == 256);
+}
 static void launch_rows_fwd_reg(const RowsFwdArgs& a, const float2* tw, hipStream_t st) {
     const dim3 grid((a.rows + XREG_ROWS_FWD - 1) / XREG_ROWS_FWD), block(SD_XREG_LINES_FWD * 16);
     if (2 * a.nin <= 256)
@@ -1421,7 +1435,7 @@ template <int N>
 static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hipStream_t st) {
     if constexpr (N == 256 && SD_ROWS_REG_FWD) {
         static_assert(!rows_fwd_carries_finalize<N>(), "the register x pass carries no reduction");
-        if (src == 0) {
+        if (src == 0 && rows_fwd_reg_fits(a.rows, a.nin)) {
             launch_rows_fwd_reg(a, tw, st);
             return;
         }
@@ -1436,8 +1450,10 @@ static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hip
 template <int N>
 static void launch_rows_inv(const RowsInvArgs& a, const float2* tw, hipStream_t st) {
     if constexpr (N == 256 && SD_ROWS_REG_INV) {
-        launch_rows_inv_reg(a, tw, st);
-        return;
+        if (rows_inv_reg_fits(a.rows, a.nout)) {
+            launch_rows_inv_reg(a, tw, st);
+            return;
+        }
     }
     dim3 grid((a.rows + XInv::ROWS - 1) / XInv::ROWS), block(XInv::LINES * Plan<N>::T);
     hipLaunchKernelGGL((k_rows_inv<N>), grid, block, 0, st, a, tw);
@@ -1491,6 +1507,14 @@ struct ColsRegForm {
     static constexpr bool HIN = P && MODE != COL_INV;
     static constexpr bool HOUT = P && MODE != COL_FWD;
 };
+// The register forms load whole lines of nin = N/2 (pruned) or N rows and
+// store nout = N/2 (pruned) or N rows, with no per-row bounds.
+template <int MODE>
+static bool cols_reg_fits(int nin, int nout) {
+    const bool pr = cols_reg_pruned<MODE>(nin, nout);
+    const bool hin = pr && MODE != COL_INV, hout = pr && MODE != COL_FWD;
+    return nin == (hin ? 128 : 256) && nout == (hout ? 128 : 256);
+}
 template <int MODE>
 static void launch_cols_reg(const ColArgs& a, dim3 grid, const float2* tw, hipStream_t st) {
     using Pr = ColsRegForm<MODE, true>;
@@ -1516,10 +1540,18 @@ static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw,
     if constexpr (N == 256 && SD_COLS_REG) {
         static_assert(TB == 16, "register-resident tiles are 16 columns wide");
         switch (mode) {
-            case COL_FWD: launch_cols_reg<COL_FWD>(a, grid, tw, st); return;
-            case COL_INV: launch_cols_reg<COL_INV>(a, grid, tw, st); return;
-            case COL_Z_ISO: launch_cols_reg<COL_Z_ISO>(a, grid, tw, st); return;
-            case COL_Z_PER: launch_cols_reg<COL_Z_PER>(a, grid, tw, st); return;
+#define SD_REG_CASE(M)                                                      \
+    case M:                                                                 \
+        if (cols_reg_fits<M>(a.nin, a.nout)) {                              \
+            launch_cols_reg<M>(a, grid, tw, st);                            \
+            return;                                                         \
+        }                                                                   \
+        break;
+            SD_REG_CASE(COL_FWD)
+            SD_REG_CASE(COL_INV)
+            SD_REG_CASE(COL_Z_ISO)
+            SD_REG_CASE(COL_Z_PER)
+#undef SD_REG_CASE
             default: break;  // the Green build keeps the staged kernel
         }
     }
@@ -1788,7 +1820,7 @@ static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
     const dim3 blk(TB * Plan<N>::T), fblk(XFwd::LINES * Plan<N>::T), iblk(XInv::LINES * Plan<N>::T);
     const int NK = g.S / 2 + 1 - (fft_packs(g.S) ? 1 : 0), colBlocks = (NK + TB - 1) / TB;
     const int fBlocks = (g.G * g.G + XFwd::ROWS - 1) / XFwd::ROWS, iBlocks = (g.G * g.G + XInv::ROWS - 1) / XInv::ROWS;
-    if constexpr (N == 256 && SD_ROWS_REG_FWD) {
+    if (N == 256 && SD_ROWS_REG_FWD && rows_fwd_reg_fits(g.G * g.G, g.G)) {
         const dim3 grid((g.G * g.G + XREG_ROWS_FWD - 1) / XREG_ROWS_FWD, g.n), block(SD_XREG_LINES_FWD * 16);
         if (2 * g.G <= 256)
             hipLaunchKernelGGL((k_rows_fwd_reg_batch<SD_XREG_LINES_FWD, SD_XREG_RPL_FWD, true>), grid, block, 0, st, g.xf, g.tw);
@@ -1797,7 +1829,11 @@ static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
     } else {
         hipLaunchKernelGGL((k_rows_fwd_batch<N>), dim3(fBlocks, g.n), fblk, 0, st, g.xf, g.tw);
     }
-    if constexpr (N == 256 && SD_COLS_REG) {
+    bool regFits = true;
+    if constexpr (N == 256 && SD_COLS_REG)
+        regFits = cols_reg_fits<COL_FWD>(g.G, g.S) && cols_reg_fits<COL_Z_ISO>(g.G, g.G) &&
+                  cols_reg_fits<COL_INV>(g.S, g.G);
+    if (N == 256 && SD_COLS_REG && regFits) {
         // the single-mesh solve's kernels and pruned forms (nin / nout as there)
         launch_cols_reg_batch<COL_FWD>(g.yf, g.G, g.S, dim3(colBlocks, g.G, g.n), g.tw, st);
         if (g.periodic)
@@ -1813,7 +1849,7 @@ static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
             hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
         hipLaunchKernelGGL((k_cols_batch<N, COL_INV>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yi, g.tw);
     }
-    if constexpr (N == 256 && SD_ROWS_REG_INV) {
+    if (N == 256 && SD_ROWS_REG_INV && rows_inv_reg_fits(g.G * g.G, g.G)) {
         const dim3 grid((g.G * g.G + XREG_ROWS_INV - 1) / XREG_ROWS_INV, g.n), block(SD_XREG_LINES_INV * 16);
         if (2 * g.G <= 256)
             hipLaunchKernelGGL((k_rows_inv_reg_batch<SD_XREG_LINES_INV, true>), grid, block, 0, st, g.xi, g.tw);
