@@ -209,6 +209,9 @@ __device__ __forceinline__ float wave_max63(float v) {  // v >= 0
 // The previous lane's value (DPP wave_shr 1; lane 0 gets `first`).
 __device__ __forceinline__ int lane_prev(int first, int v) { return dpp_or<0x138, 0xF>(first, v); }
 
+#ifndef SD_RUN_SCAN_PREFIX
+#define SD_RUN_SCAN_PREFIX 1
+#endif
 // Segmented inclusive scan over the wavefront; returns true on the lane that
 // holds a run's total in v[8]. A run lies inside one wavefront, so its sums
 // are <= 64 * 2^24 = 2^30 and fit 32 bits. The scan is Hillis-Steele on DPP:
@@ -234,6 +237,30 @@ __device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, uint32_t v[8
     const uint64_t le = lane == 63 ? headMask : (headMask & ((2ull << lane) - 1ull));
     const int start = 63 - __clzll(le);
     const bool tail = lane == 63 || ((headMask >> (lane + 1)) & 1ull);
+#if SD_RUN_SCAN_PREFIX
+    // a run's total is P(tail) - P(start - 1) for the wavefront's inclusive
+    // prefix sums P (unsegmented: one DPP add per value and step); P(start - 1)
+    // = P(start) - v(start) comes from the run's first lane. Integer sums, so
+    // the totals are exact and equal the segmented scan's.
+    uint32_t P[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = v[k];
+#define SD_PSTEP(CTRL, ROWMASK, BC)                                                                 \
+    _Pragma("unroll") for (int k = 0; k < 8; ++k) P[k] +=                                          \
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)P[k], CTRL, ROWMASK, 0xF, BC);
+    SD_PSTEP(0x111, 0xF, true)
+    SD_PSTEP(0x112, 0xF, true)
+    SD_PSTEP(0x114, 0xF, true)
+    SD_PSTEP(0x118, 0xF, true)
+    SD_PSTEP(0x142, 0xA, false)
+    SD_PSTEP(0x143, 0xC, false)
+#undef SD_PSTEP
+    const int src = start << 2;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        v[k] = P[k] - (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(P[k] - v[k]));
+    return alive && tail;
+#endif
 #define SD_DPP_STEP(CTRL, ROWMASK, TAKE)                          \
     {                                                             \
         const bool take_ = (TAKE);                                \
@@ -332,16 +359,21 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
     const int nv = (int)vol, ndx = (int)dx, ndy = (int)dy;
     for (int t = threadIdx.x; t < nv; t += PT) s_tile[t] = 0ull;
     __syncthreads();
+    // corner q of a run's base cell sits at base + off(q): the tile spans
+    // fewer than 2^10 cells, so 24-bit multiplies give the base, and the
+    // offsets are block-uniform; a zero weight adds nothing and is not skipped
+    const unsigned txy = __umul24((unsigned)ndx, (unsigned)ndy);
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
         uint32_t v[8];
         if (wave_runs(alive[k], c[k], v)) {
-            const int bx = c[k].ix - lo[0], by = c[k].iy - lo[1], bz = c[k].iz - lo[2];
+            const unsigned bx = (unsigned)(c[k].ix - lo[0]), by = (unsigned)(c[k].iy - lo[1]),
+                           bz = (unsigned)(c[k].iz - lo[2]);
+            unsigned long long* t0 = s_tile + (__umul24(__umul24(bz, (unsigned)ndy) + by, (unsigned)ndx) + bx);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int ox = q & 1, oy = (q >> 1) & 1, oz = (q >> 2) & 1;
-                if (v[q] != 0u)
-                    atomicAdd(&s_tile[((bz + oz) * ndy + (by + oy)) * ndx + (bx + ox)], (unsigned long long)v[q]);
+                const unsigned off = ((q & 1) ? 1u : 0u) + (((q >> 1) & 1) ? (unsigned)ndx : 0u) + (((q >> 2) & 1) ? txy : 0u);
+                atomicAdd(t0 + off, (unsigned long long)v[q]);
             }
         }
     }
