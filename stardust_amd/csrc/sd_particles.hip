@@ -181,33 +181,45 @@ __device__ __forceinline__ int dpp_or(int identity, int v) {
     return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWMASK, 0xF, false);
 }
 #define SD_WAVE63(STEP) STEP(0xB1, 0xF) STEP(0x4E, 0xF) STEP(0x141, 0xF) STEP(0x140, 0xF) STEP(0x142, 0xA) STEP(0x143, 0xC)
+// Only lane 63's result is used, and it never depends on a lane that a
+// broadcast left unwritten (row 3 is written by both broadcasts, and each
+// reads the last lane of a row that holds a complete partial). So the
+// unwritten lanes take whatever the register holds: no identity fill.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_self(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_any(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, ROWMASK, 0xF, false);
+}
 __device__ __forceinline__ int wave_min63(int v) {
-#define SD_STEP(C, M) v = min(v, dpp_or<C, M>(INT_MAX, v));
+#define SD_STEP(C, M) v = min(v, dpp_any<C, M>(v));
     SD_WAVE63(SD_STEP)
 #undef SD_STEP
     return v;
 }
 __device__ __forceinline__ int wave_max63(int v) {
-#define SD_STEP(C, M) v = max(v, dpp_or<C, M>(INT_MIN, v));
+#define SD_STEP(C, M) v = max(v, dpp_any<C, M>(v));
     SD_WAVE63(SD_STEP)
 #undef SD_STEP
     return v;
 }
 __device__ __forceinline__ double wave_sum63(double v) {
-#define SD_STEP(C, M)                                                                              \
-    v += __hiloint2double(dpp_or<C, M>(0, __double2hiint(v)), dpp_or<C, M>(0, __double2loint(v)));
+#define SD_STEP(C, M) v += __hiloint2double(dpp_any<C, M>(__double2hiint(v)), dpp_any<C, M>(__double2loint(v)));
     SD_WAVE63(SD_STEP)
 #undef SD_STEP
     return v;
 }
-__device__ __forceinline__ float wave_max63(float v) {  // v >= 0
-#define SD_STEP(C, M) v = fmaxf(v, __int_as_float(dpp_or<C, M>(0, __float_as_int(v))));
-    SD_WAVE63(SD_STEP)
-#undef SD_STEP
-    return v;
+// v >= 0 and finite: the IEEE order of such values is the integer order of
+// their bits, so the maximum is an integer maximum (no NaN canonicalisation)
+__device__ __forceinline__ float wave_max63(float v) {
+    return __int_as_float(wave_max63(__float_as_int(v)));
 }
 // The previous lane's value (DPP wave_shr 1; lane 0 gets `first`).
 __device__ __forceinline__ int lane_prev(int first, int v) { return dpp_or<0x138, 0xF>(first, v); }
+// The same where lane 0's result is not used (it gets its own value).
+__device__ __forceinline__ int lane_prev_any(int v) { return dpp_self<0x138, 0xF>(v); }
 
 #ifndef SD_RUN_SCAN_PREFIX
 #define SD_RUN_SCAN_PREFIX 1
@@ -228,7 +240,7 @@ __device__ __forceinline__ bool wave_runs(bool alive, const Cic& c, uint32_t v[8
     }
     const int ix = alive ? c.ix : INT_MIN, iy = alive ? c.iy : INT_MIN, iz = alive ? c.iz : INT_MIN;
 #if SD_DPP_REDUCE
-    const int pix = lane_prev(ix, ix), piy = lane_prev(iy, iy), piz = lane_prev(iz, iz);
+    const int pix = lane_prev_any(ix), piy = lane_prev_any(iy), piz = lane_prev_any(iz);  // lane 0 is a head
 #else
     const int pix = __shfl_up(ix, 1, 64), piy = __shfl_up(iy, 1, 64), piz = __shfl_up(iz, 1, 64);
 #endif
@@ -863,6 +875,7 @@ __device__ __forceinline__ void block_partial(double sx, double sy, double sz, d
                                               ReducePartial* out, int blk) {
     __shared__ double s_d[PT / 64][4];
     __shared__ float s_f[PT / 64];
+    vmax = vmax >= 0.f ? vmax : 0.f;  // a NaN speed counts as 0, as fmaxf would drop it
 #if SD_DPP_REDUCE
     sx = wave_sum63(sx);
     sy = wave_sum63(sy);
@@ -915,7 +928,8 @@ __device__ __forceinline__ float speed3(float x, float y, float z) {
 template <bool GAS>
 __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) {
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
-    float vm = 0.f;
+    float vm2 = 0.f;  // the largest squared speed: sqrt is monotone and correctly rounded, so
+                      // sqrt(max v^2) is the largest speed3() bit for bit
     bool depAlive[KP];
     Cic depC[KP];
     // phase 1: every particle load of this thread in flight together
@@ -1066,7 +1080,7 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
                 sy += (double)v.y;
                 sz += (double)v.z;
                 cnt += 1.0;
-                vm = fmaxf(vm, speed3(v.x, v.y, v.z));
+                vm2 = fmaxf(vm2, fmaf(v.z, v.z, fmaf(v.y, v.y, v.x * v.x)));
             }
         } else if (a.order) {
             // an inactive slot is left untouched in place, but a fused reorder
@@ -1076,7 +1090,7 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
         }
     }
     if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
-    if (a.partials) block_partial(sx, sy, sz, cnt, vm, a.partials, blk);
+    if (a.partials) block_partial(sx, sy, sz, cnt, sqrtf(vm2), a.partials, blk);
 }
 
 // Consecutive particle tiles (Morton-sorted: neighbouring stencils) run on one
