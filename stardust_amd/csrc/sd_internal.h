@@ -155,7 +155,7 @@ struct IntegrateArgs {
 };
 int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
 int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)
-void launch_integrate(const IntegrateArgs& a, hipStream_t st);
+void launch_integrate(const IntegrateArgs& a, hipStream_t st, hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
 
 // reductions / scalars
 void launch_vmax(const float4* pos, const float4* vel, int n, ReducePartial* partials,

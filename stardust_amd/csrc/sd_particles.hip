@@ -18,6 +18,8 @@
 // where nvcc contracts "a*b + c"), written as explicit fmaf with contraction
 // otherwise off, so results are reproducible bit for bit against the CPU
 // restatement given the same mesh accelerations.
+#include <hip/hip_ext.h>
+
 #include "sd_device.h"
 #include "sd_internal.h"
 
@@ -1113,12 +1115,22 @@ __global__ void __launch_bounds__(PT) k_integrate_batch(const IntegrateArgs* __r
 int integrate_blocks(int n) { return (n + BP - 1) / BP; }
 int particle_blocks(int n) { return (n + PT - 1) / PT; }
 
-void launch_integrate(const IntegrateArgs& a, hipStream_t st) {
+// evStart / evStop (optional): events the dispatch itself records at the
+// kernel's start and end (no marker packets of their own in the queue).
+void launch_integrate(const IntegrateArgs& a, hipStream_t st, hipEvent_t evStart, hipEvent_t evStop) {
     if (a.n <= 0) return;
+    const dim3 grid(integrate_blocks(a.n)), block(PT);
+    if (evStart && evStop) {
+        if (a.accPress)
+            hipExtLaunchKernelGGL(k_integrate<true>, grid, block, 0, st, evStart, evStop, 0, a);
+        else
+            hipExtLaunchKernelGGL(k_integrate<false>, grid, block, 0, st, evStart, evStop, 0, a);
+        return;
+    }
     if (a.accPress)
-        hipLaunchKernelGGL(k_integrate<true>, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
+        hipLaunchKernelGGL(k_integrate<true>, grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL(k_integrate<false>, dim3(integrate_blocks(a.n)), dim3(PT), 0, st, a);
+        hipLaunchKernelGGL(k_integrate<false>, grid, block, 0, st, a);
 }
 
 // kMaxSpeed (Sim.cu:2923-2930) as block partials.
