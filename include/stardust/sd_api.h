@@ -119,6 +119,10 @@ typedef struct sd_modes {
     int exactGrid;      /* 0: round G down to a power of two and cap it as
                            Sim::reconfigure (Sim.cu:4081-4086, 4158);
                            1: accept G = 2^k or 3*2^k (<= 512 / 256) as given */
+    int sortMerge;      /* re-sorts that start from the last sort's keys (only
+                           the keys that changed are radix-sorted, then merged;
+                           same permutation as a full sort): 0 (default) for
+                           sorts of more than 2^23 keys, 1 always, -1 never */
 } sd_modes;
 
 /* Grids readable with sd_get_grid. */

@@ -89,6 +89,7 @@ class Modes(C.Structure):
         ("pipeline", C.c_int),
         ("momentumZero", C.c_int),
         ("exactGrid", C.c_int),
+        ("sortMerge", C.c_int),
     ]
 
 

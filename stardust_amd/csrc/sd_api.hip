@@ -164,6 +164,9 @@ static void free_buffers(sd_sim* s) {
         dfree(s->pos); dfree(s->vel); dfree(s->posTmp); dfree(s->velTmp);
     }
     dfree(s->keysA); dfree(s->keysB); dfree(s->idxA); dfree(s->idxB);
+    dfree(s->mb.mKeys); dfree(s->mb.mKeysAlt); dfree(s->mb.sKeys); dfree(s->mb.mIdx); dfree(s->mb.mIdxAlt);
+    dfree(s->mb.sIdx); dfree(s->mb.tileMovers); dfree(s->mb.counts);
+    s->keysAscending = false;
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact); dfree(s->disp); dfree(s->accPress);
@@ -383,13 +386,56 @@ static int key_bits(const sd_sim* s) {
 
 // Keys and the stable radix sort only: the permutation is left in s->sb.idx
 // for an integrate launch that reads through it (the fused reorder).
-static void sort_keys(sd_sim* s, bool forContact = false) {
+// The sort of nearly sorted keys, when the last sort left its ascending keys
+// in sb.keys: between step sorts few particles change cell, and only those
+// keys are radix-sorted before the merge (exact for any ascending sb.keys).
+#ifndef SD_SORT_MERGE
+#define SD_SORT_MERGE 1
+#endif
+#ifndef SD_SORT_MERGE_MIN_N
+#define SD_SORT_MERGE_MIN_N (1 << 23)
+#endif
+// (An allocation that fails here leaves the full sort in use; it is not a
+// device failure.)
+template <typename T>
+static bool try_alloc(T*& p, size_t count) {
+    return hipMalloc((void**)&p, sizeof(T) * count) == hipSuccess;
+}
+static bool ensure_mover_buffers(sd_sim* s) {
+    if (s->mb.counts) return true;
+    const size_t N = (size_t)(s->allocN > 0 ? s->allocN : 1);
+    MoverBuffers& m = s->mb;
+    const bool ok = try_alloc(m.mKeys, N) && try_alloc(m.mKeysAlt, N) && try_alloc(m.sKeys, N) &&
+                    try_alloc(m.mIdx, N) && try_alloc(m.mIdxAlt, N) && try_alloc(m.sIdx, N) &&
+                    try_alloc(m.tileMovers, (size_t)mover_tiles_for((int)N)) && try_alloc(m.counts, 2);
+    if (!ok) {
+        (void)hipGetLastError();
+        dfree(m.mKeys); dfree(m.mKeysAlt); dfree(m.sKeys); dfree(m.mIdx); dfree(m.mIdxAlt); dfree(m.sIdx);
+        dfree(m.tileMovers); dfree(m.counts);
+        return false;
+    }
+    return true;
+}
+
+// Cell keys of the current positions, stably sorted into sb.keys / sb.idx.
+static void sort_cell_keys(sd_sim* s, int kmode, hipStream_t st) {
     const int n = s->allocN;
+    // automatic: large sorts only (below, the three full radix passes take
+    // about as long as the changed keys' radix passes plus the merge)
+    const int pol = s->modes.sortMerge;
+    const bool want = pol > 0 || (pol == 0 && n > SD_SORT_MERGE_MIN_N);
+    if (SD_SORT_MERGE && want && s->keysAscending && n > 1 && ensure_mover_buffers(s)) {
+        sort_nearly_sorted(s->sb, s->mb, s->pos, n, s->G, kmode, key_bits(s), st);
+    } else {
+        launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, st);
+        radix_sort_pairs(s->sb, n, key_bits(s), st, true);
+    }
+    s->keysAscending = true;
+}
+
+static void sort_keys(sd_sim* s, bool forContact = false) {
     const int kmode = s->modes.keyMode == SD_KEY_MORTON ? (forContact ? 1 : 2) : 0;
-    TIMED(s, "sort", {
-        launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, s->st);
-        radix_sort_pairs(s->sb, n, key_bits(s), s->st, true);
-    });
+    TIMED(s, "sort", sort_cell_keys(s, kmode, s->st));
     s->keysSorted = true;
     s->corrValid = false;
 }
@@ -400,8 +446,7 @@ static void sort_particles(sd_sim* s, bool forContact = false) {
     const int n = s->allocN;
     const int kmode = s->modes.keyMode == SD_KEY_MORTON ? (forContact ? 1 : 2) : 0;
     TIMED(s, "sort", {
-        launch_cell_keys_hist(s->pos, n, s->G, kmode, s->sb, s->st);
-        radix_sort_pairs(s->sb, n, key_bits(s), s->st, true);
+        sort_cell_keys(s, kmode, s->st);
         launch_reorder(s->pos, s->vel, s->sb.idx, s->posTmp, s->velTmp, n, s->st);
     });
     if (s->borrowed) {
@@ -968,6 +1013,7 @@ void sd_modes_default(sd_modes* m) {
     m->pipeline = SD_PIPELINE_FUSED;
     m->momentumZero = 1;
     m->exactGrid = 0;
+    m->sortMerge = 0;
 }
 
 int sd_device_available(void) {
@@ -1092,7 +1138,7 @@ void sd_destroy(sd_sim* s) {
 int sd_set_modes(sd_sim* s, const sd_modes* m) {
     if (!s || !m) return SD_ERR_ARG;
     if (m->keyMode < 0 || m->keyMode > 1 || m->depositMode < 0 || m->depositMode > 1 || m->integrator < 0 ||
-        m->integrator > 1 || m->pipeline < 0 || m->pipeline > 1)
+        m->integrator > 1 || m->pipeline < 0 || m->pipeline > 1 || m->sortMerge < -1 || m->sortMerge > 1)
         return arg_error("invalid mode value");
     if (s->ready) {
         if (g_failed) return SD_ERR_FAILED;

@@ -85,6 +85,19 @@ void launch_cell_keys_hist(const float4* pos, int n, int G, int mode, SortBuffer
 // buf.keys / buf.idx on return. firstHistReady: buf.hist already holds the
 // first pass's histogram (launch_cell_keys_hist).
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady = false);
+// Lists of the sort of nearly sorted keys (n entries each, tileMovers one per
+// 2048-key tile, counts 2).
+struct MoverBuffers {
+    uint32_t *mKeys, *mKeysAlt, *sKeys, *tileMovers, *counts;
+    int32_t *mIdx, *mIdxAlt, *sIdx;
+};
+int mover_tiles_for(int n);
+// Cell keys of pos (as launch_cell_keys_hist) stably sorted into buf.keys /
+// buf.idx, given that buf.keys holds any ascending array of n keys on entry
+// (the previous sort's): the keys that differ from it are sorted alone and
+// merged into the rest. Exact whatever that array was; fast when few differ.
+void sort_nearly_sorted(SortBuffers& buf, const MoverBuffers& mb, const float4* pos, int n, int G, int mode,
+                        int bits, hipStream_t st);
 void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,
                     float4* dstV, int n, hipStream_t st);
 // Cell table of sorted keys (key = cell + 1, 0 = inactive), when range is not

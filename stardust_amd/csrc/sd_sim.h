@@ -60,6 +60,8 @@ struct sd_sim {
     uint32_t *hist = nullptr, *chunk = nullptr;
     sd::SortBuffers sb{};
     bool keysSorted = false;
+    bool keysAscending = false;   // sb.keys holds an ascending array of allocN keys (the last sort's)
+    sd::MoverBuffers mb{};        // lists of the sort of nearly sorted keys (allocated on first use)
 
     // mesh
     long long* rhoFix = nullptr;

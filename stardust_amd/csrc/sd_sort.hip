@@ -20,6 +20,7 @@
 //   scatter : digit bases from the totals, stable in-tile ranks (same ballot
 //             match, running per-digit counters across rounds and waves) ->
 //             global positions.
+#include <algorithm>
 #include <cmath>
 
 #include "sd_device.h"
@@ -35,6 +36,10 @@ constexpr int SORT_ROUNDS_SMALL = 8, SORT_ROUNDS_LARGE = 16;
 constexpr int SORT_SMALL_N = 1 << 23;
 static int sort_rounds_for(int n) { return n <= SORT_SMALL_N ? SORT_ROUNDS_SMALL : SORT_ROUNDS_LARGE; }
 constexpr int SCAN_CHUNK = 1024;  // hist entries per scan block
+#ifndef SD_SORT_DEV_GRID
+#define SD_SORT_DEV_GRID 512
+#endif
+constexpr int SORT_DEV_GRID = SD_SORT_DEV_GRID;  // blocks of a sort on a device count
 constexpr int WAVES = SORT_THREADS / 64;
 
 // Key of one particle: cell (mode 0 row-major, 1 Morton) or CIC base node
@@ -85,18 +90,23 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     return m;
 }
 
+// Element count of a sort: the launch's n, or a count on the device (nDev) when
+// the list was built on the device; launches are then sized for the largest
+// count, and tiles beyond the device count leave at once.
+__device__ __forceinline__ int sort_count(int n, const uint32_t* nDev) { return nDev ? (int)*nDev : n; }
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
 template <int R>
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __restrict__ keys, int n, int shift,
-                                                            uint32_t* __restrict__ hist, int ntiles) {
+__device__ __forceinline__ void sort_hist_tile(const uint32_t* __restrict__ keys, int n, int shift,
+                                               uint32_t* __restrict__ hist, int ntiles, int tile) {
     __shared__ uint32_t cnt[256];
+    const int base = tile * SORT_THREADS * R;
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * SORT_THREADS * R;
     // every key load of the thread in flight at once, then the counting
     uint32_t kk[R];
 #pragma unroll
@@ -118,7 +128,25 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __re
         }
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    hist[(size_t)threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+}
+
+// A block per tile; with a count on the device (nDev), a grid of fewer blocks
+// walks the tiles in use.
+template <int R>
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                            uint32_t* __restrict__ hist, int ntiles,
+                                                            const uint32_t* __restrict__ nDev) {
+    if (!nDev) {
+        sort_hist_tile<R>(keys, n, shift, hist, ntiles, blockIdx.x);
+        return;
+    }
+    n = (int)*nDev;
+    const int used = (n + SORT_THREADS * R - 1) / (SORT_THREADS * R);
+    for (int tile = blockIdx.x; tile < used; tile += gridDim.x) {  // uniform
+        sort_hist_tile<R>(keys, n, shift, hist, ntiles, tile);
+        __syncthreads();  // the tile's counts are read out before the next clears them
+    }
 }
 
 // Keys and identity values of one sort tile, plus the tile's histogram of the
@@ -193,10 +221,13 @@ __device__ uint32_t block_scan_excl(uint32_t* v4, uint32_t* s_wave) {
 // hist[d * ntiles + tile]), in place, one block per digit; totals[d] = the
 // digit's key count. The scatter adds the digit's base (the exclusive scan of
 // totals) itself.
-__global__ void __launch_bounds__(256) k_scan_digit_rows(uint32_t* __restrict__ h, int ntiles,
-                                                         uint32_t* __restrict__ totals) {
+__global__ void __launch_bounds__(256) k_scan_digit_rows(uint32_t* __restrict__ h, int stride,
+                                                         uint32_t* __restrict__ totals,
+                                                         const uint32_t* __restrict__ nDev, int tile) {
     __shared__ uint32_t s_wave[WAVES];
-    uint32_t* row = h + (size_t)blockIdx.x * ntiles;
+    // rows hold `stride` tile entries; the tiles in use cover the sort's count
+    const int ntiles = nDev ? (int)((*nDev + (uint32_t)tile - 1u) / (uint32_t)tile) : stride;
+    uint32_t* row = h + (size_t)blockIdx.x * stride;
     uint32_t carry = 0;
     for (int off = 0; off < ntiles; off += SCAN_CHUNK) {  // uniform
         const int base = off + threadIdx.x * 4;
@@ -226,19 +257,19 @@ __global__ void __launch_bounds__(256) k_scan_digit_rows(uint32_t* __restrict__ 
 // that pays when a round's 256 keys spread over many digits (sparse keys), and
 // costs LDS traffic and occupancy when equal digits already come in long runs.
 template <bool LOCAL, int R>
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* __restrict__ keys,
-                                                               const int32_t* __restrict__ vals, int n, int shift,
-                                                               const uint32_t* __restrict__ offs,
-                                                               const uint32_t* __restrict__ totals, int ntiles,
-                                                               uint32_t* __restrict__ keysOut,
-                                                               int32_t* __restrict__ valsOut) {
+__device__ __forceinline__ void sort_scatter_tile(const uint32_t* __restrict__ keys,
+                                                  const int32_t* __restrict__ vals, int n, int shift,
+                                                  const uint32_t* __restrict__ offs,
+                                                  const uint32_t* __restrict__ totals, int ntiles,
+                                                  uint32_t* __restrict__ keysOut, int32_t* __restrict__ valsOut,
+                                                  int tile, int usedTiles) {
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_run[256];
     __shared__ uint32_t s_wcnt[WAVES][256];
     __shared__ uint32_t s_wbase[WAVES][256];
     __shared__ uint32_t s_wave[WAVES];
     const int t = threadIdx.x;
-    const size_t flat = (size_t)t * ntiles + blockIdx.x;
+    const size_t flat = (size_t)t * ntiles + tile;  // ntiles: the row stride of offs
     const uint32_t loc = offs[flat], tot = totals[t];
     {
         uint32_t v4[4] = {tot, 0u, 0u, 0u};
@@ -252,13 +283,13 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
     __shared__ uint32_t s_lk[LOCAL ? SORT_THREADS * R : 1];
     __shared__ int32_t s_lv[LOCAL ? SORT_THREADS * R : 1];
     if constexpr (LOCAL) {
-        const uint32_t next = (int)blockIdx.x + 1 < ntiles ? offs[flat + 1] : tot;
+        const uint32_t next = tile + 1 < usedTiles ? offs[flat + 1] : tot;
         uint32_t v4[4] = {next - loc, 0u, 0u, 0u};
         block_scan_excl(v4, s_wave);
         s_start[t] = v4[0];
     }
     __syncthreads();
-    const int base = blockIdx.x * SORT_THREADS * R;
+    const int base = tile * SORT_THREADS * R;
     const int lane = t & 63, wv = t >> 6;
     // every key and value load of the thread in flight at once
     uint32_t kk[R];
@@ -326,6 +357,28 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* _
     }
 }
 
+// A block per tile; with a count on the device (nDev), a grid of fewer blocks
+// walks the tiles in use.
+template <bool LOCAL, int R>
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t* __restrict__ keys,
+                                                               const int32_t* __restrict__ vals, int n, int shift,
+                                                               const uint32_t* __restrict__ offs,
+                                                               const uint32_t* __restrict__ totals, int ntiles,
+                                                               uint32_t* __restrict__ keysOut,
+                                                               int32_t* __restrict__ valsOut,
+                                                               const uint32_t* __restrict__ nDev) {
+    if (!nDev) {
+        sort_scatter_tile<LOCAL, R>(keys, vals, n, shift, offs, totals, ntiles, keysOut, valsOut, blockIdx.x, ntiles);
+        return;
+    }
+    n = (int)*nDev;
+    const int used = (n + SORT_THREADS * R - 1) / (SORT_THREADS * R);
+    for (int tile = blockIdx.x; tile < used; tile += gridDim.x) {  // uniform
+        sort_scatter_tile<LOCAL, R>(keys, vals, n, shift, offs, totals, ntiles, keysOut, valsOut, tile, used);
+        __syncthreads();  // the tile's LDS is read out before the next tile writes it
+    }
+}
+
 __global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP, const float4* __restrict__ srcV,
                                                  const int32_t* __restrict__ order, float4* __restrict__ dstP,
                                                  float4* __restrict__ dstV, int n) {
@@ -366,6 +419,263 @@ __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__
     }
 }
 
+// ---- sort of nearly sorted keys ------------------------------------------------
+// Between two step sorts a particle moves a fraction of a cell, so most keys
+// do not change. Let prev be the previous sort's key array (any ascending
+// array of n keys). The slots whose new key equals prev[i] ("stayers") are in
+// ascending (key, index) order already: their keys ascend with the index. The
+// others ("movers") are gathered in index order and sorted by a stable radix
+// sort, so they too are in (key, index) order; a merge path of the two lists by
+// (key, index) is then the stable sort of all n keys, whatever prev held. Only
+// the speed depends on prev matching the particles.
+constexpr int NS_R = 8;                       // keys per thread per tile
+constexpr int NS_TILE = SORT_THREADS * NS_R;  // 2048
+constexpr int MERGE_T = 2048;                 // outputs per merge block (8 per thread)
+
+// New keys into nk and each tile's mover count.
+__global__ void __launch_bounds__(SORT_THREADS) k_keys_movers(const float4* __restrict__ pos, int n, int G, int mode,
+                                                              const uint32_t* __restrict__ prev,
+                                                              uint32_t* __restrict__ nk,
+                                                              uint32_t* __restrict__ tileMovers) {
+    __shared__ uint32_t s_w[WAVES];
+    const int base = blockIdx.x * NS_TILE, t = threadIdx.x;
+    float4 pp[NS_R];
+    uint32_t pk[NS_R];
+#pragma unroll
+    for (int r = 0; r < NS_R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        pp[r] = i < n ? pos[i] : make_float4(-1.f, -1.f, -1.f, 0.f);
+        pk[r] = i < n ? prev[i] : 0u;
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int r = 0; r < NS_R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        if (i < n) {
+            const uint32_t k = cell_key(pp[r], G, mode);
+            nk[i] = k;
+            cnt += k != pk[r] ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((t & 63) == 0) s_w[t >> 6] = (uint32_t)cnt;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < WAVES; ++w) sum += s_w[w];
+        tileMovers[blockIdx.x] = sum;
+    }
+}
+
+// Exclusive scan of the tile mover counts in place (one block); counts[0] =
+// movers, counts[1] = stayers.
+__global__ void __launch_bounds__(1024) k_scan_tile_movers(uint32_t* __restrict__ tileMovers, int ntiles, int n,
+                                                           uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_w[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t carry = 0;
+    for (int off = 0; off < ntiles; off += 1024) {  // uniform
+        const int i = off + t;
+        const uint32_t v = i < ntiles ? tileMovers[i] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w] = x;
+        __syncthreads();
+        uint32_t wb = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            if (k < w) wb += s_w[k];
+            tot += s_w[k];
+        }
+        if (i < ntiles) tileMovers[i] = carry + wb + x - v;
+        carry += tot;
+        __syncthreads();
+    }
+    if (t == 0) {
+        counts[0] = carry;
+        counts[1] = (uint32_t)n - carry;
+    }
+}
+
+// Stayers and movers of each tile to their lists, both in index order.
+__global__ void __launch_bounds__(SORT_THREADS) k_split_movers(const uint32_t* __restrict__ nk,
+                                                               const uint32_t* __restrict__ prev, int n,
+                                                               const uint32_t* __restrict__ tileBase,
+                                                               uint32_t* __restrict__ sKeys, int32_t* __restrict__ sIdx,
+                                                               uint32_t* __restrict__ mKeys,
+                                                               int32_t* __restrict__ mIdx) {
+    __shared__ uint32_t s_w[NS_R][WAVES];
+    const int base = blockIdx.x * NS_TILE, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t kk[NS_R], pk[NS_R];
+#pragma unroll
+    for (int r = 0; r < NS_R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        kk[r] = i < n ? nk[i] : 0u;
+        pk[r] = i < n ? prev[i] : 0u;
+    }
+    bool mv[NS_R];
+    uint32_t pre[NS_R];
+#pragma unroll
+    for (int r = 0; r < NS_R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        mv[r] = i < n && kk[r] != pk[r];
+        const uint64_t b = __ballot(mv[r]);
+        pre[r] = (uint32_t)__popcll(b & lanemask_lt());
+        if (lane == 0) s_w[r][w] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    const uint32_t mBase = tileBase[blockIdx.x];
+    const uint32_t sBase = (uint32_t)base - mBase;  // stayers of the tiles before this one
+    uint32_t before = 0;                            // movers of this tile in earlier rounds
+#pragma unroll
+    for (int r = 0; r < NS_R; ++r) {
+        const int i = base + r * SORT_THREADS + t;
+        uint32_t wb = 0, rt = 0;
+#pragma unroll
+        for (int k = 0; k < WAVES; ++k) {
+            const uint32_t c = s_w[r][k];
+            wb += k < w ? c : 0u;
+            rt += c;
+        }
+        const uint32_t rank = before + wb + pre[r];  // movers of the tile before slot i
+        if (mv[r]) {
+            mKeys[mBase + rank] = kk[r];
+            mIdx[mBase + rank] = i;
+        } else if (i < n) {
+            const uint32_t sr = (uint32_t)(r * SORT_THREADS + t) - rank;
+            sKeys[sBase + sr] = kk[r];
+            sIdx[sBase + sr] = i;
+        }
+        before += rt;
+    }
+}
+
+__device__ __forceinline__ uint64_t key_idx(uint32_t k, int32_t i) { return ((uint64_t)k << 32) | (uint32_t)i; }
+
+// Merge path of the stayers (nS) and the sorted movers (nM) into the stable
+// order: block b writes outputs [b T, b T + T). Its first (last) stayer is the
+// first a in [max(0, d - nM), min(d, nS)] with a = nS, a = d or
+// S[a] > M[d - 1 - a] (a monotone predicate), found by a 128-ary search of
+// each half of the block; the tile then merges in LDS, 8 outputs per thread.
+__global__ void __launch_bounds__(256) k_merge_movers(const uint32_t* __restrict__ sKeys,
+                                                      const int32_t* __restrict__ sIdx,
+                                                      const uint32_t* __restrict__ mKeys,
+                                                      const int32_t* __restrict__ mIdx,
+                                                      const uint32_t* __restrict__ counts, int n,
+                                                      uint32_t* __restrict__ outKeys,
+                                                      int32_t* __restrict__ outIdx) {
+    // element e at PADM(e) = e + e/8: the 8-element runs of consecutive threads
+    // (64 B apart) then fall in distinct banks
+    __shared__ uint64_t s_v[MERGE_T + MERGE_T / 8];
+    __shared__ int s_lo[2], s_hi[2];
+    __shared__ uint64_t s_first[2][2];  // per half: (found lanes of the two waves)
+    const int t = threadIdx.x, half = t >> 7, ht = t & 127;
+    const int nM = (int)counts[0], nS = n - nM;
+    const int d0 = blockIdx.x * MERGE_T, d1 = min(d0 + MERGE_T, n);
+    const int d = half ? d1 : d0;
+    if (ht == 0) {
+        s_lo[half] = max(0, d - nM);
+        s_hi[half] = min(d, nS);
+    }
+    __syncthreads();
+    // each half narrows its range [lo, hi] (P(hi) true) by 128 probes a round
+    for (;;) {
+        const int lo = s_lo[half], hi = s_hi[half];
+        const bool done0 = s_lo[0] >= s_hi[0], done1 = s_lo[1] >= s_hi[1];
+        if (done0 && done1) break;  // uniform
+        bool p = false;
+        int c = hi;
+        if (lo < hi) {
+            // probe c = lo + (ht + 1) * (hi - lo) / 129 ... the last probe is hi - 1 at most
+            c = lo + (int)(((long long)(ht + 1) * (hi - lo)) / 129);
+            if (c >= hi) c = hi - 1;
+            const int b = d - 1 - c;
+            p = c >= nS || b < 0 || key_idx(sKeys[c], sIdx[c]) > key_idx(mKeys[b], mIdx[b]);
+        }
+        const uint64_t bal = __ballot(p);
+        __syncthreads();  // every lane has read s_lo / s_hi
+        if ((t & 63) == 0) s_first[half][(t >> 6) & 1] = bal;
+        __syncthreads();
+        if (ht == 0 && lo < hi) {
+            // first true probe of this half (probes ascend with ht); none: the answer is hi
+            const uint64_t b0 = s_first[half][0], b1 = s_first[half][1];
+            const int f = b0 ? __ffsll((long long)b0) - 1 : (b1 ? 64 + __ffsll((long long)b1) - 1 : -1);
+            auto probe = [&](int q) {
+                int cc = lo + (int)(((long long)(q + 1) * (hi - lo)) / 129);
+                return cc >= hi ? hi - 1 : cc;
+            };
+            if (f < 0) {
+                s_lo[half] = probe(127) + 1;
+            } else {
+                s_hi[half] = probe(f);
+                if (f > 0) s_lo[half] = probe(f - 1) + 1;
+            }
+        }
+        __syncthreads();
+    }
+    const int a0 = s_lo[0], a1 = s_lo[1];
+    const int b0 = d0 - a0, b1 = d1 - a1;
+    const int ns = a1 - a0, nm = b1 - b0, nt = d1 - d0;
+    if (nm == 0 || ns == 0) {  // (uniform) one list only: a copy
+        const uint32_t* k = nm == 0 ? sKeys + a0 : mKeys + b0;
+        const int32_t* x = nm == 0 ? sIdx + a0 : mIdx + b0;
+        for (int e = t; e < nt; e += 256) {
+            outKeys[d0 + e] = k[e];
+            outIdx[d0 + e] = x[e];
+        }
+        return;
+    }
+    auto PADM = [](int e) { return e + (e >> 3); };
+    for (int e = t; e < nt; e += 256) {
+        if (e < ns)
+            s_v[PADM(e)] = key_idx(sKeys[a0 + e], sIdx[a0 + e]);
+        else
+            s_v[PADM(e)] = key_idx(mKeys[b0 + e - ns], mIdx[b0 + e - ns]);
+    }
+    __syncthreads();
+    // this thread's outputs [8t, 8t + 8): its split x (stayers among the first 8t)
+    const int q = 8 * t;
+    uint64_t out[8];
+    int cnt = 0;
+    if (q < nt) {
+        int lo = max(0, q - nm), hi = min(q, ns);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int b = q - 1 - mid;
+            if (b < 0 || s_v[PADM(mid)] > s_v[PADM(ns + b)])
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        int i = lo, j = q - lo;
+        cnt = min(8, nt - q);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (e < cnt) {
+                const uint64_t vs = i < ns ? s_v[PADM(i)] : ~0ull, vm = j < nm ? s_v[PADM(ns + j)] : ~0ull;
+                const bool takeS = j >= nm || (i < ns && vs < vm);
+                out[e] = takeS ? vs : vm;
+                i += takeS ? 1 : 0;
+                j += takeS ? 0 : 1;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        if (e < cnt) s_v[PADM(q + e)] = out[e];
+    __syncthreads();
+    for (int e = t; e < nt; e += 256) {
+        const uint64_t v = s_v[PADM(e)];
+        outKeys[d0 + e] = (uint32_t)(v >> 32);
+        outIdx[d0 + e] = (int32_t)(uint32_t)v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 void launch_cell_keys_batch(const float4* pos, const KeyJob* jobs, const int* prefix, int njobs,
@@ -392,20 +702,25 @@ void launch_cell_keys_hist(const float4* pos, int n, int G, int mode, SortBuffer
 }
 
 template <int R>
-static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady);
+static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady,
+                         const uint32_t* nDev);
 
 void radix_sort_pairs(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady) {
     if (n <= 1) return;
     if (sort_rounds_for(n) == SORT_ROUNDS_SMALL)
-        radix_passes<SORT_ROUNDS_SMALL>(buf, n, bits, st, firstHistReady);
+        radix_passes<SORT_ROUNDS_SMALL>(buf, n, bits, st, firstHistReady, nullptr);
     else
-        radix_passes<SORT_ROUNDS_LARGE>(buf, n, bits, st, firstHistReady);
+        radix_passes<SORT_ROUNDS_LARGE>(buf, n, bits, st, firstHistReady, nullptr);
 }
 
+// n: the element count, or with nDev the largest count the device count can
+// take (launch sizes; the kernels read the count itself).
 template <int R>
-static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady) {
-    if (n <= 1) return;
+static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool firstHistReady,
+                         const uint32_t* nDev) {
+    if (n <= 0 || (n <= 1 && !nDev)) return;
     const int ntiles = sort_tiles_for(n);
+    constexpr int TILE = SORT_THREADS * R;
     // Keys sparse in their code space (under one key per two codes) spread each
     // round over many digits, and so do the lowest digits of any keys: order
     // the tile in LDS first there, except in the last pass, whose few digit
@@ -414,17 +729,19 @@ static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool
     // ordered; two-disk G=128, 4.2 M keys of 21 bits: 43/24/23 direct,
     // 38/34/33 ordered.)
     const bool sparse = SD_SORT_LOCAL && 2.0 * (double)n < std::ldexp(1.0, bits);
+    // with a device count: a grid of at most SORT_DEV_GRID blocks walks the tiles
+    const int grid = nDev ? std::min(ntiles, SORT_DEV_GRID) : ntiles;
     for (int shift = 0; shift < bits; shift += 8) {
         if (shift > 0 || !firstHistReady)
-            hipLaunchKernelGGL(k_sort_hist<R>, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
-                               buf.hist, ntiles);
-        hipLaunchKernelGGL(k_scan_digit_rows, dim3(256), dim3(256), 0, st, buf.hist, ntiles, buf.chunk);
+            hipLaunchKernelGGL(k_sort_hist<R>, dim3(grid), dim3(SORT_THREADS), 0, st, buf.keys, n, shift,
+                               buf.hist, ntiles, nDev);
+        hipLaunchKernelGGL(k_scan_digit_rows, dim3(256), dim3(256), 0, st, buf.hist, ntiles, buf.chunk, nDev, TILE);
         if (SD_SORT_LOCAL && (sparse || shift == 0) && shift + 8 < bits)
-            hipLaunchKernelGGL((k_sort_scatter<true, R>), dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
-                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt);
+            hipLaunchKernelGGL((k_sort_scatter<true, R>), dim3(grid), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
+                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt, nDev);
         else
-            hipLaunchKernelGGL((k_sort_scatter<false, R>), dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
-                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt);
+            hipLaunchKernelGGL((k_sort_scatter<false, R>), dim3(grid), dim3(SORT_THREADS), 0, st, buf.keys, buf.idx, n,
+                               shift, buf.hist, buf.chunk, ntiles, buf.keysAlt, buf.idxAlt, nDev);
         uint32_t* tk = buf.keys;
         buf.keys = buf.keysAlt;
         buf.keysAlt = tk;
@@ -432,6 +749,28 @@ static void radix_passes(SortBuffers& buf, int n, int bits, hipStream_t st, bool
         buf.idx = buf.idxAlt;
         buf.idxAlt = ti;
     }
+}
+
+int mover_tiles_for(int n) { return (n + NS_TILE - 1) / NS_TILE; }
+
+void sort_nearly_sorted(SortBuffers& buf, const MoverBuffers& mb, const float4* pos, int n, int G, int mode,
+                        int bits, hipStream_t st) {
+    if (n <= 0) return;
+    const int ntiles = mover_tiles_for(n);
+    // buf.keys: the previous (ascending) keys; buf.keysAlt: the new keys
+    hipLaunchKernelGGL(k_keys_movers, dim3(ntiles), dim3(SORT_THREADS), 0, st, pos, n, G, mode, buf.keys, buf.keysAlt,
+                       mb.tileMovers);
+    hipLaunchKernelGGL(k_scan_tile_movers, dim3(1), dim3(1024), 0, st, mb.tileMovers, ntiles, n, mb.counts);
+    hipLaunchKernelGGL(k_split_movers, dim3(ntiles), dim3(SORT_THREADS), 0, st, buf.keysAlt, buf.keys, n,
+                       mb.tileMovers, mb.sKeys, mb.sIdx, mb.mKeys, mb.mIdx);
+    // the movers' stable sort, sized for n, on the device count
+    SortBuffers ms{mb.mKeys, mb.mIdx, mb.mKeysAlt, mb.mIdxAlt, buf.hist, buf.chunk, buf.maxTiles};
+    if (sort_rounds_for(n) == SORT_ROUNDS_SMALL)
+        radix_passes<SORT_ROUNDS_SMALL>(ms, n, bits, st, false, mb.counts);
+    else
+        radix_passes<SORT_ROUNDS_LARGE>(ms, n, bits, st, false, mb.counts);
+    hipLaunchKernelGGL(k_merge_movers, dim3((n + MERGE_T - 1) / MERGE_T), dim3(256), 0, st, mb.sKeys, mb.sIdx, ms.keys,
+                       ms.idx, mb.counts, n, buf.keys, buf.idx);
 }
 
 void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order, float4* dstP,

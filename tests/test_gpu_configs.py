@@ -61,20 +61,23 @@ def test_config0_reference_order_vs_oracle(gpu, oracle):
     assert abs(t - st.state.simTime) < 1e-9 + 1e-6 * st.state.simTime
 
 
-def test_config0_product_path_vs_oracle(gpu, oracle):
+@pytest.mark.parametrize("interval,merge", [(40, 0), (3, 1)])
+def test_config0_product_path_vs_oracle(gpu, oracle, interval, merge):
+    """merge = 1: every re-sort after the first starts from the last sort's
+    keys (the changed keys sorted alone, then merged)."""
     c, pos, vel = _config0()
     n, G, steps, dt = pos.shape[0], c["G"], c["steps"], c["dt"]
-    _say("config0 product path", n, G, steps)
+    _say("config0 product path", n, G, steps, "sort every", interval, "merge", merge)
     cfg = sd.default_config(particleCount=n, gridSize=G, gravity=c["gravity"], softeningCells=c["soft"],
-                            sortInterval=40)
-    s = sd.Sim(cfg, sd.default_modes())  # fused, leapfrog, Morton keys
+                            sortInterval=interval)
+    s = sd.Sim(cfg, sd.default_modes(sortMerge=merge))  # fused, leapfrog, Morton keys
     s.set_fixed_dt(dt)
     s.set_particles(pos, vel)
     s.run(steps)
     gp, gv = s.get_particles()
     s.close()
     # leapfrog from v0 = kick-drift from v0 - a(x0) dt / 2 (positions); a(x0) from the oracle
-    prm = oracle.make_params(G, gravity=c["gravity"], softeningCells=c["soft"], sortInterval=40,
+    prm = oracle.make_params(G, gravity=c["gravity"], softeningCells=c["soft"], sortInterval=interval,
                              keyMode=oracle.KEY_MORTON, fixedDt=dt, allocN=n)
     a0 = oracle.particle_accel(prm, pos)
     v0 = vel.copy()

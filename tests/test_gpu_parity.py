@@ -67,6 +67,42 @@ def test_sort_bit_exact(gpu, oracle, key_mode, n, G):
     s.close()
 
 
+@pytest.mark.parametrize("key_mode", [sd.KEY_ROWMAJOR, sd.KEY_MORTON])
+@pytest.mark.parametrize("n,G,frac", [(5000, 16, 0.0), (20000, 32, 0.03), (300000, 128, 0.1),
+                                      (300000, 128, 1.0), (1000003, 64, 0.3)])
+def test_resort_of_nearly_sorted_keys(gpu, oracle, key_mode, n, G, frac):
+    """A second sort starts from the first one's ascending keys: the keys that
+    changed are sorted alone and merged into the unchanged ones. The result
+    must be the stable sort of all keys, for any fraction of changed keys
+    (frac of the particles jump anywhere, the rest move by up to 0.2 cells),
+    and also when the previous keys belong to other particles entirely."""
+    _say("resort", key_mode, n, G, frac)
+    pos, vel = random_particles(n, seed=n + G + 1)
+    s = make_sim(n, G, keyMode=key_mode, sortMerge=1)
+    s.set_particles(pos, vel)
+    s.sort_now()
+    p1, v1 = s.get_particles()
+    rng = np.random.default_rng(n + 7)
+    live = p1[:, 0] >= 0
+    p2 = p1.copy()
+    p2[live, :3] += rng.uniform(-0.2, 0.2, (int(live.sum()), 3)).astype(np.float32) / G
+    jump = live & (rng.random(n) < frac)
+    p2[jump, :3] = rng.uniform(0.01, 0.99, (int(jump.sum()), 3)).astype(np.float32)
+    p2[:, :3] = np.where(live[:, None], np.clip(p2[:, :3], 0.003, 0.997), p2[:, :3])
+    mode = oracle.KEY_MORTON_NODE if key_mode else oracle.KEY_ROWMAJOR
+    for label, p in (("moved", p2), ("unrelated", random_particles(n, seed=n + G + 2)[0])):
+        s.set_particles(p, v1)
+        keys, order = s.sort_now()
+        ok, oi = oracle.cell_keys(p, G, mode)
+        sk, so = oracle.stable_sort_pairs(ok, oi)
+        _say("  ", label)
+        assert np.array_equal(keys, sk), label
+        assert np.array_equal(order, so), label
+        q, _ = s.get_particles()
+        assert np.array_equal(q, p[so]), label
+    s.close()
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("n,G", [(5000, 16), (100000, 32), (400000, 64)])
 def test_deposit_fixed_bit_exact(gpu, oracle, periodic, n, G):
