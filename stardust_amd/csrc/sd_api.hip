@@ -797,6 +797,8 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         // kGridAccel's differences taken in the gather (bit-identical accelerations)
         a.pot = s->pot;
         a.gradH = 0.5f * (float)s->G * pot_scale(s);
+        // under half a particle per cell, each block gathers from an LDS copy of its potential box
+        a.boxGather = (double)s->allocN < 0.5 * (double)s->G * s->G * s->G ? 1 : 0;
     }
     a.pos = s->pos;
     a.vel = s->vel;

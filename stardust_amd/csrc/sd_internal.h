@@ -165,6 +165,10 @@ struct IntegrateArgs {
     float gradH;
     // gas pressure acceleration grid (float4[G^3]); nullptr = off
     const float4* accPress;
+    // sparse meshes (under half a particle per cell): each block copies the
+    // potential box its particles' stencils cover into LDS once and gathers
+    // from there (same operations, bit-identical accelerations)
+    int boxGather;
 };
 int integrate_blocks(int n);  // k_integrate blocks (partials it writes)
 int particle_blocks(int n);   // one-particle-per-thread kernels (vmax, momentum)

@@ -310,11 +310,10 @@ __device__ __forceinline__ void add_global8(const Cic& c, const uint32_t v[8], i
     }
 }
 
-// Block-wide deposit of KP particles per thread. Must be called by every
-// thread of the block (it synchronises).
-__device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[KP], int G, int periodic,
-                                              long long* rho) {
-    __shared__ unsigned long long s_tile[DEP_CAP];
+// Block-wide deposit of KP particles per thread into an LDS tile of DEP_CAP
+// cells (s_tile). Must be called by every thread of the block (it synchronises).
+__device__ __forceinline__ void block_deposit_in(const bool alive[KP], const Cic c[KP], int G, int periodic,
+                                                 long long* rho, unsigned long long* __restrict__ s_tile) {
     __shared__ int s_box[PT / 64][6];
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
@@ -408,6 +407,12 @@ __device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[
                   gz = wrap_or_clamp(lo[2] + z, G, periodic);
         atomicAdd((unsigned long long*)elem32(rho, cell_idx(gx, gy, gz, G)), v);
     }
+}
+
+__device__ __forceinline__ void block_deposit(const bool alive[KP], const Cic c[KP], int G, int periodic,
+                                              long long* rho) {
+    __shared__ unsigned long long s_tile[DEP_CAP];
+    block_deposit_in(alive, c, G, periodic, rho, s_tile);
 }
 
 __global__ void __launch_bounds__(PT) k_deposit_fixed(const float4* __restrict__ pos, int n, int G, int periodic,
@@ -872,6 +877,165 @@ __device__ __forceinline__ float3 sample_acc_pot(const float* __restrict__ pot, 
     return make_float3(ax, ay, az);
 }
 
+// ---- the potential box of a block in LDS (sparse meshes) --------------------------
+// Under half a particle per cell, neighbouring particles' stencil rows rarely
+// share cache lines, so the global gather fetches the potential several times
+// over. Instead a block copies the box its particles' stencils cover (nodes
+// ix - 1 .. ix + 2 of every live particle, x widened to whole float4 lines)
+// into LDS once, and gathers from there with the same operations. The x origin
+// and extent are multiples of 4, so each float4 of the copy is one aligned
+// global load and one aligned LDS store, and (G a multiple of 4) never crosses
+// the periodic boundary. Values outside an isolated mesh are clamped copies:
+// only particles whose stencil lies inside the mesh read the box there.
+#ifndef SD_BOX_CAP
+#define SD_BOX_CAP 9216
+#endif
+#ifndef SD_BOX_GATHER
+#define SD_BOX_GATHER 1
+#endif
+constexpr int BOX_CAP = SD_BOX_CAP;  // floats (36 KB: four blocks per CU); the deposit's tile reuses the memory
+
+struct PotBox {
+    int x0, y0, z0;  // origin (node coordinates, unwrapped)
+    int dx, dxy;     // row and plane pitch (floats)
+    bool ok;         // the block has live particles and its box fits BOX_CAP (uniform)
+};
+
+// Exact quotient q = t / d for 0 <= t < 2^15, d >= 1: the float estimate is within
+// one of it, and one correction each way fixes it.
+__device__ __forceinline__ int quot_small(int t, int d, float invD) {
+    int q = (int)(((float)t + 0.5f) * invD);
+    if (__mul24(q, d) > t) --q;
+    if (__mul24(q + 1, d) <= t) ++q;
+    return q;
+}
+
+// Every thread of the block calls it (it synchronises).
+__device__ __forceinline__ PotBox load_pot_box(const bool alive[KP], const Cic c[KP], const float* __restrict__ pot,
+                                               int G, int periodic, float* __restrict__ box) {
+    __shared__ int s_bb[PT / 64][6];
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        if (alive[k]) {
+            lo[0] = min(lo[0], c[k].ix); hi[0] = max(hi[0], c[k].ix);
+            lo[1] = min(lo[1], c[k].iy); hi[1] = max(hi[1], c[k].iy);
+            lo[2] = min(lo[2], c[k].iz); hi[2] = max(hi[2], c[k].iz);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_min63(lo[d]);
+        hi[d] = wave_max63(hi[d]);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) {
+        for (int d = 0; d < 3; ++d) {
+            s_bb[w][d] = lo[d];
+            s_bb[w][3 + d] = hi[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PT / 64; ++q) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], s_bb[q][d]);
+            hi[d] = max(hi[d], s_bb[q][3 + d]);
+        }
+    }
+    PotBox b;
+    b.ok = false;
+    b.x0 = b.y0 = b.z0 = b.dx = b.dxy = 0;
+    if (lo[0] == INT_MAX) return b;  // no live particle (uniform)
+    // x from a multiple of 4 at or below ix - 1 to one above ix + 2 (floor division)
+    b.x0 = (lo[0] - 1) & ~3;
+    const int x1 = ((hi[0] + 3) + 3) & ~3;
+    b.y0 = lo[1] - 1;
+    b.z0 = lo[2] - 1;
+    const long long dx = x1 - b.x0, dy = (long long)hi[1] - lo[1] + 4, dz = (long long)hi[2] - lo[2] + 4;
+    if (dx * dy * dz > BOX_CAP) return b;
+    b.ok = true;
+    b.dx = (int)dx;
+    b.dxy = (int)(dx * dy);
+    const int nx4 = (int)dx >> 2, rows = (int)(dy * dz), units = nx4 * rows;
+    const float inv4 = 1.0f / (float)nx4, invY = 1.0f / (float)dy;
+    const unsigned g1 = (unsigned)G;
+    for (int u = threadIdx.x; u < units; u += PT) {
+        const int row = quot_small(u, nx4, inv4), x4 = u - __mul24(row, nx4);
+        const int z = quot_small(row, (int)dy, invY), y = row - __mul24(z, (int)dy);
+        const int gx = wrap_or_clamp(b.x0 + 4 * x4, G, periodic), gy = wrap_or_clamp(b.y0 + y, G, periodic),
+                  gz = wrap_or_clamp(b.z0 + z, G, periodic);
+        float4 v;
+        if (periodic || (b.x0 + 4 * x4 >= 0 && b.x0 + 4 * x4 + 3 < G)) {
+            v = *reinterpret_cast<const float4*>(elem32(pot, __umul24(__umul24((unsigned)gz, g1) + (unsigned)gy, g1) +
+                                                             (unsigned)gx));
+        } else {  // an isolated mesh's edge: clamp each x
+            const unsigned rb = __umul24(__umul24((unsigned)gz, g1) + (unsigned)gy, g1);
+            float e[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) e[q] = *elem32(pot, rb + (unsigned)wrap_or_clamp(b.x0 + 4 * x4 + q, G, 0));
+            v = make_float4(e[0], e[1], e[2], e[3]);
+        }
+        *reinterpret_cast<float4*>(box + __mul24(row, (int)dx) + 4 * x4) = v;
+    }
+    __syncthreads();
+    return b;
+}
+
+// sample_acc_pot's interior form on the LDS box (same loads, same operations).
+__device__ __forceinline__ float3 sample_acc_box(const float* __restrict__ box, const PotBox& b, const Cic& c,
+                                                 float h) {
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const int g1 = b.dx, g2 = b.dxy;
+    const int base = __mul24(c.iz - b.z0, g2) + __mul24(c.iy - b.y0, g1) + (c.ix - b.x0);
+    float4 r[2][2];   // [oz][oy]: x = ix - 1 .. ix + 2
+    float2 fy[2][2];  // [oz][0: y = iy - 1, 1: y = iy + 2]: x = ix, ix + 1
+    float2 fz[2][2];  // [0: z = iz - 1, 1: z = iz + 2][oy]
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const float* q = box + base + oz * g2 + oy * g1 - 1;
+            r[oz][oy] = make_float4(q[0], q[1], q[2], q[3]);
+        }
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz) {
+        const float* q0 = box + base + oz * g2 - g1;
+        const float* q1 = box + base + oz * g2 + 2 * g1;
+        fy[oz][0] = make_float2(q0[0], q0[1]);
+        fy[oz][1] = make_float2(q1[0], q1[1]);
+    }
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy) {
+        const float* q0 = box + base - g2 + oy * g1;
+        const float* q1 = box + base + 2 * g2 + oy * g1;
+        fz[0][oy] = make_float2(q0[0], q0[1]);
+        fz[1][oy] = make_float2(q1[0], q1[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? c.fx : 1.f - c.fx) * (oy ? c.fy : 1.f - c.fy) * (oz ? c.fz : 1.f - c.fz);
+        const float4 R = r[oz][oy];
+        const float xm = ox ? R.y : R.x, xp = ox ? R.w : R.z;
+        const float4 Ro = r[oz][oy ^ 1];
+        const float2 Fy = fy[oz][oy];
+        const float yo = ox ? Ro.z : Ro.y;
+        const float yf = ox ? Fy.y : Fy.x;
+        const float ym = oy ? yo : yf, yp = oy ? yf : yo;
+        const float4 Zo = r[oz ^ 1][oy];
+        const float2 Fz = fz[oz][oy];
+        const float zo = ox ? Zo.z : Zo.y;
+        const float zf = ox ? Fz.y : Fz.x;
+        const float zm = oz ? zo : zf, zp = oz ? zf : zo;
+        const float gx = -(xp - xm) * h, gy = -(yp - ym) * h, gz = -(zp - zm) * h;
+        ax = fmaf(gx, w, ax);
+        ay = fmaf(gy, w, ay);
+        az = fmaf(gz, w, az);
+    }
+    return make_float3(ax, ay, az);
+}
+
 // Block reduction of (sum v, count, max |v|) into one partial per block.
 __device__ __forceinline__ void block_partial(double sx, double sy, double sz, double cnt, float vmax,
                                               ReducePartial* out, int blk) {
@@ -927,7 +1091,7 @@ __device__ __forceinline__ float speed3(float x, float y, float z) {
 // positions feed the block deposit when depositTo is set. GAS adds the gas
 // pressure gather (a.accPress); it is a separate instantiation because its
 // registers would cost the plain step a wave per SIMD (79 against 71 VGPRs).
-template <bool GAS>
+template <bool GAS, bool BOX = false>
 __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) {
     double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
     float vm2 = 0.f;  // the largest squared speed: sqrt is monotone and correctly rounded, so
@@ -953,12 +1117,36 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
     // phase 2: the CIC gathers (8 x KP loads, no stores in between)
     const float fG = (float)a.G;
     float3 ACC[KP];
+    // BOX: the potential box of the block in LDS; the deposit's tile reuses it
+    __shared__ __attribute__((aligned(16))) unsigned long long s_boxmem[BOX ? BOX_CAP / 2 : 2];
+    if constexpr (BOX) {
+        float* box = reinterpret_cast<float*>(s_boxmem);
+        bool live[KP];
+        Cic cc[KP];
 #pragma unroll
-    for (int k = 0; k < KP; ++k) {
-        ACC[k] = make_float3(0.f, 0.f, 0.f);
-        if (P[k].x >= 0.f)
-            ACC[k] = a.pot ? sample_acc_pot(a.pot, cic_setup(P[k], fG), a.G, a.periodic, a.gradH)
-                           : sample_acc(a.accG, cic_setup(P[k], fG), a.G, a.periodic);
+        for (int k = 0; k < KP; ++k) {
+            live[k] = P[k].x >= 0.f;
+            cc[k] = cic_setup(P[k], fG);
+        }
+        const PotBox pb = load_pot_box(live, cc, a.pot, a.G, a.periodic, box);
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            ACC[k] = make_float3(0.f, 0.f, 0.f);
+            if (!live[k]) continue;
+            const Cic& c = cc[k];
+            const bool inner = c.ix >= 1 && c.ix + 2 <= a.G - 1 && c.iy >= 1 && c.iy + 2 <= a.G - 1 && c.iz >= 1 &&
+                               c.iz + 2 <= a.G - 1;
+            ACC[k] = pb.ok && (a.periodic || inner) ? sample_acc_box(box, pb, c, a.gradH)
+                                                    : sample_acc_pot(a.pot, c, a.G, a.periodic, a.gradH);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            ACC[k] = make_float3(0.f, 0.f, 0.f);
+            if (P[k].x >= 0.f)
+                ACC[k] = a.pot ? sample_acc_pot(a.pot, cic_setup(P[k], fG), a.G, a.periodic, a.gradH)
+                               : sample_acc(a.accG, cic_setup(P[k], fG), a.G, a.periodic);
+        }
     }
     const float kick = a.sc->kick, drift = a.sc->dt;
     // phase 3: kick, drift, boundaries, stores
@@ -1091,15 +1279,20 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs& a, int blk) 
             store_pv(a.vel + i, V[k]);
         }
     }
-    if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
+    if constexpr (BOX) {
+        static_assert(BOX_CAP / 2 >= DEP_CAP, "the deposit tile fits the box memory");
+        if (a.depositTo) block_deposit_in(depAlive, depC, a.G, a.periodic, a.depositTo, s_boxmem);
+    } else {
+        if (a.depositTo) block_deposit(depAlive, depC, a.G, a.periodic, a.depositTo);
+    }
     if (a.partials) block_partial(sx, sy, sz, cnt, sqrtf(vm2), a.partials, blk);
 }
 
 // Consecutive particle tiles (Morton-sorted: neighbouring stencils) run on one
 // XCD, so the potential lines their gathers share stay in that XCD's L2.
-template <bool GAS>
+template <bool GAS, bool BOX = false>
 __global__ void __launch_bounds__(PT) k_integrate(IntegrateArgs a) {
-    integrate_body<GAS>(a, SD_XCD_REMAP ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x);
+    integrate_body<GAS, BOX>(a, SD_XCD_REMAP ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x);
 }
 
 // Integrate of many simulations in one launch: block b belongs to job j with
@@ -1120,15 +1313,21 @@ int particle_blocks(int n) { return (n + PT - 1) / PT; }
 void launch_integrate(const IntegrateArgs& a, hipStream_t st, hipEvent_t evStart, hipEvent_t evStop) {
     if (a.n <= 0) return;
     const dim3 grid(integrate_blocks(a.n)), block(PT);
+    // the LDS box gather serves the potential gather of sparse meshes (no gas term)
+    const bool box = SD_BOX_GATHER && a.boxGather && a.pot && !a.accPress;
     if (evStart && evStop) {
         if (a.accPress)
             hipExtLaunchKernelGGL(k_integrate<true>, grid, block, 0, st, evStart, evStop, 0, a);
+        else if (box)
+            hipExtLaunchKernelGGL((k_integrate<false, true>), grid, block, 0, st, evStart, evStop, 0, a);
         else
             hipExtLaunchKernelGGL(k_integrate<false>, grid, block, 0, st, evStart, evStop, 0, a);
         return;
     }
     if (a.accPress)
         hipLaunchKernelGGL(k_integrate<true>, grid, block, 0, st, a);
+    else if (box)
+        hipLaunchKernelGGL((k_integrate<false, true>), grid, block, 0, st, a);
     else
         hipLaunchKernelGGL(k_integrate<false>, grid, block, 0, st, a);
 }
