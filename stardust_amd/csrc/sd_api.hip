@@ -170,7 +170,7 @@ static void free_buffers(sd_sim* s) {
     dfree(s->hist); dfree(s->chunk);
     dfree(s->rhoFix); dfree(s->rhoF); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact); dfree(s->disp); dfree(s->accPress);
-    dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr);
+    dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr); dfree(s->logRho);
     s->cellTableCells = 0;
     s->blockCells = 0;
     if (s->scHost) {
@@ -655,6 +655,7 @@ static bool mlp_stage(sd_mlp* m, int rows) {
 static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     if (!s->mlp) return false;
     if (!s->accCorr && !dalloc(s->accCorr, (size_t)s->allocN)) return false;
+    if (!s->logRho && !dalloc(s->logRho, (size_t)s->G * s->G * s->G)) return false;
     const bool hadRho = s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED;
     if (!hadRho) {
         // rhoFix is zero whenever it is not valid (fixed-point invariant)
@@ -664,11 +665,13 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
             clean_rho(s);
         TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
     }
+    const long long cells = (long long)s->G * s->G * s->G;
+    TIMED(s, "logrho", launch_log_density(s->rhoFix, cells, s->logRho, s->st));
     MlpInferArgs a{};
     a.pos = s->pos;
     a.order = order;
     a.n = s->allocN;
-    a.rho = s->rhoFix;
+    a.lrho = s->logRho;
     a.G = s->G;
     a.periodic = s->periodic;
     const MlpDev& d = s->mlp->d;

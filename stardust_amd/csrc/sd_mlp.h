@@ -36,7 +36,7 @@ struct MlpInferArgs {
     const float4* pos;
     const int32_t* order;  // nullptr, or the sort's permutation: particle i is pos[order[i]]
     int n;
-    const long long* rho;  // fixed-point density of the current positions
+    const uint16_t* lrho;  // log-density grid of the current positions (launch_log_density)
     int G;
     int periodic;
     const __bf16 *W1p, *W2q, *W3q, *W4q;
@@ -52,6 +52,9 @@ void mlp_forward(MlpDev& d, const float* X, int n, hipStream_t st);  // -> d.Y
 void mlp_batch_grad(MlpDev& d, const float* X, const float* T, int n, hipStream_t st);
 void mlp_adam(MlpDev& d, float lr, int t, hipStream_t st);
 size_t mlp_infer_smem();
+// The inference kernel's density features: one 2-byte inference element per
+// node (G^3 of them) from the fixed-point density.
+void launch_log_density(const long long* rho, long long cells, uint16_t* out, hipStream_t st);
 void launch_mlp_infer(const MlpInferArgs& a, int blocks, hipStream_t st);
 void launch_features(const float4* pos, const int32_t* targets, int nt, const long long* rho, int G, int periodic,
                      float* X, hipStream_t st);

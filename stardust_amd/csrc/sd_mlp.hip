@@ -386,7 +386,8 @@ constexpr int WH_LD = KH + 8;
 #define SD_INF_WAVES 16
 #endif
 // Waves per block. The weights take 90 KB of LDS, so one block runs per CU;
-// 16 waves are the 4 per SIMD that 118 registers allow (12: 5% slower).
+// 16 waves are the 4 per SIMD that 124 registers allow (12: 5% slower; two
+// tiles per wave sharing each weight fragment read, in 8 waves: 12% slower).
 constexpr int INF_WAVES = SD_INF_WAVES;
 
 // One layer, Y^T = W X^T: the accumulators start from the layer's biases, so
@@ -434,57 +435,112 @@ __device__ __forceinline__ void mlp_activate(const f32x16* acc, infx8* B) {
     }
 }
 
-// Inference features of one lane's half only: lane half g needs k = 16 s + 8 g + e
-// (s = 0, 1; e = 0..7), so g = 0 computes densities 0-7 and 16-23, g = 1 densities
-// 8-15 and 24-26 plus the three offsets (27-29; 30, 31 are zero padding). Same
-// values as particle_features: neighbour node q = 9 dz + 3 dy + dx around the
-// particle's cell, log(1 + count), count = rho / 2^24.
-__device__ __forceinline__ float log1p_count(const long long* __restrict__ rho, int idx) {
-    // count = rho / 2^24 from the two 32-bit halves (a generic int64 -> float
-    // conversion costs several more instructions); within an ulp of (float)rho / 2^24
-    const long long r = rho[idx];
+// log(1 + count), count = rho / 2^24, as the inference features take it.
+__device__ __forceinline__ float log1p_count(long long r) {
+    // count from the two 32-bit halves (a generic int64 -> float conversion
+    // costs several more instructions); within an ulp of (float)rho / 2^24
     const float c = fmaf((float)(int)(r >> 32), 256.0f, (float)(unsigned int)r * (1.0f / (float)SD_FIX_ONE));
     // log2 by the hardware instruction (its argument is >= 1: no denormal path), times ln 2
     return __builtin_amdgcn_logf(1.0f + c) * 0.69314718055994531f;
 }
 
-__device__ __forceinline__ void infer_feature_frags(float4 p, bool alive, const long long* __restrict__ rho, int G,
-                                                    int periodic, int g, infx8* B) {
+// The density features of every node, once per correction: log1p_count of the
+// node's fixed-point density as an inference element (G^3 x 2 bytes). Each
+// particle reads 27 of these instead of converting 27 densities itself.
+__global__ void __launch_bounds__(256) k_log_density(const long long* __restrict__ rho, long long cells,
+                                                     inf_t* __restrict__ out) {
+    const long long i0 = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i0 + 4 <= cells) {
+        const longlong2 r0 = *reinterpret_cast<const longlong2*>(rho + i0);
+        const longlong2 r1 = *reinterpret_cast<const longlong2*>(rho + i0 + 2);
+        inf_t v[4] = {(inf_t)log1p_count(r0.x), (inf_t)log1p_count(r0.y), (inf_t)log1p_count(r1.x),
+                      (inf_t)log1p_count(r1.y)};
+        *reinterpret_cast<uint2*>(out + i0) = __builtin_bit_cast(uint2, v);
+    } else {
+        for (long long i = i0; i < cells; ++i) out[i] = (inf_t)log1p_count(rho[i]);
+    }
+}
+
+// Input order of the inference kernel. Lane half g holds k = 16 s + 8 g + e
+// (s = 0, 1; e = 0..7) of a particle's inputs, and the order puts the same
+// (dy, dx) neighbour in both halves at a given (s, e) wherever it can, so a
+// lane's node index is one select and one add. With j = 3 dy + dx (dy, dx in
+// 0..2 for offsets -1..1) and density feature 9 dz + j:
+//   s = 0:                 density (dz = 2 g, j = e)
+//   s = 1, e = 0:          density (dz = 2 g, j = 8)
+//   s = 1, g = 0, e >= 1:  density (dz = 1, j = e - 1)
+//   s = 1, g = 1:          e = 1, 2 density (dz = 1, j = e + 6); e = 3..5 the
+//                          offsets (features 27-29); e = 6, 7 padding (30, 31)
+// Returns the feature index (particle_features order) of inference input k.
+__host__ __device__ constexpr int infer_input_feature(int k) {
+    const int s = k >> 4, g = (k >> 3) & 1, e = k & 7;
+    if (s == 0) return 18 * g + e;
+    if (e == 0) return 8 + 18 * g;
+    if (g == 0) return 8 + e;
+    return e <= 2 ? 15 + e : 24 + e;
+}
+
+constexpr bool infer_order_ok() {
+    bool seen[MLP_INP] = {};
+    for (int k = 0; k < MLP_INP; ++k) {
+        const int q = infer_input_feature(k);
+        if (q < 0 || q >= MLP_INP || seen[q]) return false;
+        seen[q] = true;
+    }
+    return infer_input_feature(30) == 30 && infer_input_feature(31) == 31;  // the bias inputs stay in place
+}
+static_assert(MLP_INP == 32 && MLP_IN == 30 && infer_order_ok(), "inference input order must be a permutation");
+
+// wrap_or_clamp for a neighbour of a node: c in [-1, G] (no division)
+__device__ __forceinline__ int near_node(int c, int G, int periodic) {
+    if (periodic) return c < 0 ? c + G : (c >= G ? c - G : c);
+    return min(max(c, 0), G - 1);
+}
+
+__device__ __forceinline__ inf_t ld_lrho(const inf_t* base, unsigned byteOff) {
+    return *reinterpret_cast<const inf_t*>(reinterpret_cast<const char*>(base) + byteOff);
+}
+
+// A lane's 16 inputs (its half g) in the order above, from the log-density
+// grid; the offsets are those of particle_features. Inactive particles (x < 0)
+// get zero offsets and the densities around node 0 (finite; their output is
+// scaled by zero).
+__device__ __forceinline__ void infer_feature_frags(float4 p, const inf_t* __restrict__ lrho, int G, int periodic,
+                                                    int g, infx8* B) {
     const float fG = (float)G;
+    const bool alive = p.x >= 0.f;
     const int cx = min(max((int)floorf(p.x * fG), 0), G - 1);
     const int cy = min(max((int)floorf(p.y * fG), 0), G - 1);
     const int cz = min(max((int)floorf(p.z * fG), 0), G - 1);
-    int xs[3], ys[3], zs[3];
+    // byte offsets of the neighbour planes, rows and columns
+    unsigned xs[3], ys[3], zs[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        xs[d] = wrap_or_clamp(cx + d - 1, G, periodic);
-        ys[d] = wrap_or_clamp(cy + d - 1, G, periodic) * G;
-        zs[d] = wrap_or_clamp(cz + d - 1, G, periodic) * G * G;
+        xs[d] = (unsigned)near_node(cx + d - 1, G, periodic) * (unsigned)sizeof(inf_t);
+        ys[d] = (unsigned)(near_node(cy + d - 1, G, periodic) * G) * (unsigned)sizeof(inf_t);
+        zs[d] = (unsigned)(near_node(cz + d - 1, G, periodic) * G * G) * (unsigned)sizeof(inf_t);
     }
-    const float off0 = 2.0f * (p.x * fG - 0.5f - (float)cx);
-    const float off1 = 2.0f * (p.y * fG - 0.5f - (float)cy);
-    const float off2 = 2.0f * (p.z * fG - 0.5f - (float)cz);
+    unsigned yx[9];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        infx8 v;
+    for (int j = 0; j < 9; ++j) yx[j] = ys[j / 3] + xs[j % 3];
+    const unsigned z02 = g ? zs[2] : zs[0];
+    infx8 v0, v1;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int q0 = 16 * s + e, q1 = q0 + 8;  // feature of half 0 / half 1 (compile time)
-            float val = 0.f;
-            if (q1 < 27) {
-                const int idx = g ? zs[q1 / 9] + ys[(q1 / 3) % 3] + xs[q1 % 3]
-                                  : zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3];
-                val = alive ? log1p_count(rho, idx) : 0.f;
-            } else {
-                const float o = q1 == 27 ? off0 : (q1 == 28 ? off1 : (q1 == 29 ? off2 : (BIAS_MM ? 1.f : 0.f)));
-                float d0 = 0.f;
-                if (alive && !g) d0 = log1p_count(rho, zs[q0 / 9] + ys[(q0 / 3) % 3] + xs[q0 % 3]);
-                val = g ? (alive ? o : 0.f) : d0;
-            }
-            v[e] = (inf_t)val;
-        }
-        B[s] = v;
+    for (int e = 0; e < 8; ++e) v0[e] = ld_lrho(lrho, z02 + yx[e]);
+    v1[0] = ld_lrho(lrho, z02 + yx[8]);
+#pragma unroll
+    for (int e = 1; e < 3; ++e) v1[e] = ld_lrho(lrho, zs[1] + (g ? yx[e + 6] : yx[e - 1]));
+    const float off[3] = {alive ? 2.0f * (p.x * fG - 0.5f - (float)cx) : 0.f,
+                          alive ? 2.0f * (p.y * fG - 0.5f - (float)cy) : 0.f,
+                          alive ? 2.0f * (p.z * fG - 0.5f - (float)cz) : 0.f};
+#pragma unroll
+    for (int e = 3; e < 8; ++e) {
+        const inf_t d = ld_lrho(lrho, zs[1] + yx[e - 1]);  // half 0's density (half 1 loads it and drops it)
+        const inf_t h1 = (inf_t)(e < 6 ? off[e - 3] : (BIAS_MM ? 1.f : 0.f));
+        v1[e] = g ? h1 : d;
     }
+    B[0] = v0;
+    B[1] = v1;
 }
 
 // Eight BF16 weights (as stored) -> eight inference elements, times a power
@@ -520,9 +576,16 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
     inf_t* sW3 = sW2 + MLP_H * WH_LD;
     inf_t* sW4 = sW3 + MLP_H * WH_LD;
     float* sB = reinterpret_cast<float*>(sW4 + MLP_OUTP * WH_LD);  // b1q | b2q | b3q | b4 (padded to 32)
+    const unsigned short* W1s = reinterpret_cast<const unsigned short*>(a.W1p);
     for (int i = threadIdx.x; i < MLP_H * (MLP_INP / 8); i += blockDim.x) {
         const int r = i / (MLP_INP / 8), c = i % (MLP_INP / 8);
-        uint4 v = to_inf(reinterpret_cast<const uint4*>(a.W1p)[i], INF_IN_SCALE);
+        // columns in the inference input order (padding columns 30, 31 stay in place)
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w[k] = (uint32_t)W1s[r * MLP_INP + infer_input_feature(8 * c + 2 * k)] |
+                   (uint32_t)W1s[r * MLP_INP + infer_input_feature(8 * c + 2 * k + 1)] << 16;
+        uint4 v = to_inf(make_uint4(w[0], w[1], w[2], w[3]), INF_IN_SCALE);
         if (BIAS_MM && c == MLP_INP / 8 - 1) v.w = bias_halves(a.b[r] * INF_IN_SCALE);  // columns 30, 31 (zero weights)
         *reinterpret_cast<uint4*>(sW1 + r * W1_LD + 8 * c) = v;
     }
@@ -572,7 +635,7 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
             B[KH / 16 - 1] = one;
         }
         f32x16 acc[4];
-        infer_feature_frags(p, p.x >= 0.f, a.rho, a.G, a.periodic, g, B);
+        infer_feature_frags(p, reinterpret_cast<const inf_t*>(a.lrho), a.G, a.periodic, g, B);
         mlp_layer<MLP_INP, W1_LD, 4, !BIAS_MM>(sW1, B, acc, l, sB);
         mlp_activate(acc, B);
         mlp_layer<KH, WH_LD, 4, !BIAS_MM>(sW2, B, acc, l, sB + MLP_H);
@@ -592,6 +655,12 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(MlpInferArgs a) {
 size_t mlp_infer_smem() {
     return sizeof(inf_t) * (MLP_H * W1_LD + 2 * MLP_H * WH_LD + MLP_OUTP * WH_LD) +
            sizeof(float) * (3 * MLP_H + MLP_OUTP);
+}
+
+void launch_log_density(const long long* rho, long long cells, uint16_t* out, hipStream_t st) {
+    const long long blocks = (cells + 1023) / 1024;
+    hipLaunchKernelGGL(k_log_density, dim3((unsigned)blocks), dim3(256), 0, st, rho, cells,
+                       reinterpret_cast<inf_t*>(out));
 }
 
 void launch_mlp_infer(const MlpInferArgs& a, int blocks, hipStream_t st) {

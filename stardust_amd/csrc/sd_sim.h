@@ -104,6 +104,7 @@ struct sd_sim {
     struct sd_mlp* mlp = nullptr;
     float mlpOutScale = 0.f;
     float4* accCorr = nullptr;
+    uint16_t* logRho = nullptr;  // learned-correction density features (G^3 inference elements)
     int mlpInterval = 1;     // steps between recomputations of the correction
     bool corrValid = false;  // accCorr matches the current particle order
     int corrAge = 0;         // steps since it was computed
