@@ -144,6 +144,38 @@ struct Dft<16, INV, UZ> {
     }
 };
 
+// 12 = 3 x 4 (Cooley-Tukey): n = n1 + 3 n2, k = k2 + 4 k1; four-point
+// transforms over n2, twiddles W12^(n1 k2), three-point transforms over n1.
+// UZ: inputs n >= 6 are zero (only n2 = 0, 1 carry data).
+template <bool INV, bool UZ>
+struct Dft<12, INV, UZ> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a[3][4];
+#pragma unroll
+        for (int n1 = 0; n1 < 3; ++n1) {
+#pragma unroll
+            for (int n2 = 0; n2 < 4; ++n2) a[n1][n2] = v[n1 + 3 * n2];
+            Dft<4, INV, UZ>::run(a[n1]);
+        }
+        // W12^j = (cos(pi j / 6), -+ sin(pi j / 6)) for j = n1 k2 in {1, 2, 3, 4, 6}
+        const float sg = INV ? 1.f : -1.f;
+        const float c = 0.86602540378443864676f;
+        a[1][1] = cmul(a[1][1], make_float2(c, sg * 0.5f));
+        a[1][2] = cmul(a[1][2], make_float2(0.5f, sg * c));
+        a[1][3] = rot90<INV>(a[1][3]);
+        a[2][1] = cmul(a[2][1], make_float2(0.5f, sg * c));
+        a[2][2] = cmul(a[2][2], make_float2(-0.5f, sg * c));
+        a[2][3] = make_float2(-a[2][3].x, -a[2][3].y);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            float2 b[3] = {a[0][k2], a[1][k2], a[2][k2]};
+            Dft<3, INV, false>::run(b);
+#pragma unroll
+            for (int k1 = 0; k1 < 3; ++k1) v[k2 + 4 * k1] = b[k1];
+        }
+    }
+};
+
 // LDS line layout: element i lives at PADI(i) = i + i/16. The first Stockham
 // stage writes with a stride of R complex values; without the pad a radix-16
 // stage would put all 16 lanes of a line in one bank.
@@ -1193,6 +1225,158 @@ __global__ void __launch_bounds__(256) k_cols_reg_batch(const ColArgs* __restric
     cols_reg_body<MODE, HIN, HOUT>(a, tw, blockIdx.x, blockIdx.y);
 }
 
+// ---- register-resident fused z pass at N = 192 -----------------------------------
+// The four-step form with 192 = 16 x 12. Thread t of a column's 16 holds
+// n = t + 16 r (r < 12): the forward transform takes the 12-point transform
+// over r, the twiddles W192^(t k2), one LDS exchange, and 16-point transforms
+// over t; threads u < 12 then hold X[u + 12 k1] (k1 < 16), multiply, and the
+// inverse runs the same steps backwards (16-point transforms over k1,
+// twiddles W192^(-t u), one exchange, 12-point transforms over u), which
+// leaves thread t with n = t + 16 r again, so loads and stores both go
+// straight from registers. The middle steps run on threads u < 12: the
+// fourth wavefront of the block (t = 12..15) waits at the barriers.
+// Isolated meshes only (a periodic mesh has a power-of-two side): the
+// zero-padded inputs (n >= 96, r >= 6) are not loaded, and only outputs
+// n < 96 are stored.
+constexpr int LP192 = 193;  // LDS pitch of one line (float2)
+
+template <bool ZIN>
+__device__ __forceinline__ void fft192_fwd_regs(float2 v[12], float2 y[16], float2* __restrict__ line,
+                                                const float2* __restrict__ s_tw, int t) {
+    Dft<12, false, ZIN>::run(v);
+#pragma unroll
+    for (int k = 1; k < 12; ++k) v[k] = cmul(v[k], s_tw[t * k]);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) line[k * 16 + t] = v[k];
+    __syncthreads();
+    if (t < 12) {  // wavefront-uniform
+#pragma unroll
+        for (int j = 0; j < 16; ++j) y[j] = line[t * 16 + j];
+        Dft<16, false, false>::run(y);
+    }
+}
+
+// y: thread u < 12 holds X[u + 12 j]; returns x[t + 16 r] in v.
+__device__ __forceinline__ void fft192_inv_regs(float2 y[16], float2 v[12], float2* __restrict__ line,
+                                                const float2* __restrict__ s_tw, int t) {
+    __syncthreads();  // earlier readers of the line are done
+    if (t < 12) {
+        Dft<16, true, false>::run(y);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) {
+            float2 w = s_tw[k * t];
+            w.y = -w.y;
+            y[k] = cmul(y[k], w);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) line[k * 12 + t] = y[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 12; ++u) v[u] = line[t * 12 + u];
+    Dft<12, true, false>::run(v);
+}
+
+__device__ __forceinline__ void cols192_z_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
+    constexpr int N = 192, h = N / 2;
+    __shared__ float2 s_x[16 * LP192];
+    __shared__ float2 s_tw[N];
+    if (threadIdx.x < N) s_tw[threadIdx.x] = tw[threadIdx.x];
+    const int c = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int kx0 = bx * 16 + c;
+    const bool okc = kx0 < a.nb;
+    const int kxl = okc ? kx0 : a.nb - 1;  // a partial tile's dead columns load a live one
+    float2 v[12];
+    {
+        const unsigned off = __umul24((unsigned)slab, (unsigned)a.inA) + (unsigned)kxl +
+                             __umul24((unsigned)t, (unsigned)a.inN),
+                       step = (unsigned)a.inN << 4;
+#pragma unroll
+        for (int r = 0; r < 12; ++r) v[r] = r < 6 ? ld_c(a.in, off + r * step) : make_float2(0.f, 0.f);
+        if (!okc) {
+#pragma unroll
+            for (int r = 0; r < 12; ++r) v[r] = make_float2(0.f, 0.f);
+        }
+    }
+    // the multiplier of output kz = t + 12 k1 (threads t < 12)
+    float m[16];
+    const bool col0 = a.pack && kx0 == 0;
+    {
+        const int ky = slab, kyf = ky <= h ? ky : N - ky;
+        const int kxa = col0 && ky > h ? h : kxl;
+        const unsigned P = (unsigned)a.P, gbase = __umul24((unsigned)kyf, P) + (unsigned)kxa,
+                       gplane = __umul24((unsigned)(h + 1), P);
+        const int tu = t < 12 ? t : 0;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const int kz = tu + 12 * k1, kzf = kz <= h ? kz : N - kz;
+            m[k1] = ld_c(a.green, gbase + __umul24((unsigned)kzf, gplane));
+        }
+    }
+    __syncthreads();  // s_tw
+    float2 y[16];
+    float2* line = s_x + c * LP192;
+    fft192_fwd_regs<true>(v, y, line, s_tw, t);
+    // ky = 0 and N/2 of packed column 0: the line is Z = a + i b of two real
+    // z-sequences (kx = 0 and kx = N/2) with multipliers m0 and mN, both even in
+    // kz: W(kz) = Z(kz) (m0 + mN) / 2 + conj Z(N - kz) (m0 - mN) / 2 (real and
+    // imaginary parts apart at kz = 0, N/2)
+    const bool both = a.pack && bx == 0 && (slab == 0 || slab == h);  // block-uniform
+    if (t < 12 && !(both && c == 0)) {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) y[k1] = make_float2(y[k1].x * m[k1], y[k1].y * m[k1]);
+    }
+    if (both) {
+        __syncthreads();
+        if (c == 0 && t < 12) {
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) s_x[t + 12 * k1] = y[k1];
+        }
+        __syncthreads();
+        if (c == 0 && t < 12) {
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                const int kz = t + 12 * k1;
+                const float mz0 = z_mult<COL_Z_ISO>(a, N, 0, slab, kz), mzn = z_mult<COL_Z_ISO>(a, N, h, slab, kz);
+                const float2 o = y[k1];
+                if (kz == 0 || kz == h) {
+                    y[k1] = make_float2(o.x * mz0, o.y * mzn);
+                } else {
+                    const float2 p = s_x[N - kz];
+                    const float sp = 0.5f * (mz0 + mzn), sm = 0.5f * (mz0 - mzn);
+                    y[k1] = make_float2(o.x * sp + p.x * sm, o.y * sp - p.y * sm);
+                }
+            }
+        }
+    }
+    fft192_inv_regs(y, v, line, s_tw, t);
+    if (okc) {
+        const unsigned off = __umul24((unsigned)slab, (unsigned)a.outA) + (unsigned)kx0 +
+                             __umul24((unsigned)t, (unsigned)a.outN),
+                       step = (unsigned)a.outN << 4;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) st_c(a.out, off + r * step, v[r]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cols192_z(ColArgs a, const float2* __restrict__ tw) {
+    if (a.fin.partials) {  // one more row of blocks in front: block (0, 0) reduces, the rest exit
+        if (blockIdx.y == 0) {
+            if (blockIdx.x == 0) finalize_body<256>(a.fin.partials, a.fin.nparts, a.fin.mode, a.fin.pol, a.fin.sc);
+            return;
+        }
+        cols192_z_body(a, tw, blockIdx.x, blockIdx.y - 1);
+        return;
+    }
+    cols192_z_body(a, tw, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_cols192_z_batch(const ColArgs* __restrict__ jobs,
+                                                         const float2* __restrict__ tw) {
+    const ColArgs a = jobs[blockIdx.z];
+    cols192_z_body(a, tw, blockIdx.x, blockIdx.y);
+}
+
 // ---- register-resident row passes at N = 256 -----------------------------------
 // The same four-step transform with lines along x: the 16 threads of a line are
 // 16 consecutive lanes (t = lane % 16), which read one row's consecutive values
@@ -1533,6 +1717,10 @@ static void launch_cols_reg_batch(const ColArgs* jobs, int nin, int nout, dim3 g
         hipLaunchKernelGGL((k_cols_reg_batch<MODE, false, false>), grid, dim3(256), 0, st, jobs, tw);
 }
 
+// The register-resident z pass at N = 192 takes whole lines of the isolated
+// G = 96 mesh: 96 rows in and out.
+static bool cols192_z_fits(int nin, int nout) { return nin == 96 && nout == 96; }
+
 template <int N>
 static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw, hipStream_t st) {
     // a carried reduction (fused z pass, no persistent form) adds a row of blocks
@@ -1553,6 +1741,13 @@ static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw,
             SD_REG_CASE(COL_Z_PER)
 #undef SD_REG_CASE
             default: break;  // the Green build keeps the staged kernel
+        }
+    }
+    if constexpr (N == 192 && SD_COLS_REG) {
+        static_assert(TB == 16, "register-resident tiles are 16 columns wide");
+        if (mode == COL_Z_ISO && cols192_z_fits(a.nin, a.nout)) {
+            hipLaunchKernelGGL(k_cols192_z, grid, dim3(256), 0, st, a, tw);
+            return;
         }
     }
     if constexpr (N >= SD_COLS_STREAM_MIN_N) {
@@ -1843,7 +2038,10 @@ static void group_solve(const MeshGroup& g, bool grad, hipStream_t st) {
         launch_cols_reg_batch<COL_INV>(g.yi, g.S, g.G, dim3(colBlocks, g.G, g.n), g.tw, st);
     } else {
         hipLaunchKernelGGL((k_cols_batch<N, COL_FWD>), dim3(colBlocks, g.G, g.n), blk, 0, st, g.yf, g.tw);
-        if (g.periodic)
+        if (N == 192 && SD_COLS_REG && !g.periodic && cols192_z_fits(g.G, g.G)) {
+            // z lines of G rows in and out (the single-mesh solve's nin / nout)
+            hipLaunchKernelGGL(k_cols192_z_batch, dim3(colBlocks, g.S, g.n), dim3(256), 0, st, g.z, g.tw);
+        } else if (g.periodic)
             hipLaunchKernelGGL((k_cols_batch<N, COL_Z_PER>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);
         else
             hipLaunchKernelGGL((k_cols_batch<N, COL_Z_ISO>), dim3(colBlocks, g.S, g.n), blk, 0, st, g.z, g.tw);

@@ -1,12 +1,13 @@
 """Mesh parity at the FFT sizes the benchmark configurations run.
 
-The transforms are planned per size (radix 3*8*8 at S=192, 16*16 with pruned
-first / last stages at S=256, 8*8*8 with persistent prefetching column passes
-at S=512), so each size is checked on its own:
+The transforms are planned per size (radix 3*8*8 at S=192, with a
+register-resident 16*12 fused z pass; 16*16 with pruned first / last stages at
+S=256; 8*8*8 with persistent prefetching column passes at S=512), so each size
+is checked on its own:
 
 * isolated G=96 (S=192) and G=128 (S=256) against the oracle's FP64 solve
   (``test_gpu_parity.test_potential_and_accel``);
-* isolated G=128 and G=256 (S=512) against a closed form: point masses at
+* isolated G=96, G=128 and G=256 (S=512) against a closed form: point masses at
   cell centres deposit their whole count on one node each, and the isolated
   potential is then exactly sum_k -m_k / (r_k + soft) at every node (no
   periodic images, the zero padding removes them);
@@ -40,7 +41,7 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("G", [128, 256])
+@pytest.mark.parametrize("G", [96, 128, 256])
 def test_isolated_point_masses_closed_form(gpu, oracle, G):
     _say("iso point masses", G)
     soft = 1.5
