@@ -144,6 +144,40 @@ struct Dft<16, INV, UZ> {
     }
 };
 
+// 32 = 2 x 16: even and odd samples, then the W32^k butterflies.
+template <bool INV, bool UZ>
+struct Dft<32, INV, UZ> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 e[16], o[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        Dft<16, INV, UZ>::run(e);
+        Dft<16, INV, UZ>::run(o);
+        // W32^k = (cos(pi k / 16), -+ sin(pi k / 16)), k < 8; k >= 8 from k - 8 times -+ i
+        const float sg = INV ? 1.f : -1.f;
+        const float cs[8] = {1.0f, 0.98078528040323044913f, 0.92387953251128675613f, 0.83146961230254523708f,
+                             0.70710678118654752440f, 0.55557023301960222474f, 0.38268343236508977173f,
+                             0.19509032201612826785f};
+#pragma unroll
+        for (int k = 1; k < 16; ++k) {
+            const int j = k & 7;
+            // W32^j = (cs[j], sg cs[8 - j]) for j < 8 (sin(pi j / 16) = cos(pi (8 - j) / 16))
+            const float2 w = make_float2(cs[j], j == 0 ? 0.f : sg * cs[8 - j]);
+            float2 ok = j == 0 ? o[k] : cmul(o[k], w);
+            if (k >= 8) ok = rot90<INV>(ok);
+            o[k] = ok;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 16] = csub(e[k], o[k]);
+        }
+    }
+};
+
 // 12 = 3 x 4 (Cooley-Tukey): n = n1 + 3 n2, k = k2 + 4 k1; four-point
 // transforms over n2, twiddles W12^(n1 k2), three-point transforms over n1.
 // UZ: inputs n >= 6 are zero (only n2 = 0, 1 carry data).
@@ -1377,6 +1411,156 @@ __global__ void __launch_bounds__(256) k_cols192_z_batch(const ColArgs* __restri
     cols192_z_body(a, tw, blockIdx.x, blockIdx.y);
 }
 
+// ---- register-resident column passes at N = 512 -----------------------------------
+// 512 = 16 x 32. Thread t of a column's 16 holds n = t + 16 r (r < 32): the
+// forward transform takes the 32-point transform over r, the twiddles
+// W512^(t k2), one LDS exchange, and the 16-point transforms over t of
+// k2 = u and u + 16; thread u then holds X[u + 32 k1] and X[u + 16 + 32 k1]
+// (k1 < 16), multiplies, and the inverse runs the same steps backwards
+// (16-point transforms, conjugate twiddles, one exchange, 32-point
+// transforms), which leaves thread t with n = t + 16 r again: loads and stores
+// go straight from registers. HIO: the isolated mesh's zero-padded inputs
+// (n >= 256) are not loaded, and only outputs n < 256 are stored. The S = 512
+// solve has no packed column (nb = 257, the last tile holds kx = 256 alone).
+#ifndef SD_Z512_LINES
+#define SD_Z512_LINES 16
+#endif
+constexpr int Z512_LINES = SD_Z512_LINES;  // columns per block
+constexpr int LP512 = 513;                 // LDS pitch of one line (float2)
+
+template <bool ZIN>
+__device__ __forceinline__ void fft512_fwd_regs(float2 v[32], float2 y0[16], float2 y1[16],
+                                                float2* __restrict__ line, const float2* __restrict__ s_tw, int t) {
+    Dft<32, false, ZIN>::run(v);
+#pragma unroll
+    for (int k = 1; k < 32; ++k) v[k] = cmul(v[k], s_tw[t * k]);
+#pragma unroll
+    for (int k = 0; k < 32; ++k) line[k * 16 + t] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        y0[j] = line[t * 16 + j];
+        y1[j] = line[(t + 16) * 16 + j];
+    }
+    Dft<16, false, false>::run(y0);
+    Dft<16, false, false>::run(y1);
+}
+
+__device__ __forceinline__ void fft512_inv_regs(float2 y0[16], float2 y1[16], float2 v[32],
+                                                float2* __restrict__ line, const float2* __restrict__ s_tw, int t) {
+    Dft<16, true, false>::run(y0);
+    Dft<16, true, false>::run(y1);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        float2 w0 = s_tw[k * t], w1 = s_tw[k * (t + 16)];
+        w0.y = -w0.y;
+        w1.y = -w1.y;
+        y0[k] = cmul(y0[k], w0);
+        y1[k] = cmul(y1[k], w1);
+    }
+    __syncthreads();  // the forward exchange's readers are done
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        line[k * 32 + t] = y0[k];
+        line[k * 32 + t + 16] = y1[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 32; ++u) v[u] = line[t * 32 + u];
+    Dft<32, true, false>::run(v);
+}
+
+// MODE: the fused z pass, or the y forward / inverse pass alone (the forward
+// stores from the middle distribution, the inverse loads into it). HIO: the
+// pruned side of the pass (forward input, inverse output, or both for z).
+template <int MODE, bool HIO>
+__device__ __forceinline__ void cols512_body(const ColArgs& a, const float2* __restrict__ tw, int bx, int slab) {
+    constexpr int N = 512, h = N / 2;
+    constexpr bool ZP = MODE == COL_Z_ISO || MODE == COL_Z_PER;
+    __shared__ float2 s_x[Z512_LINES * LP512];
+    __shared__ float2 s_tw[N];
+    for (int i = threadIdx.x; i < N; i += Z512_LINES * 16) s_tw[i] = tw[i];
+    const int c = threadIdx.x % Z512_LINES, t = threadIdx.x / Z512_LINES;
+    const int kx0 = bx * Z512_LINES + c;
+    const bool okc = kx0 < a.nb;
+    const int kxl = okc ? kx0 : a.nb - 1;  // a partial tile's dead columns load a live one
+    const unsigned inOff = (unsigned)slab * (unsigned)a.inA + (unsigned)kxl, inN = (unsigned)a.inN;
+    float2 v[32];
+    float2 y0[16], y1[16];
+    if constexpr (MODE != COL_INV) {
+        const unsigned off = inOff + (unsigned)t * inN, step = inN << 4;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) v[r] = r < (HIO ? 16 : 32) ? ld_c(a.in, off + r * step) : make_float2(0.f, 0.f);
+    } else {
+        const unsigned off = inOff + (unsigned)t * inN, step = inN << 5;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            y0[k1] = ld_c(a.in, off + k1 * step);
+            y1[k1] = ld_c(a.in, off + 16u * inN + k1 * step);
+        }
+    }
+    if (!okc) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) v[r] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) y0[k1] = y1[k1] = make_float2(0.f, 0.f);
+    }
+    // the z pass's multipliers of outputs kz = t + 32 k1 and t + 16 + 32 k1
+    float m0[16], m1[16];
+    const int ky = slab;
+    if constexpr (MODE == COL_Z_ISO) {
+        const int kyf = ky <= h ? ky : N - ky;
+        const unsigned P = (unsigned)a.P, gbase = (unsigned)kyf * P + (unsigned)kxl, gplane = (unsigned)(h + 1) * P;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const int kz0 = t + 32 * k1, kz1 = kz0 + 16;
+            m0[k1] = ld_c(a.green, gbase + (unsigned)(kz0 <= h ? kz0 : N - kz0) * gplane);
+            m1[k1] = ld_c(a.green, gbase + (unsigned)(kz1 <= h ? kz1 : N - kz1) * gplane);
+        }
+    }
+    __syncthreads();  // s_tw
+    float2* line = s_x + c * LP512;
+    if constexpr (MODE != COL_INV) fft512_fwd_regs<HIO && MODE != COL_INV>(v, y0, y1, line, s_tw, t);
+    if constexpr (MODE == COL_FWD) {
+        if (okc) {
+            const unsigned off = (unsigned)slab * (unsigned)a.outA + (unsigned)kx0 + (unsigned)t * (unsigned)a.outN,
+                           step = (unsigned)a.outN << 5;
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                st_c(a.out, off + k1 * step, y0[k1]);
+                st_c(a.out, off + 16u * (unsigned)a.outN + k1 * step, y1[k1]);
+            }
+        }
+        return;
+    }
+    if constexpr (MODE == COL_Z_PER) {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            m0[k1] = z_mult<COL_Z_PER>(a, N, kxl, ky, t + 32 * k1);
+            m1[k1] = z_mult<COL_Z_PER>(a, N, kxl, ky, t + 16 + 32 * k1);
+        }
+    }
+    if constexpr (ZP) {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            y0[k1] = make_float2(y0[k1].x * m0[k1], y0[k1].y * m0[k1]);
+            y1[k1] = make_float2(y1[k1].x * m1[k1], y1[k1].y * m1[k1]);
+        }
+    }
+    fft512_inv_regs(y0, y1, v, line, s_tw, t);
+    if (okc) {
+        const unsigned off = (unsigned)slab * (unsigned)a.outA + (unsigned)kx0 + (unsigned)t * (unsigned)a.outN,
+                       step = (unsigned)a.outN << 4;
+#pragma unroll
+        for (int r = 0; r < (HIO ? 16 : 32); ++r) st_c(a.out, off + r * step, v[r]);
+    }
+}
+
+template <int MODE, bool HIO>
+__global__ void __launch_bounds__(Z512_LINES * 16) k_cols512_reg(ColArgs a, const float2* __restrict__ tw) {
+    cols512_body<MODE, HIO>(a, tw, blockIdx.x, blockIdx.y);
+}
+
 // ---- register-resident row passes at N = 256 -----------------------------------
 // The same four-step transform with lines along x: the 16 threads of a line are
 // 16 consecutive lanes (t = lane % 16), which read one row's consecutive values
@@ -1748,6 +1932,31 @@ static void launch_cols(const ColArgs& a, int mode, int slabs, const float2* tw,
         if (mode == COL_Z_ISO && cols192_z_fits(a.nin, a.nout)) {
             hipLaunchKernelGGL(k_cols192_z, grid, dim3(256), 0, st, a, tw);
             return;
+        }
+    }
+    if constexpr (N == 512 && SD_COLS_REG) {
+        // register-resident passes: whole lines, no packed column, no carried
+        // reduction; isolated meshes are padded (G = 256), periodic ones full (G = 512)
+        if (!a.pack && !a.fin.partials) {
+            const dim3 g((a.nb + Z512_LINES - 1) / Z512_LINES, slabs), b(Z512_LINES * 16);
+            const bool inH = a.nin == 256, inF = a.nin == 512, outH = a.nout == 256, outF = a.nout == 512;
+            switch (mode) {
+                case COL_Z_ISO:
+                    if (inH && outH) { hipLaunchKernelGGL((k_cols512_reg<COL_Z_ISO, true>), g, b, 0, st, a, tw); return; }
+                    break;
+                case COL_Z_PER:
+                    if (inF && outF) { hipLaunchKernelGGL((k_cols512_reg<COL_Z_PER, false>), g, b, 0, st, a, tw); return; }
+                    break;
+                case COL_FWD:
+                    if (inH && outF) { hipLaunchKernelGGL((k_cols512_reg<COL_FWD, true>), g, b, 0, st, a, tw); return; }
+                    if (inF && outF) { hipLaunchKernelGGL((k_cols512_reg<COL_FWD, false>), g, b, 0, st, a, tw); return; }
+                    break;
+                case COL_INV:
+                    if (inF && outH) { hipLaunchKernelGGL((k_cols512_reg<COL_INV, true>), g, b, 0, st, a, tw); return; }
+                    if (inF && outF) { hipLaunchKernelGGL((k_cols512_reg<COL_INV, false>), g, b, 0, st, a, tw); return; }
+                    break;
+                default: break;
+            }
         }
     }
     if constexpr (N >= SD_COLS_STREAM_MIN_N) {
