@@ -34,6 +34,13 @@ __device__ __forceinline__ int wrap_or_clamp(int c, int G, int periodic) {
     return min(max(c, 0), G - 1);
 }
 
+// The same for c in [-1, G] (a cell index inside the mesh, plus or minus one):
+// no division.
+__device__ __forceinline__ int wrap_or_clamp1(int c, int G, int periodic) {
+    if (periodic) return c < 0 ? c + G : (c >= G ? c - G : c);
+    return min(max(c, 0), G - 1);
+}
+
 __device__ __forceinline__ uint32_t morton_spread10(uint32_t v) {
     v &= 0x3ffu;
     v = (v | (v << 16)) & 0x030000FFu;

@@ -491,12 +491,6 @@ constexpr bool infer_order_ok() {
 }
 static_assert(MLP_INP == 32 && MLP_IN == 30 && infer_order_ok(), "inference input order must be a permutation");
 
-// wrap_or_clamp for a neighbour of a node: c in [-1, G] (no division)
-__device__ __forceinline__ int near_node(int c, int G, int periodic) {
-    if (periodic) return c < 0 ? c + G : (c >= G ? c - G : c);
-    return min(max(c, 0), G - 1);
-}
-
 __device__ __forceinline__ inf_t ld_lrho(const inf_t* base, unsigned byteOff) {
     return *reinterpret_cast<const inf_t*>(reinterpret_cast<const char*>(base) + byteOff);
 }
@@ -516,9 +510,9 @@ __device__ __forceinline__ void infer_feature_frags(float4 p, const inf_t* __res
     unsigned xs[3], ys[3], zs[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        xs[d] = (unsigned)near_node(cx + d - 1, G, periodic) * (unsigned)sizeof(inf_t);
-        ys[d] = (unsigned)(near_node(cy + d - 1, G, periodic) * G) * (unsigned)sizeof(inf_t);
-        zs[d] = (unsigned)(near_node(cz + d - 1, G, periodic) * G * G) * (unsigned)sizeof(inf_t);
+        xs[d] = (unsigned)wrap_or_clamp1(cx + d - 1, G, periodic) * (unsigned)sizeof(inf_t);
+        ys[d] = (unsigned)(wrap_or_clamp1(cy + d - 1, G, periodic) * G) * (unsigned)sizeof(inf_t);
+        zs[d] = (unsigned)(wrap_or_clamp1(cz + d - 1, G, periodic) * G * G) * (unsigned)sizeof(inf_t);
     }
     unsigned yx[9];
 #pragma unroll

@@ -1460,10 +1460,11 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
 }
 
 // ---- contact: kContact, Sim.cu:1030-1203 (contact term) ---------------------------
+// x, y, z in [-1, G] (a neighbour of a cell inside the mesh).
 __device__ __forceinline__ int cell_slot(int x, int y, int z, int G, int periodic, int keyMode) {
-    x = wrap_or_clamp(x, G, periodic);
-    y = wrap_or_clamp(y, G, periodic);
-    z = wrap_or_clamp(z, G, periodic);
+    x = wrap_or_clamp1(x, G, periodic);
+    y = wrap_or_clamp1(y, G, periodic);
+    z = wrap_or_clamp1(z, G, periodic);
     return keyMode ? (int)morton3((uint32_t)x, (uint32_t)y, (uint32_t)z) : (z * G + y) * G + x;
 }
 
@@ -1482,7 +1483,7 @@ __device__ __forceinline__ void axis_gaps(float o, int c, int G, int periodic, f
 
 // Distinct blocks (at most 3) holding the cells c - 1, c, c + 1 along one axis.
 __device__ __forceinline__ int axis_blocks(int c, int G, int periodic, int b[3]) {
-    const int b0 = wrap_or_clamp(c - 1, G, periodic) >> 2, b1 = c >> 2, b2 = wrap_or_clamp(c + 1, G, periodic) >> 2;
+    const int b0 = wrap_or_clamp1(c - 1, G, periodic) >> 2, b1 = c >> 2, b2 = wrap_or_clamp1(c + 1, G, periodic) >> 2;
     int m = 0;
     b[m++] = b0;
     if (b1 != b0) b[m++] = b1;
@@ -1617,21 +1618,39 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
         int bx[3], by[3], bz[3];
         const int mx = axis_blocks(cx, G, periodic, bx), my = axis_blocks(cy, G, periodic, by),
                   mz = axis_blocks(cz, G, periodic, bz);
+        // Morton codes of the blocks from their per-axis parts
+        uint32_t sx[3], sy[3], sz[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            sx[q] = morton_spread10((uint32_t)bx[q]);
+            sy[q] = morton_spread10((uint32_t)by[q]) << 1;
+            sz[q] = morton_spread10((uint32_t)bz[q]) << 2;
+        }
         int total = 0;
         for (int c = 0; c < mz; ++c)
             for (int b = 0; b < my; ++b)
                 for (int a = 0; a < mx; ++a) {
-                    const int2 r = blockRange[morton3((uint32_t)bx[a], (uint32_t)by[b], (uint32_t)bz[c])];
+                    const int2 r = blockRange[sx[a] | sy[b] | sz[c]];
                     total += r.y - r.x;
                 }
         sparse = total <= kMaxPeers + 1;
     }
     if (sparse) {
+        // Morton codes of the 27 cells from the per-axis parts of the three
+        // neighbouring coordinates (wrapped on a periodic mesh; an isolated
+        // mesh takes this path only inside, where no neighbour is clamped)
+        uint32_t px[3], py[3], pz[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            px[q] = morton_spread10((uint32_t)wrap_or_clamp1(cx + q - 1, G, periodic));
+            py[q] = morton_spread10((uint32_t)wrap_or_clamp1(cy + q - 1, G, periodic)) << 1;
+            pz[q] = morton_spread10((uint32_t)wrap_or_clamp1(cz + q - 1, G, periodic)) << 2;
+        }
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
-                    const int c = cell_slot(cx + dx, cy + dy, cz + dz, G, periodic, keyMode);
+                    const int c = (int)(px[dx + 1] | py[dy + 1] | pz[dz + 1]);
                     const int2 br = blockRange[c >> 6];
                     if (br.y <= br.x) continue;
                     const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
