@@ -342,7 +342,8 @@ static bool ensure_contact_buffers(sd_sim* s) {
         const long long blocks = (long long)(cells + 63) / 64;
         if (s->blockCells < blocks) dfree(s->blockRange);
         s->blockCells = blocks;
-        if (!s->blockRange && !dalloc(s->blockRange, (size_t)blocks)) return false;
+        // the block runs, then the blocks' cell-occupancy words
+        if (!s->blockRange && !dalloc(s->blockRange, 2 * (size_t)blocks)) return false;
         return true;
     }
     if (s->cellTableCells < (long long)cells) dfree(s->cellRange);
@@ -605,7 +606,8 @@ static void contact_forces(sd_sim* s, bool keysOnly) {
         int2* blocks = morton ? s->blockRange : nullptr;
         launch_cell_range(s->sb.keys, s->allocN, morton ? nullptr : s->cellRange, blocks, s->blockCells, s->st);
         launch_contact(s->pos, s->vel, keysOnly ? s->sb.idx : nullptr, s->allocN, s->G, s->periodic,
-                       s->modes.keyMode, s->cellRange, s->sb.keys, blocks, s->cfg.contactRadiusCells / (float)s->G,
+                       s->modes.keyMode, s->cellRange, s->sb.keys, blocks, s->blockCells,
+                       s->cfg.contactRadiusCells / (float)s->G,
                        s->cfg.contactStiffness, s->cfg.contactDamping, s->accContact, s->st);
     });
 }

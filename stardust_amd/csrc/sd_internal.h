@@ -107,8 +107,10 @@ void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order
 // since an earlier call keep stale values, so the table is never cleared).
 // With blockRange (Morton keys: cell c = Morton(x, y, z), block c >> 6 = Morton
 // of the 4^3-cell block), blockRange[b] = (first, one past last) sorted index of
-// block b's particles, (0, 0) for an empty block: the table (blockEntries int2)
-// is zeroed here first. Contact with Morton keys needs only the block table.
+// block b's particles, (0, 0) for an empty block; the blockEntries int2 after
+// them hold each block's 64 cell-occupancy bits (uint64, bit c & 63 of block
+// c >> 6). The table (2 * blockEntries int2) is zeroed here first. Contact with
+// Morton keys needs only the block table.
 void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRange, long long blockEntries,
                        hipStream_t st);
 
@@ -202,7 +204,8 @@ void launch_sync_velocity(const float4* accG, const float4* pos, const float4* v
 // particle j in slot order[j]); accOut is in sorted order either way.
 void launch_contact(const float4* pos, const float4* vel, const int32_t* order, int n, int G, int periodic,
                     int keyMode, const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange,
-                    float radius, float stiffness, float damping, float4* accOut, hipStream_t st);
+                    long long blockEntries, float radius, float stiffness, float damping, float4* accOut,
+                    hipStream_t st);
 
 // ---- direct sum -------------------------------------------------------------
 void launch_direct_sum(const float4* pos, int n, const int32_t* targets, int nt, float eps2,

@@ -1584,8 +1584,11 @@ template <bool BLOCKS>
 __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                const int32_t* __restrict__ order, int n, int G, int periodic,
                                                const int2* __restrict__ range, const uint32_t* __restrict__ keys,
-                                               const int2* __restrict__ blockRange, float radius, float stiffness,
-                                               float damping, float4* __restrict__ accOut) {
+                                               const int2* __restrict__ blockRange, long long blockEntries,
+                                               float radius, float stiffness, float damping,
+                                               float4* __restrict__ accOut) {
+    // the blocks' cell-occupancy words follow the block runs (launch_cell_range)
+    const unsigned long long* occ = reinterpret_cast<const unsigned long long*>(blockRange + blockEntries);
     // consecutive (sorted: neighbouring) particles on one XCD share candidates in its L2
     const int i = (SD_XCD_REMAP ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * PT + threadIdx.x;
     if (i >= n) return;
@@ -1651,6 +1654,7 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
                     const int c = (int)(px[dx + 1] | py[dy + 1] | pz[dz + 1]);
+                    if (!((occ[c >> 6] >> (c & 63)) & 1ull)) continue;  // an empty cell: no search
                     const int2 br = blockRange[c >> 6];
                     if (br.y <= br.x) continue;
                     const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
@@ -1693,14 +1697,16 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
 
 void launch_contact(const float4* pos, const float4* vel, const int32_t* order, int n, int G, int periodic,
                     int keyMode, const int2* cellRange, const uint32_t* sortedKeys, const int2* blockRange,
-                    float radius, float stiffness, float damping, float4* accOut, hipStream_t st) {
+                    long long blockEntries, float radius, float stiffness, float damping, float4* accOut,
+                    hipStream_t st) {
     if (n <= 0) return;
     if (keyMode)
         hipLaunchKernelGGL(k_contact<true>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, order, n, G,
-                           periodic, nullptr, sortedKeys, blockRange, radius, stiffness, damping, accOut);
+                           periodic, nullptr, sortedKeys, blockRange, blockEntries, radius, stiffness, damping,
+                           accOut);
     else
         hipLaunchKernelGGL(k_contact<false>, dim3(particle_blocks(n)), dim3(PT), 0, st, pos, vel, order, n, G,
-                           periodic, cellRange, sortedKeys, nullptr, radius, stiffness, damping, accOut);
+                           periodic, cellRange, sortedKeys, nullptr, 0ll, radius, stiffness, damping, accOut);
 }
 
 // ---- direct sum (method of proto/cuda/x3.cu:84-103, 3D, Plummer) -------------------

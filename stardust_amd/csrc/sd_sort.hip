@@ -389,13 +389,31 @@ __global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP
     dstV[i] = srcV[j];
 }
 
+// occ (with blockRange): one bit per cell of each 4^3-cell block (Morton order
+// inside the block), set when the cell holds a particle. A wavefront's keys are
+// sorted, so the lanes of one block are consecutive: an OR scan over them leaves
+// the block's bits in its last lane, which adds them with one atomic.
 __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
-                                                    int2* __restrict__ range, int2* __restrict__ blockRange) {
+                                                    int2* __restrict__ range, int2* __restrict__ blockRange,
+                                                    unsigned long long* __restrict__ occ) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (k == 0u) return;
+    const uint32_t k = i < n ? keys[i] : 0u;
     const int c = (int)(k - 1u);
+    if (occ) {  // every lane of the wavefront takes part in the scan
+        const bool valid = k != 0u;
+        unsigned long long v = valid ? 1ull << (c & 63) : 0ull;
+        const uint32_t bk = valid ? (uint32_t)c >> 6 : 0xFFFFFFFFu;
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long u = __shfl_up(v, o, 64);
+            const uint32_t bo = __shfl_up(bk, o, 64);
+            if (lane >= o && bo == bk) v |= u;
+        }
+        const uint32_t bn = __shfl_down(bk, 1, 64);
+        if (valid && (lane == 63 || bn != bk)) atomicOr(&occ[bk], v);
+    }
+    if (i >= n || k == 0u) return;
     if (range) {
         const bool first = i == 0 || keys[i - 1] != k, last = i == n - 1 || keys[i + 1] != k;
         if (first && last)
@@ -781,9 +799,11 @@ void launch_reorder(const float4* srcP, const float4* srcV, const int32_t* order
 
 void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRange, long long blockEntries,
                        hipStream_t st) {
-    if (blockRange) launch_fill_i32(reinterpret_cast<int32_t*>(blockRange), 2 * blockEntries, 0, st);
+    // blockRange holds 2 * blockEntries int2: the runs, then the cells' occupancy bits
+    unsigned long long* occ = blockRange ? reinterpret_cast<unsigned long long*>(blockRange + blockEntries) : nullptr;
+    if (blockRange) launch_fill_i32(reinterpret_cast<int32_t*>(blockRange), 4 * blockEntries, 0, st);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range, blockRange);
+    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range, blockRange, occ);
 }
 
 }  // namespace sd
