@@ -34,8 +34,8 @@ __device__ __forceinline__ int wrap_or_clamp(int c, int G, int periodic) {
     return min(max(c, 0), G - 1);
 }
 
-// The same for c in [-1, G] (a cell index inside the mesh, plus or minus one):
-// no division.
+// The same for c in [-G, 2G) (for instance a cell index inside the mesh plus
+// or minus a few cells): no division.
 __device__ __forceinline__ int wrap_or_clamp1(int c, int G, int periodic) {
     if (periodic) return c < 0 ? c + G : (c >= G ? c - G : c);
     return min(max(c, 0), G - 1);

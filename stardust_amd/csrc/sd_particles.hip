@@ -600,9 +600,10 @@ __global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __res
         int rx[2], ry[2], rz[2];
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
-            rx[o] = wrap_or_clamp(c.ix + o, G, periodic) - X0;
-            ry[o] = wrap_or_clamp(c.iy + o, G, periodic) - Y0;
-            rz[o] = wrap_or_clamp(c.iz + o, G, periodic) - Z0;
+            // the integrate's positions: a periodic one lies in [0, 1], so ix + o is in [-1, G]
+            rx[o] = wrap_or_clamp1(c.ix + o, G, periodic) - X0;
+            ry[o] = wrap_or_clamp1(c.iy + o, G, periodic) - Y0;
+            rz[o] = wrap_or_clamp1(c.iz + o, G, periodic) - Z0;
         }
         const bool inx0 = (unsigned)rx[0] < GDX, inx1 = (unsigned)rx[1] < GDX;
         const bool iny0 = (unsigned)ry[0] < GDY, iny1 = (unsigned)ry[1] < GDY;
@@ -954,6 +955,8 @@ __device__ __forceinline__ PotBox load_pot_box(const bool alive[KP], const Cic c
     b.z0 = lo[2] - 1;
     const long long dx = x1 - b.x0, dy = (long long)hi[1] - lo[1] + 4, dz = (long long)hi[2] - lo[2] + 4;
     if (dx * dy * dz > BOX_CAP) return b;
+    // box coordinates in [-G, 2G): the wrap below needs no division
+    if (b.x0 < -G || x1 > 2 * G || b.y0 < -G || hi[1] + 3 >= 2 * G || b.z0 < -G || hi[2] + 3 >= 2 * G) return b;
     b.ok = true;
     b.dx = (int)dx;
     b.dxy = (int)(dx * dy);
@@ -963,8 +966,8 @@ __device__ __forceinline__ PotBox load_pot_box(const bool alive[KP], const Cic c
     for (int u = threadIdx.x; u < units; u += PT) {
         const int row = quot_small(u, nx4, inv4), x4 = u - __mul24(row, nx4);
         const int z = quot_small(row, (int)dy, invY), y = row - __mul24(z, (int)dy);
-        const int gx = wrap_or_clamp(b.x0 + 4 * x4, G, periodic), gy = wrap_or_clamp(b.y0 + y, G, periodic),
-                  gz = wrap_or_clamp(b.z0 + z, G, periodic);
+        const int gx = wrap_or_clamp1(b.x0 + 4 * x4, G, periodic), gy = wrap_or_clamp1(b.y0 + y, G, periodic),
+                  gz = wrap_or_clamp1(b.z0 + z, G, periodic);
         float4 v;
         if (periodic || (b.x0 + 4 * x4 >= 0 && b.x0 + 4 * x4 + 3 < G)) {
             v = *reinterpret_cast<const float4*>(elem32(pot, __umul24(__umul24((unsigned)gz, g1) + (unsigned)gy, g1) +
@@ -973,7 +976,7 @@ __device__ __forceinline__ PotBox load_pot_box(const bool alive[KP], const Cic c
             const unsigned rb = __umul24(__umul24((unsigned)gz, g1) + (unsigned)gy, g1);
             float e[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) e[q] = *elem32(pot, rb + (unsigned)wrap_or_clamp(b.x0 + 4 * x4 + q, G, 0));
+            for (int q = 0; q < 4; ++q) e[q] = *elem32(pot, rb + (unsigned)wrap_or_clamp1(b.x0 + 4 * x4 + q, G, 0));
             v = make_float4(e[0], e[1], e[2], e[3]);
         }
         *reinterpret_cast<float4*>(box + __mul24(row, (int)dx) + 4 * x4) = v;
