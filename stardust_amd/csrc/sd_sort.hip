@@ -90,6 +90,25 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     return m;
 }
 
+// Adds one round of digits (one per lane) to the block's LDS counts. Equal
+// digits in consecutive lanes form a run, and each run's first lane adds the
+// run's length: the keys are nearly sorted, so the higher digits come in long
+// runs (and the lowest ones mostly alone, with few same-address atomics).
+// Invalid lanes form runs of their own and add nothing.
+__device__ __forceinline__ void count_digit_runs(uint32_t d, bool valid, uint32_t* cnt) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t dv = valid ? d : 0xFFFFFFFFu;
+    // the previous lane's digit (wave_shr 1); lane 0 gets ~dv, so it heads a run
+    const uint32_t dp = (uint32_t)__builtin_amdgcn_update_dpp((int)~dv, (int)dv, 0x138, 0xF, 0xF, false);
+    const bool head = dv != dp;
+    const uint64_t H = __ballot(head);
+    if (head && valid) {
+        const uint64_t above = lane == 63 ? 0ull : H >> (lane + 1);
+        const int len = above ? __builtin_ctzll(above) + 1 : 64 - lane;
+        atomicAdd(&cnt[d], (uint32_t)len);
+    }
+}
+
 // Element count of a sort: the launch's n, or a count on the device (nDev) when
 // the list was built on the device; launches are then sized for the largest
 // count, and tiles beyond the device count leave at once.
@@ -118,14 +137,7 @@ __device__ __forceinline__ void sort_hist_tile(const uint32_t* __restrict__ keys
     for (int r = 0; r < R; ++r) {
         const int i = base + r * SORT_THREADS + threadIdx.x;
         const bool valid = i < n;
-        const uint32_t d = (kk[r] >> shift) & 255u;
-        const uint64_t active = __ballot(valid);
-        if (valid) {
-            const uint64_t m = match_digit(d, active);
-            const int lane = threadIdx.x & 63;
-            // highest lane of the group adds the group size
-            if ((m >> lane) == 1ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
-        }
+        count_digit_runs((kk[r] >> shift) & 255u, valid, cnt);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
@@ -178,11 +190,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_cell_keys_hist(const float4* _
             idx[i] = i;
             d = k & 255u;
         }
-        const uint64_t active = __ballot(valid);
-        if (valid) {
-            const uint64_t m = match_digit(d, active);
-            if ((m >> (threadIdx.x & 63)) == 1ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
-        }
+        count_digit_runs(d, valid, cnt);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
