@@ -1645,6 +1645,9 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
         // Morton codes of the 27 cells from the per-axis parts of the three
         // neighbouring coordinates (wrapped on a periodic mesh; an isolated
         // mesh takes this path only inside, where no neighbour is clamped)
+        // the particle's own cell holds only itself unless a sorted neighbour shares its key
+        const uint32_t ownKey = keys[i];
+        const bool alone = (i == 0 || keys[i - 1] != ownKey) && (i + 1 >= n || keys[i + 1] != ownKey);
         uint32_t px[3], py[3], pz[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1656,6 +1659,7 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
+                    if (alone && dx == 0 && dy == 0 && dz == 0) continue;  // no candidate but itself
                     const int c = (int)(px[dx + 1] | py[dy + 1] | pz[dz + 1]);
                     if (!((occ[c >> 6] >> (c & 63)) & 1ull)) continue;  // an empty cell: no search
                     const int2 br = blockRange[c >> 6];
