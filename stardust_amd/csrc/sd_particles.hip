@@ -1655,19 +1655,28 @@ __global__ void __launch_bounds__(PT) k_contact(const float4* __restrict__ pos, 
             py[q] = morton_spread10((uint32_t)wrap_or_clamp1(cy + q - 1, G, periodic)) << 1;
             pz[q] = morton_spread10((uint32_t)wrap_or_clamp1(cz + q - 1, G, periodic)) << 2;
         }
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
-                    if (alone && dx == 0 && dy == 0 && dz == 0) continue;  // no candidate but itself
-                    const int c = (int)(px[dx + 1] | py[dy + 1] | pz[dz + 1]);
-                    if (!((occ[c >> 6] >> (c & 63)) & 1ull)) continue;  // an empty cell: no search
-                    const int2 br = blockRange[c >> 6];
-                    if (br.y <= br.x) continue;
-                    const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
-                    if (r.y <= r.x) continue;
-                    contact_cell(pos, vel, order, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
-                }
+        // bit q = 9 (dz + 1) + 3 (dy + 1) + (dx + 1): the cell is in reach and occupied
+        // (the 27 occupancy loads issue together; q ascending is the visiting order)
+        uint32_t visit = 0u;
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+            if (gx[dx + 1] + gy[dy + 1] + gz[dz + 1] > reach2) continue;
+            if (alone && q == 13) continue;  // the own cell holds no candidate but itself
+            const uint32_t c = px[dx + 1] | py[dy + 1] | pz[dz + 1];
+            if ((occ[c >> 6] >> (c & 63u)) & 1ull) visit |= 1u << q;  // an empty cell: no search
+        }
+        while (visit) {
+            const int q = __builtin_ctz(visit);
+            visit &= visit - 1u;
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+            const int c = (int)(px[dx + 1] | py[dy + 1] | pz[dz + 1]);
+            const int2 br = blockRange[c >> 6];
+            if (br.y <= br.x) continue;
+            const int2 r = run_of_key(keys, br, (uint32_t)c + 1u);
+            if (r.y <= r.x) continue;
+            contact_cell(pos, vel, order, i, p, v, r.x, r.y, periodic, d0, d02, stiffness, damping, seen, ax, ay, az);
+        }
     } else {
         for (int dz = -1; dz <= 1 && seen < kMaxPeers; ++dz) {
             int cs[9];
