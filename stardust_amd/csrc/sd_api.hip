@@ -168,7 +168,8 @@ static void free_buffers(sd_sim* s) {
     dfree(s->mb.sIdx); dfree(s->mb.tileMovers); dfree(s->mb.counts);
     s->keysAscending = false;
     dfree(s->hist); dfree(s->chunk);
-    dfree(s->rhoFix); dfree(s->rhoF); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
+    dfree(s->rhoFix); dfree(s->rhoF); dfree(s->rhoX); dfree(s->gfix.fixCount); dfree(s->gfix.tileFlag);
+    dfree(s->gfix.tileList); dfree(s->proj); dfree(s->A); dfree(s->B); dfree(s->green);
     dfree(s->pot); dfree(s->accG); dfree(s->tw); dfree(s->accContact); dfree(s->disp); dfree(s->accPress);
     dfree(s->cellRange); dfree(s->blockRange); dfree(s->partials); dfree(s->sc); dfree(s->accCorr); dfree(s->logRho);
     s->cellTableCells = 0;
@@ -503,7 +504,13 @@ static void deposit_now(sd_sim* s) {
         clean_rho(s);
         TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
         s->rhoValid = true;
+        s->rhoFloatOnly = false;
     }
+}
+
+// rhoFix holds the deposit of the current positions
+static bool rho_fix_ready(const sd_sim* s) {
+    return s->rhoValid && !s->rhoFloatOnly && s->modes.depositMode == SD_DEPOSIT_FIXED;
 }
 
 // X pass for the float deposit: convert rhoF into the int64 staging format is
@@ -539,11 +546,17 @@ static void solve_mesh(sd_sim* s, hipStream_t st, bool grad, bool keepRho = fals
                            s->rhoFix, cells);
     }
     const FinalizeJob* fin = s->finDeferred ? &s->finJob : nullptr;
-    // The x pass clears the density it reads for the atomic deposit of the
-    // next integrate; a gather deposit writes every node, so it needs no clear.
-    const bool keep = keepRho || gather_deposit(s);
-    TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, keep ? 0 : 1, fin, st));
-    s->rhoDirty = keep;
+    if (s->rhoValid && s->rhoFloatOnly) {
+        // a gather deposit's float nodes (rhoFix is left as it is)
+        TIMED_ON(s, st, "fft_x", mesh_rows_fwd_f32(s->plan, s->rhoX, s->A, fin, st));
+    } else {
+        // The x pass clears the density it reads for the atomic deposit of the
+        // next integrate; a gather deposit writes every node, so it needs no clear.
+        const bool keep = keepRho || gather_deposit(s);
+        TIMED_ON(s, st, "fft_x", mesh_rows_fwd(s->plan, s->rhoFix, s->A, keep ? 0 : 1, fin, st));
+        s->rhoDirty = keep;
+    }
+    s->rhoFloatOnly = false;
     TIMED_ON(s, st, "fft_yzy", mesh_solve_rest(s->plan, s->A, s->B, s->green, s->pot, fin, st));
     s->finDeferred = false;
     if (grad) grad_now(s, st);
@@ -555,6 +568,10 @@ static void solve_mesh(sd_sim* s, hipStream_t st, bool grad, bool keepRho = fals
 // Invariant (fixed-point mode): rhoFix is all zero whenever !rhoValid.
 static void ensure_rho(sd_sim* s) {
     if (s->modes.depositMode == SD_DEPOSIT_FLOAT || !s->rhoValid) deposit_now(s);
+}
+// The same, in rhoFix (also when the valid deposit is a gather deposit's float grid)
+static void ensure_rho_fix(sd_sim* s) {
+    if (!rho_fix_ready(s)) deposit_now(s);
 }
 
 static void ensure_mesh(sd_sim* s, bool grad = true) {
@@ -588,6 +605,27 @@ static bool gather_deposit(const sd_sim* s) {
     return SD_GATHER_DEPOSIT && contact_due(s) && s->modes.keyMode == SD_KEY_MORTON &&
            s->modes.pipeline == SD_PIPELINE_FUSED && s->modes.depositMode == SD_DEPOSIT_FIXED && !s->borrowed &&
            gather_deposit_supported(s->G, s->periodic) && (double)s->allocN < SD_GATHER_DENSITY * (double)s->G * s->G * s->G;
+}
+
+// A gather deposit writes float nodes (and the x pass reads 4 bytes a node
+// instead of 8) wherever the x pass is the staged one (S != 256).
+#ifndef SD_GATHER_FLOAT
+#define SD_GATHER_FLOAT 1
+#endif
+static bool gather_float(const sd_sim* s) { return SD_GATHER_FLOAT && gather_deposit(s) && s->plan.S != 256; }
+
+static bool ensure_gather_float_buffers(sd_sim* s) {
+    if (s->rhoX) return true;
+    const int tiles = gather_deposit_tiles(s->G);
+    if (!dalloc(s->gfix.fixCount, 2) || !dalloc(s->gfix.tileFlag, (size_t)tiles) ||
+        !dalloc(s->gfix.tileList, (size_t)tiles) || !dalloc(s->rhoX, (size_t)s->G * s->G * s->G)) {
+        dfree(s->gfix.fixCount); dfree(s->gfix.tileFlag); dfree(s->gfix.tileList); dfree(s->rhoX);
+        return false;
+    }
+    launch_fill_i32(s->gfix.fixCount, 2, 0, s->st);
+    launch_fill_i32(s->gfix.tileFlag, tiles, 0, s->st);
+    s->gfix.parity = 0;
+    return true;
 }
 
 static bool contact_sort(sd_sim* s, bool keysOnly) {
@@ -658,7 +696,7 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     if (!s->mlp) return false;
     if (!s->accCorr && !dalloc(s->accCorr, (size_t)s->allocN)) return false;
     if (!s->logRho && !dalloc(s->logRho, (size_t)s->G * s->G * s->G)) return false;
-    const bool hadRho = s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED;
+    const bool hadRho = rho_fix_ready(s);
     if (!hadRho) {
         // rhoFix is zero whenever it is not valid (fixed-point invariant)
         if (s->modes.depositMode == SD_DEPOSIT_FLOAT)
@@ -685,6 +723,7 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     if (!hadRho) {
         if (s->modes.depositMode == SD_DEPOSIT_FIXED && !s->meshValid) {
             s->rhoValid = true;
+            s->rhoFloatOnly = false;
         } else {
             fill_rho(s);
         }
@@ -843,7 +882,15 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
             std::swap(s->pos, s->posTmp);
             std::swap(s->vel, s->velTmp);
         }
-        if (gatherDep) {
+        bool floatOnly = false;
+        if (gatherDep && gather_float(s) && ensure_gather_float_buffers(s)) {
+            // float nodes into rhoX; far movers add into rhoFix, which must be
+            // all zero (and is again afterwards)
+            clean_rho(s);
+            TIMED(s, "deposit", launch_deposit_gather_f32(s->pos, s->blockRange, s->G, s->periodic, s->rhoX,
+                                                          s->rhoFix, s->gfix, s->st));
+            floatOnly = true;
+        } else if (gatherDep) {
             // the new positions are in the contact sort's order (sb.keysAlt is
             // the sort's free buffer: the far-mover list)
             TIMED(s, "deposit", launch_deposit_gather(s->pos, s->blockRange, s->G, s->periodic, s->rhoFix,
@@ -861,6 +908,7 @@ static void enqueue_step(sd_sim* s, bool timed, bool last) {
         }
         s->primed = true;
         s->rhoValid = fusedDeposit;
+        s->rhoFloatOnly = floatOnly;
     } else {
         // kMaxSpeed + dt policy, then kIntegrate, then momentum removal
         launch_vmax(s->pos, s->vel, s->allocN, s->partials, s->st);
@@ -936,12 +984,13 @@ float sd_light_speed_sq(float lengthScale, float timeUnitScale) {
 }
 
 void sd_prepare_fused(sd_sim* s) {
-    ensure_rho(s);
+    ensure_rho_fix(s);
     if (!s->primed) prime_scalars(s);
 }
 
 void sd_note_fused_steps(sd_sim* s, int steps) {
     s->rhoValid = true;
+    s->rhoFloatOnly = false;
     s->meshValid = false;
     s->primed = true;
     s->keysSorted = false;
@@ -1426,11 +1475,14 @@ __global__ void k_gather_acc(const float4* __restrict__ accG, const float4* __re
 // if the caller must clear rhoFix afterwards (the mesh already consumed it and
 // the fused pipeline expects it empty).
 static bool query_density(sd_sim* s) {
-    if (s->rhoValid && s->modes.depositMode == SD_DEPOSIT_FIXED) return false;
+    if (rho_fix_ready(s)) return false;
     const bool clearAfter = s->meshValid && s->modes.pipeline == SD_PIPELINE_FUSED;
     fill_rho(s);
     TIMED(s, "deposit", launch_deposit_fixed(s->pos, s->allocN, s->G, s->periodic, s->rhoFix, s->st));
-    if (!clearAfter && s->modes.depositMode == SD_DEPOSIT_FIXED) s->rhoValid = true;
+    if (!clearAfter && s->modes.depositMode == SD_DEPOSIT_FIXED) {
+        s->rhoValid = true;
+        s->rhoFloatOnly = false;
+    }
     return clearAfter || s->modes.depositMode != SD_DEPOSIT_FIXED;
 }
 

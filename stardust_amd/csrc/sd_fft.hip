@@ -381,8 +381,11 @@ __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __rest
 //        read when clearSrc != 0 so the next deposit starts from an empty grid).
 // SRC 1: the isolated Green function g = -1/(|d| + soft) of kGreen
 //        (Sim.cu:563-573) generated on the fly for an S^3 grid (rows = S^2).
+// SRC 2: float density rows (srcF, nin values per row, nin a multiple of 4),
+//        each the value SRC 0 makes of its int64 node: (float)v * 2^-24.
 struct RowsFwdArgs {
     long long* src;
+    const float* srcF;
     float2* out;
     int rows;      // number of rows
     int nin;       // input values per row (even)
@@ -495,6 +498,34 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs& a, const float2
                 }
                 if (a.clearSrc && row0 + r < a.rows)
                     reinterpret_cast<longlong2*>(a.src + (size_t)(row0 + r) * a.nin)[j] = make_longlong2(0, 0);
+            }
+        }
+    } else if (SRC == 2) {
+        constexpr int ITERS = (TB * (N / 4) + THREADS - 1) / THREADS;  // float quads of the TB rows
+        const int quads = a.nin / 4;
+        float4 buf[ITERS];
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int q = threadIdx.x + it * THREADS;
+            const int r = q / quads, j = q - r * quads;
+            buf[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < TB && row0 + r < a.rows)
+                buf[it] = reinterpret_cast<const float4*>(a.srcF + (size_t)(row0 + r) * a.nin)[j];
+        }
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int q = threadIdx.x + it * THREADS;
+            const int r = q / quads, j = q - r * quads;
+            if (r < TB) {
+                const int c = r / RPL, h = r % RPL;
+                const float e[4] = {buf[it].x, buf[it].y, buf[it].z, buf[it].w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if constexpr (RPL == 1)
+                        s_line[c * LP + PADI(4 * j + k)] = make_float2(e[k], 0.f);
+                    else
+                        s_f[2 * (c * LP + PADI(4 * j + k)) + h] = e[k];
+                }
             }
         }
     } else {
@@ -1812,6 +1843,8 @@ static void launch_rows_fwd(const RowsFwdArgs& a, int src, const float2* tw, hip
     dim3 grid((a.rows + XFwd::ROWS - 1) / XFwd::ROWS + carry), block(XFwd::LINES * Plan<N>::T);
     if (src == 0)
         hipLaunchKernelGGL((k_rows_fwd<N, 0>), grid, block, 0, st, a, tw);
+    else if (src == 2)
+        hipLaunchKernelGGL((k_rows_fwd<N, 2>), grid, block, 0, st, a, tw);
     else
         hipLaunchKernelGGL((k_rows_fwd<N, 1>), grid, block, 0, st, a, tw);
 }
@@ -2030,6 +2063,28 @@ int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc,
     a.clearSrc = clearSrc;
     a.pack = fft_packs(m.S);
 #define CALL(NN) launch_rows_fwd<NN>(a, 0, m.tw, st)
+    SD_DISPATCH_N(m.S, CALL)
+#undef CALL
+    return 0;
+}
+
+// The same x pass from a float density grid (read only). Not at S = 256,
+// whose register x pass reads the int64 grid.
+int mesh_rows_fwd_f32(const MeshPlan& m, const float* rho, float2* A, const FinalizeJob* fin, hipStream_t st) {
+    if (m.S == 256 || m.G % 4) return -1;
+    RowsFwdArgs a{};
+    if (fin) {
+        const int c = finalize_carrier(m.S);
+        if (c == 1) a.fin = *fin;
+        if (c == 0) launch_finalize(fin->partials, fin->nparts, fin->mode, fin->pol, fin->sc, st);
+    }
+    a.srcF = rho;
+    a.out = A;
+    a.rows = m.G * m.G;
+    a.nin = m.G;
+    a.P = m.P;
+    a.pack = fft_packs(m.S);
+#define CALL(NN) launch_rows_fwd<NN>(a, 2, m.tw, st)
     SD_DISPATCH_N(m.S, CALL)
 #undef CALL
     return 0;

@@ -28,6 +28,9 @@ int fft_pitch(int S);
 // the x pass.
 int mesh_rows_fwd(const MeshPlan& m, long long* rhoFix, float2* A, int clearSrc, const FinalizeJob* fin,
                   hipStream_t st);
+// The x pass from a float density grid (the gather deposit's float output);
+// -1 at S = 256, whose register x pass reads only the int64 grid.
+int mesh_rows_fwd_f32(const MeshPlan& m, const float* rho, float2* A, const FinalizeJob* fin, hipStream_t st);
 int mesh_solve_rest(const MeshPlan& m, float2* A, float2* B, const float* green, float* pot,
                     const FinalizeJob* fin, hipStream_t st);
 int mesh_build_green(const MeshPlan& m, float softCells, float2* tmpA, float2* tmpB,
@@ -134,6 +137,20 @@ void launch_pressure(const float* disp, float4* accP, int G, int periodic, float
 bool gather_deposit_supported(int G, int periodic);
 void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int periodic, long long* rho,
                            int32_t* farList, StepScalars* sc, hipStream_t st);
+// The same deposit with float nodes (each the value the x pass makes of the
+// int64 node) into rhoF. Far movers go to farGrid, an int64 grid that must be
+// all zero and is all zero again afterwards, and the tiles they land in are
+// summed again exactly (k_deposit_fix). b: device bookkeeping, zero at
+// allocation; b.parity flips with each call.
+struct GatherFixBuffers {
+    int* fixCount = nullptr;  // two counts, used by alternate calls
+    int* tileFlag = nullptr;  // gather_deposit_tiles(G) flags
+    int* tileList = nullptr;  // gather_deposit_tiles(G) entries
+    int parity = 0;
+};
+int gather_deposit_tiles(int G);
+void launch_deposit_gather_f32(const float4* pos, const int2* blockRange, int G, int periodic, float* rhoF,
+                               long long* farGrid, GatherFixBuffers& b, hipStream_t st);
 void launch_fill_i64(long long* p, long long n, hipStream_t st);
 void launch_fill_f32(float* p, long long n, float v, hipStream_t st);
 void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st);

@@ -23,6 +23,7 @@
 #include "sd_device.h"
 #include "sd_internal.h"
 
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 
@@ -508,16 +509,39 @@ __device__ __forceinline__ int block_offset(int base, int bw, int G, int periodi
     return d;
 }
 
-__global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __restrict__ pos,
-                                                            const int2* __restrict__ blockRange, int G, int periodic,
-                                                            long long* __restrict__ rho, int32_t* __restrict__ farList,
-                                                            int* __restrict__ farCount) {
-    __shared__ unsigned long long s_tile[GDX * GDY * GDZ];
+// Far-mover bookkeeping of a gather deposit. int64 output: the far movers,
+// listed by the tile that owns their block (farList, farCount), are added by
+// k_deposit_far. Float output (tileFlag set): the owning tile adds a far
+// mover's weights to farGrid (an all-zero int64 grid) with global atomics and
+// marks the tiles its eight nodes fall in; a tile marked for the first time is
+// appended to tileList (count fixCount), and k_deposit_fix sums those tiles
+// again with farGrid's nodes added (and cleared).
+struct FarCtl {
+    int32_t* farList;
+    int* farCount;
+    int* tileFlag;  // nullptr: int64 output
+    int* tileList;
+    int* fixCount;
+    long long* farGrid;
+};
+
+__device__ __forceinline__ int gather_tile_id(int x, int y, int z, int G) {
+    return ((z / GDZ) * (G / GDY) + y / GDY) * (G / GDX) + x / GDX;
+}
+
+// The exact int64 weights of the tile at (X0, Y0, Z0) from the particles of
+// the Morton block runs around it, in s_tile (cleared here). A far mover is
+// skipped; with LIST the tile that owns its block lists it (and marks the
+// tiles it lands in, when f.tileFlag is set). Called by every thread of the
+// block; s_tile is complete when it returns.
+template <bool LIST>
+__device__ __forceinline__ void gather_tile(const float4* __restrict__ pos, const int2* __restrict__ blockRange,
+                                            int G, int periodic, int X0, int Y0, int Z0,
+                                            unsigned long long* __restrict__ s_tile, const FarCtl& f) {
     __shared__ int s_first[GRUNS], s_pre[GRUNS + 1];
     __shared__ int s_blk[GRUNS];  // wrapped block coordinates, 10 bits each
     __shared__ unsigned char s_run[GLUT];  // visit index -> run, when the visits fit
     const int tid = threadIdx.x;
-    const int X0 = blockIdx.x * GDX, Y0 = blockIdx.y * GDY, Z0 = blockIdx.z * GDZ;
     const int NB = G / 4;
     for (int t = tid; t < GDX * GDY * GDZ; t += GTHREADS) s_tile[t] = 0ull;
     if (tid < 64) {
@@ -593,8 +617,27 @@ __global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __res
         if (dx < -2 || dx > 4 || dy < -2 || dy > 4 || dz < -2 || dz > 4) {
             // a far mover: listed once, by the tile that owns its block
             const int ri = r % GBX, rj = (r / GBX) % GBY, rk = r / (GBX * GBY);
-            if (ri >= 1 && ri < GBX - 1 && rj >= 1 && rj < GBY - 1 && rk >= 1 && rk < GBZ - 1)
-                farList[atomicAdd(farCount, 1)] = slot;
+            if (LIST && ri >= 1 && ri < GBX - 1 && rj >= 1 && rj < GBY - 1 && rk >= 1 && rk < GBZ - 1) {
+                if (f.tileFlag) {
+                    uint32_t w[8];
+                    fixed_weights(c, w);
+                    add_global8(c, w, G, periodic, f.farGrid);
+                    int xs[2], ys[2], zs[2];
+#pragma unroll
+                    for (int o = 0; o < 2; ++o) {
+                        xs[o] = wrap_or_clamp(c.ix + o, G, periodic);
+                        ys[o] = wrap_or_clamp(c.iy + o, G, periodic);
+                        zs[o] = wrap_or_clamp(c.iz + o, G, periodic);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int t = gather_tile_id(xs[q & 1], ys[(q >> 1) & 1], zs[q >> 2], G);
+                        if (atomicOr(f.tileFlag + t, 1) == 0) f.tileList[atomicAdd(f.fixCount, 1)] = t;
+                    }
+                } else {
+                    f.farList[atomicAdd(f.farCount, 1)] = slot;
+                }
+            }
             continue;
         }
         int rx[2], ry[2], rz[2];
@@ -621,9 +664,30 @@ __global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __res
         }
     }
     __syncthreads();
-    for (int t = tid; t < GDX * GDY * GDZ; t += GTHREADS) {
+}
+
+// The float node value the x pass makes of an int64 node (so a float grid
+// written here reads as the int64 grid would).
+__device__ __forceinline__ float fix_to_float(unsigned long long v) {
+    return (float)(long long)v * (1.0f / (float)SD_FIX_ONE);
+}
+
+// FOUT = false: int64 nodes into rho; FOUT = true: their float values into rhoF.
+template <bool FOUT>
+__global__ void __launch_bounds__(GTHREADS) k_deposit_gather(const float4* __restrict__ pos,
+                                                            const int2* __restrict__ blockRange, int G, int periodic,
+                                                            long long* __restrict__ rho, float* __restrict__ rhoF,
+                                                            FarCtl f) {
+    __shared__ unsigned long long s_tile[GDX * GDY * GDZ];
+    const int X0 = blockIdx.x * GDX, Y0 = blockIdx.y * GDY, Z0 = blockIdx.z * GDZ;
+    gather_tile<true>(pos, blockRange, G, periodic, X0, Y0, Z0, s_tile, f);
+    for (int t = threadIdx.x; t < GDX * GDY * GDZ; t += GTHREADS) {
         const int x = t % GDX, y = (t / GDX) % GDY, z = t / (GDX * GDY);
-        *elem32(rho, cell_idx(X0 + x, Y0 + y, Z0 + z, G)) = (long long)s_tile[t];
+        const unsigned cell = cell_idx(X0 + x, Y0 + y, Z0 + z, G);
+        if (FOUT)
+            *elem32(rhoF, cell) = fix_to_float(s_tile[t]);
+        else
+            *elem32(rho, cell) = (long long)s_tile[t];
     }
 }
 
@@ -644,12 +708,61 @@ __global__ void __launch_bounds__(GTHREADS) k_deposit_far(const float4* __restri
     if (threadIdx.x == 0) *farCount = 0;
 }
 
+// Float output: the tiles the far movers land in, summed again in int64 with
+// farGrid's nodes added (and cleared, so farGrid is all zero again), then
+// stored as floats; each tile's flag is cleared. fixCountNext, the count of
+// the other parity, was consumed by the previous float deposit and is reset
+// here for the next one.
+__global__ void __launch_bounds__(GTHREADS) k_deposit_fix(const float4* __restrict__ pos,
+                                                         const int2* __restrict__ blockRange, int G, int periodic,
+                                                         float* __restrict__ rhoF, long long* __restrict__ farGrid,
+                                                         const int* __restrict__ fixCount, int* __restrict__ tileFlag,
+                                                         const int* __restrict__ tileList, int* __restrict__ fixCountNext) {
+    __shared__ unsigned long long s_tile[GDX * GDY * GDZ];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *fixCountNext = 0;
+    const int nfix = *fixCount;
+    const int ntx = G / GDX, nty = G / GDY;
+    const FarCtl none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int k = blockIdx.x; k < nfix; k += gridDim.x) {
+        const int t = tileList[k];
+        const int X0 = (t % ntx) * GDX, Y0 = ((t / ntx) % nty) * GDY, Z0 = (t / (ntx * nty)) * GDZ;
+        gather_tile<false>(pos, blockRange, G, periodic, X0, Y0, Z0, s_tile, none);
+        for (int e = threadIdx.x; e < GDX * GDY * GDZ; e += GTHREADS) {
+            const int x = e % GDX, y = (e / GDX) % GDY, z = e / (GDX * GDY);
+            const unsigned cell = cell_idx(X0 + x, Y0 + y, Z0 + z, G);
+            long long* g = elem32(farGrid, cell);
+            const unsigned long long v = s_tile[e] + (unsigned long long)*g;
+            *g = 0;
+            *elem32(rhoF, cell) = fix_to_float(v);
+        }
+        if (threadIdx.x == 0) tileFlag[t] = 0;
+        __syncthreads();  // s_tile is cleared for the next tile
+    }
+}
+
 void launch_deposit_gather(const float4* pos, const int2* blockRange, int G, int periodic, long long* rho,
                            int32_t* farList, StepScalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_deposit_gather, dim3(G / GDX, G / GDY, G / GDZ), dim3(GTHREADS), 0, st, pos, blockRange, G,
-                       periodic, rho, farList, &sc->farCount);
+    const FarCtl f{farList, &sc->farCount, nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_deposit_gather<false>, dim3(G / GDX, G / GDY, G / GDZ), dim3(GTHREADS), 0, st, pos,
+                       blockRange, G, periodic, rho, nullptr, f);
     hipLaunchKernelGGL(k_deposit_far, dim3(1), dim3(GTHREADS), 0, st, pos, G, periodic, rho, farList,
                        &sc->farCount);
+}
+
+int gather_deposit_tiles(int G) { return (G / GDX) * (G / GDY) * (G / GDZ); }
+
+void launch_deposit_gather_f32(const float4* pos, const int2* blockRange, int G, int periodic, float* rhoF,
+                               long long* farGrid, GatherFixBuffers& b, hipStream_t st) {
+    int* cnt = b.fixCount + b.parity;
+    int* cntNext = b.fixCount + (b.parity ^ 1);
+    b.parity ^= 1;
+    const FarCtl f{nullptr, nullptr, b.tileFlag, b.tileList, cnt, farGrid};
+    hipLaunchKernelGGL(k_deposit_gather<true>, dim3(G / GDX, G / GDY, G / GDZ), dim3(GTHREADS), 0, st, pos,
+                       blockRange, G, periodic, nullptr, rhoF, f);
+    // a grid that walks the listed tiles (none on most steps: every block leaves at once)
+    const int grid = std::min(gather_deposit_tiles(G), 256);
+    hipLaunchKernelGGL(k_deposit_fix, dim3(grid), dim3(GTHREADS), 0, st, pos, blockRange, G, periodic, rhoF, farGrid,
+                       cnt, b.tileFlag, b.tileList, cntNext);
 }
 
 // ---- gas physics: cooling and pressure (Sim.cu:1212-1415) ---------------------

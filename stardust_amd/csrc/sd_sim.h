@@ -66,6 +66,8 @@ struct sd_sim {
     // mesh
     long long* rhoFix = nullptr;
     float* rhoF = nullptr;   // float density (float deposit mode; the density view)
+    float* rhoX = nullptr;   // float density of a gather deposit with float nodes (the next x pass's input)
+    sd::GatherFixBuffers gfix{};  // its far-mover tile bookkeeping
     float* proj = nullptr;   // G^2 projection of the density along z (the density field view)
     float2 *A = nullptr, *B = nullptr;
     float* green = nullptr;
@@ -95,6 +97,7 @@ struct sd_sim {
     // state
     bool rhoValid = false;   // rhoFix holds the deposit of the current positions
     bool rhoDirty = false;   // rhoFix not zero although !rhoValid (an x pass that left it for a gather deposit)
+    bool rhoFloatOnly = false;  // with rhoValid: the deposit is in rhoX only (rhoFix is zero or stale)
     bool meshValid = false;  // pot matches the current positions
     bool accGValid = false;  // accG holds pot's gradient (with meshValid)
     bool primed = false;     // sc->dt / sc->kick are set for the next integrate

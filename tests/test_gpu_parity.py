@@ -581,15 +581,21 @@ def test_determinism_and_measures(gpu):
     assert np.array_equal(rhos[0], rhos[1])
 
 
-@pytest.mark.parametrize("periodic,G", [(False, 64), (True, 64), (False, 96)])
+@pytest.mark.parametrize("periodic,G", [(False, 64), (True, 64), (False, 96), (False, 128)])
 def test_gather_deposit_bit_exact(gpu, oracle, periodic, G):
     """Sparse contact steps deposit in gather form (node tiles read the
-    particles of the surrounding Morton block runs and store each node once);
-    particles that move more than a cell in one step are added by the
-    separate atomic pass. The grid after each step equals the oracle's
-    fixed-point deposit of the new positions bit for bit, on the first step
-    (sort with reorder) and later ones (keys-only sort, reorder in the
-    integrate)."""
+    particles of the surrounding Morton block runs and store each node once).
+    Where the x pass is the staged one (S != 256: the first three cases) the
+    tiles store float nodes, which the next x pass reads instead of the int64
+    grid; particles that move more than a cell in one step go to an int64
+    side grid, and the tiles they land in are summed again. At S = 256 (the
+    last case) the tiles store int64 nodes and such particles are added by an
+    atomic pass. Two runs step in lockstep: A solves its potential from the
+    step's own deposit, B from the int64 grid of a fresh atomic deposit (its
+    density is read first). The potentials agree bit for bit and both
+    densities equal the oracle's fixed-point deposit of the new positions, on
+    the first step (sort with reorder) and later ones (keys-only sort, reorder
+    in the integrate)."""
     _say("gather deposit", periodic, G)
     n, dt = 30000, 1e-3
     rng = np.random.default_rng(61)
@@ -605,14 +611,23 @@ def test_gather_deposit_bit_exact(gpu, oracle, periodic, G):
                             boundary=sd.BOUNDARY_PERIODIC if periodic else sd.BOUNDARY_ISOLATED,
                             softeningCells=1.0, gravity=0.05, sortInterval=1, contactEnabled=1,
                             contactRadiusCells=0.2, contactStiffness=1e3)
-    s = sd.Sim(cfg, sd.default_modes(integrator=sd.INTEGRATOR_KD, exactGrid=1))
-    s.set_fixed_dt(dt)
-    s.set_particles(pos, vel)
+    sims = [sd.Sim(cfg, sd.default_modes(integrator=sd.INTEGRATOR_KD, exactGrid=1)) for _ in range(2)]
+    for s in sims:
+        s.set_fixed_dt(dt)
+        s.set_particles(pos, vel)
+    a, b = sims
     for step in range(3):
-        s.step()
-        p, _ = s.get_particles()
-        rho = s.get_grid(sd.GRID_DENSITY_FIXED)
+        a.step()
+        b.step()
+        pot_a = a.get_grid(sd.GRID_POTENTIAL)
+        rho_a = a.get_grid(sd.GRID_DENSITY_FIXED)
+        rho_b = b.get_grid(sd.GRID_DENSITY_FIXED)
+        pot_b = b.get_grid(sd.GRID_POTENTIAL)
+        p, _ = a.get_particles()
         ref = oracle.deposit_fixed(p, G, periodic)
-        _say("  step", step, "mass", int(rho.sum()) // sd.FIX_ONE)
-        assert np.array_equal(rho, ref)
-    s.close()
+        _say("  step", step, "mass", int(rho_a.sum()) // sd.FIX_ONE)
+        assert np.array_equal(rho_a, ref)
+        assert np.array_equal(rho_b, ref)
+        assert np.array_equal(pot_a.view(np.uint32), pot_b.view(np.uint32))
+    for s in sims:
+        s.close()
