@@ -43,27 +43,42 @@ namespace sd {
 constexpr int PT = SD_PT;  // threads per particle block
 
 // ---- fills -------------------------------------------------------------------
-__global__ void k_fill_i64(long long* p, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+// A thread stores one 16-byte chunk (the last, partial one element by
+// element); a base that is not 16-byte aligned takes the one-element kernel.
+template <typename T>
+__global__ void k_fill_vec(T* p, long long n, T v) {
+    constexpr int E = 16 / (int)sizeof(T);
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * E;
+    if (i + E <= n) {
+        T t[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) t[e] = v;
+        uint4 u;
+        __builtin_memcpy(&u, t, sizeof(u));
+        *reinterpret_cast<uint4*>(p + i) = u;
+    } else {
+        for (long long j = i; j < n; ++j) p[j] = v;
+    }
 }
-__global__ void k_fill_f32(float* p, long long n, float v) {
+template <typename T>
+__global__ void k_fill_one(T* p, long long n, T v) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_fill_i32(int32_t* p, long long n, int v) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+template <typename T>
+static void launch_fill(T* p, long long n, T v, hipStream_t st) {
+    if (n <= 0) return;
+    if (reinterpret_cast<uintptr_t>(p) % 16 == 0) {
+        constexpr long long E = 16 / (long long)sizeof(T);
+        const long long chunks = (n + E - 1) / E;
+        hipLaunchKernelGGL(k_fill_vec<T>, dim3((unsigned)((chunks + 255) / 256)), dim3(256), 0, st, p, n, v);
+    } else {
+        hipLaunchKernelGGL(k_fill_one<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    }
 }
-void launch_fill_i64(long long* p, long long n, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
-}
-void launch_fill_f32(float* p, long long n, float v, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
-}
-void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
-}
+void launch_fill_i64(long long* p, long long n, hipStream_t st) { launch_fill<long long>(p, n, 0ll, st); }
+void launch_fill_f32(float* p, long long n, float v, hipStream_t st) { launch_fill<float>(p, n, v, st); }
+void launch_fill_i32(int32_t* p, long long n, int v, hipStream_t st) { launch_fill<int32_t>(p, n, v, st); }
 
 // ---- CIC helpers -------------------------------------------------------------
 struct Cic {
