@@ -722,8 +722,13 @@ static bool mlp_correction(sd_sim* s, const int32_t* order = nullptr) {
     TIMED(s, "mlp", launch_mlp_infer(a, cus > 0 ? cus : 256, s->st));
     if (!hadRho) {
         if (s->modes.depositMode == SD_DEPOSIT_FIXED && !s->meshValid) {
+            if (s->rhoValid && s->rhoFloatOnly) {
+                // The mesh still reads the gather deposit's float grid (an
+                // overlapped solve may already have been forked before this
+                // deposit); rhoFix is cleared before it next takes far movers.
+                s->rhoDirty = true;
+            }
             s->rhoValid = true;
-            s->rhoFloatOnly = false;
         } else {
             fill_rho(s);
         }
