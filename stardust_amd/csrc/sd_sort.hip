@@ -401,11 +401,16 @@ __global__ void __launch_bounds__(256) k_reorder(const float4* __restrict__ srcP
 // inside the block), set when the cell holds a particle. A wavefront's keys are
 // sorted, so the lanes of one block are consecutive: an OR scan over them leaves
 // the block's bits in its last lane, which adds them with one atomic.
-__global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
-                                                    int2* __restrict__ range, int2* __restrict__ blockRange,
-                                                    unsigned long long* __restrict__ occ) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t k = i < n ? keys[i] : 0u;
+// Keys per thread (CR_R chunks of 256 consecutive keys per block, each
+// chunk's loads issued before any is used).
+#ifndef SD_CR_R
+#define SD_CR_R 4
+#endif
+constexpr int CR_R = SD_CR_R;
+
+__device__ __forceinline__ void cell_range_one(const uint32_t* __restrict__ keys, int n, int i, uint32_t k,
+                                               int2* __restrict__ range, int2* __restrict__ blockRange,
+                                               unsigned long long* __restrict__ occ) {
     const int c = (int)(k - 1u);
     if (occ) {  // every lane of the wavefront takes part in the scan
         const bool valid = k != 0u;
@@ -443,6 +448,20 @@ __global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__
         else if (blast)
             blockRange[b].y = i + 1;
     }
+}
+
+__global__ void __launch_bounds__(256) k_cell_range(const uint32_t* __restrict__ keys, int n,
+                                                    int2* __restrict__ range, int2* __restrict__ blockRange,
+                                                    unsigned long long* __restrict__ occ) {
+    const int base = blockIdx.x * (256 * CR_R) + (int)threadIdx.x;
+    uint32_t k[CR_R];
+#pragma unroll
+    for (int r = 0; r < CR_R; ++r) {
+        const int i = base + r * 256;
+        k[r] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < CR_R; ++r) cell_range_one(keys, n, base + r * 256, k[r], range, blockRange, occ);
 }
 
 // ---- sort of nearly sorted keys ------------------------------------------------
@@ -811,7 +830,8 @@ void launch_cell_range(const uint32_t* keys, int n, int2* range, int2* blockRang
     unsigned long long* occ = blockRange ? reinterpret_cast<unsigned long long*>(blockRange + blockEntries) : nullptr;
     if (blockRange) launch_fill_i32(reinterpret_cast<int32_t*>(blockRange), 4 * blockEntries, 0, st);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_range, dim3((n + 255) / 256), dim3(256), 0, st, keys, n, range, blockRange, occ);
+    hipLaunchKernelGGL(k_cell_range, dim3((n + 256 * CR_R - 1) / (256 * CR_R)), dim3(256), 0, st, keys, n, range,
+                       blockRange, occ);
 }
 
 }  // namespace sd
